@@ -1,0 +1,144 @@
+/*
+ * onion-gen candidate-key search on MI355X (gfx950): the C ABI.
+ *
+ * This is the drop-in boundary for the reference's device engine.  Each entry
+ * point names the reference interface it replaces (paths relative to the
+ * reference repository root).  All functions are extern "C", take plain
+ * pointers and sizes, throw nothing, and report errors as a negative status
+ * with a message copied into the caller's (err, errcap) buffer.
+ *
+ * Conventions
+ *   - A field element / coordinate is 32 bytes, little-endian, canonical.
+ *   - A point "xyzt" is 128 bytes: X, Y, Z, T of extended coordinates
+ *     (src/curve.rs:43-49), the start point s0*B of a block.
+ *   - Candidate k is the point (s0 + 8k)*B; a survivor's offset is 8k, the
+ *     scalar delta from s0 (src/gpu/mod.rs:113-119, src/batch.rs:74-77).
+ */
+#ifndef ONION_GEN_HIP_H
+#define ONION_GEN_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- devices */
+
+/* Replaces gpu::cuda::look (src/gpu/cuda.rs:35-82) for the HIP/ROCm path:
+ * the number of visible devices, 0 when there is no driver or no device. */
+int og_hip_device_count(void);
+
+/* Replaces the Device record fields (src/gpu/mod.rs:49-70): name, memory
+ * (bytes) and compute units of device `ordinal`. */
+int og_hip_device_info(int ordinal, char *name, size_t namecap, uint64_t *memory,
+                       int *compute_units);
+
+/* ----------------------------------------------------------------- engine */
+
+/* Replaces pub struct Survivor (src/gpu/mod.rs:113-119). */
+typedef struct {
+  uint64_t offset;    /* scalar delta from s0: 8k */
+  uint8_t packed[32]; /* canonical y, little-endian, sign bit clear */
+} og_survivor;
+
+typedef struct og_engine og_engine;
+typedef struct og_filterset og_filterset;
+
+/* Replaces gpu::hit_room (src/gpu/mod.rs:306-312): survivor room per launch. */
+uint32_t og_hit_room(uint64_t batch, uint32_t bitmap_bits);
+
+/* Replaces gpu::Engine::new (src/gpu/mod.rs:346-353) with the arguments the
+ * reference binds: the start point, the chain count (0 = take the shape the
+ * tuning cache recorded, else the default), and the prefix bitmap of
+ * FilterSet::index_words (src/filter.rs:1857-1859) as 64-bit words. */
+int og_hip_engine_create(int ordinal, const uint8_t start_xyzt[128], uint32_t threads,
+                         const uint64_t *bitmap, size_t bitmap_words, uint32_t bitmap_bits,
+                         og_engine **out, char *err, size_t errcap);
+
+/* Shape of an engine.  Zero fields take defaults. */
+typedef struct {
+  uint32_t threads; /* chains (multiple of 256); 0 = default */
+  uint32_t half;    /* pairs per window: one window is 2*half+1 candidates */
+  uint32_t rounds;  /* windows per chain per advance */
+  uint32_t hit_cap; /* survivor room per advance; 0 = og_hit_room */
+} og_engine_opts;
+
+/* The extended constructor: the filter set compiled by og_filters_parse,
+ * which adds to the reference's bitmap-only device path a literal-prefix
+ * compare, an exact table behind the bitmap (src/filter.rs:2645-2701) and the
+ * checksum forms (suffix, contains) evaluated on the device with SHA3-256. */
+int og_hip_engine_create_ex(int ordinal, const uint8_t start_xyzt[128], const og_engine_opts *opts,
+                            const og_filterset *filters, og_engine **out, char *err,
+                            size_t errcap);
+
+/* Replaces Engine::threads / Engine::batch (src/gpu/mod.rs:446-467). */
+uint32_t og_hip_engine_threads(const og_engine *e);
+uint64_t og_hip_engine_batch(const og_engine *e);
+
+/* Replaces Engine::advance (src/gpu/mod.rs:470-476, cuda.rs:282-365): examines
+ * the next batch() candidates and writes the survivors.  Blocking; not
+ * re-entrant.  Overflowing `cap` or the device room is an error, never a
+ * silent truncation (src/gpu/cuda.rs:356-363). */
+int og_hip_engine_advance(og_engine *e, og_survivor *out, uint32_t cap, uint32_t *n, char *err,
+                          size_t errcap);
+
+/* Time, in milliseconds, of the device work of the last advance (HIP events
+ * around the launches on the engine's stream). */
+double og_hip_engine_last_ms(const og_engine *e);
+
+/* Replaces Drop for the engine (src/gpu/cuda.rs:103-122). */
+void og_hip_engine_destroy(og_engine *e);
+
+/* Elementwise GF(2^255-19) arithmetic on the device for parity tests of the
+ * field itself (the device form of src/field.rs:82-255): op 0 mul, 1 sq, 2 add,
+ * 3 sub, 4 invert, 5 canonical.  Inputs are n raw 32-byte little-endian words
+ * (any value below 2^256); outputs are canonical encodings. */
+int og_hip_field_op(int ordinal, int op, const uint8_t *a, const uint8_t *b, uint8_t *out,
+                    size_t n, char *err, size_t errcap);
+
+/* ---------------------------------------------------------------- filters */
+
+/* Replaces FilterSet::parse_all + FilterSet::new / with_index_bits
+ * (src/filter.rs:1930-1950, 1326-1476): newline-separated filters in the
+ * reference's syntax ("abc", "prefix:abc", "suffix:..d", "contains:abcde",
+ * '?' and '[a-d]' classes).  index_bits = 0 chooses the width as
+ * narrowest_acceptable does (src/filter.rs:2279-2299). */
+og_filterset *og_filters_parse(const char *text, uint32_t index_bits, char *err, size_t errcap);
+void og_filters_free(og_filterset *s);
+/* Filters kept after dedupe and absorption. */
+size_t og_filters_count(const og_filterset *s);
+/* Replaces FilterSet::index_words (src/filter.rs:1857-1859): copies the bitmap
+ * into words (when cap suffices) and returns its width in bits, 0 if none. */
+uint32_t og_filters_index_words(const og_filterset *s, uint64_t *words, size_t cap);
+/* Replaces FilterSet::match_key (src/filter.rs:2098-2150) on a packed key
+ * without sign, restricted to literal prefixes: index of the matching filter
+ * or -1. */
+int og_filters_match_key(const og_filterset *s, const uint8_t packed[32]);
+/* Replaces FilterSet::match_whole_address (src/filter.rs:1673-1691): index of
+ * a filter the 35 address bytes match, or -1. */
+int og_filters_match_address(const og_filterset *s, const uint8_t address[35]);
+
+/* -------------------------------------------------------------------- key */
+
+/* src/key.rs:23-31: SHA-512(seed)[0..32], clamped. */
+void og_key_secret_scalar(const uint8_t seed[32], uint8_t sk[32]);
+/* src/key.rs:33-40: both halves, the first clamped. */
+void og_key_expanded_secret(const uint8_t seed[32], uint8_t out[64]);
+/* s0*B of a seed as 128 bytes xyzt (src/batch.rs:165-168 start_point). */
+void og_key_start_point(const uint8_t seed[32], uint8_t xyzt[128]);
+/* scalar*B packed with its sign bit (src/key.rs:55-62 over curve::scalar_base_mult). */
+void og_key_public_from_scalar(const uint8_t scalar[32], uint8_t pk[32]);
+/* src/key.rs:167-177: the 64-byte secret of the candidate at `offset`;
+ * returns 0 when the offset broke clamping (no key). */
+int og_key_expanded_secret_at_offset(const uint8_t seed[32], uint64_t offset, uint8_t out[64]);
+/* src/key.rs:126-132 and :116: pk || checksum || 0x03, and its base32 text. */
+void og_key_address_bytes(const uint8_t pk[32], uint8_t out[35]);
+void og_key_address(const uint8_t pk[32], char out[57]);
+/* src/run.rs:211-220 */
+void og_run_block_seed(const uint8_t root[32], uint64_t index, uint8_t out[32]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,74 @@
+"""Loads the HIP library and declares the C ABI (include/onion_gen_hip.h).
+
+There is no fallback: if ``libonion_gen_hip.so`` is missing, importing the
+engine raises, so nothing can silently run anything but the HIP path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libonion_gen_hip.so")
+
+# Every entry point include/onion_gen_hip.h declares, with (restype, argtypes).
+_c = ctypes
+_u8p = _c.POINTER(_c.c_uint8)
+SIGNATURES = {
+    "og_hip_device_count": (_c.c_int, []),
+    "og_hip_device_info": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_size_t,
+                                      _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_int)]),
+    "og_hit_room": (_c.c_uint32, [_c.c_uint64, _c.c_uint32]),
+    "og_hip_engine_create": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_uint32, _c.c_void_p,
+                                        _c.c_size_t, _c.c_uint32, _c.POINTER(_c.c_void_p),
+                                        _c.c_char_p, _c.c_size_t]),
+    "og_hip_engine_create_ex": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_void_p, _c.c_void_p,
+                                           _c.POINTER(_c.c_void_p), _c.c_char_p, _c.c_size_t]),
+    "og_hip_engine_threads": (_c.c_uint32, [_c.c_void_p]),
+    "og_hip_engine_batch": (_c.c_uint64, [_c.c_void_p]),
+    "og_hip_engine_advance": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_uint32,
+                                         _c.POINTER(_c.c_uint32), _c.c_char_p, _c.c_size_t]),
+    "og_hip_engine_last_ms": (_c.c_double, [_c.c_void_p]),
+    "og_hip_engine_destroy": (None, [_c.c_void_p]),
+    "og_hip_field_op": (_c.c_int, [_c.c_int, _c.c_int, _c.c_char_p, _c.c_char_p, _c.c_char_p,
+                                   _c.c_size_t, _c.c_char_p, _c.c_size_t]),
+    "og_filters_parse": (_c.c_void_p, [_c.c_char_p, _c.c_uint32, _c.c_char_p, _c.c_size_t]),
+    "og_filters_free": (None, [_c.c_void_p]),
+    "og_filters_count": (_c.c_size_t, [_c.c_void_p]),
+    "og_filters_index_words": (_c.c_uint32, [_c.c_void_p, _c.c_void_p, _c.c_size_t]),
+    "og_filters_match_key": (_c.c_int, [_c.c_void_p, _c.c_char_p]),
+    "og_filters_match_address": (_c.c_int, [_c.c_void_p, _c.c_char_p]),
+    "og_key_secret_scalar": (None, [_c.c_char_p, _c.c_char_p]),
+    "og_key_expanded_secret": (None, [_c.c_char_p, _c.c_char_p]),
+    "og_key_start_point": (None, [_c.c_char_p, _c.c_char_p]),
+    "og_key_public_from_scalar": (None, [_c.c_char_p, _c.c_char_p]),
+    "og_key_expanded_secret_at_offset": (_c.c_int, [_c.c_char_p, _c.c_uint64, _c.c_char_p]),
+    "og_key_address_bytes": (None, [_c.c_char_p, _c.c_char_p]),
+    "og_key_address": (None, [_c.c_char_p, _c.c_char_p]),
+    "og_run_block_seed": (None, [_c.c_char_p, _c.c_uint64, _c.c_char_p]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises OSError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(
+                f"{LIB_PATH} is missing: build it with `make -C onion_gen_amd` "
+                "or __graft_entry__.build(); there is no CPU fallback")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+class OgError(RuntimeError):
+    """An error reported through the C ABI's (status, message) pair."""
+
+
+def err_buf():
+    return ctypes.create_string_buffer(512)

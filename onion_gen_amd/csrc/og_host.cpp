@@ -1,0 +1,298 @@
+// Host side of the library: curve constants, the offset table, key derivation
+// (src/key.rs) and the filter-set entry points of the C ABI.
+#include "og_host.hpp"
+
+#include <cstring>
+#include <string>
+
+#include "keccak.hpp"
+#include "sha512.hpp"
+
+struct og_filterset {
+  og::FilterSet set;
+};
+
+namespace og {
+
+namespace {
+
+Fe fe_small(uint32_t v) {
+  Fe r = fe_zero();
+  r.v[0] = v;
+  return r;
+}
+
+void put_err(char *err, size_t cap, const std::string &msg) {
+  if (err && cap) {
+    size_t n = msg.size() < cap - 1 ? msg.size() : cap - 1;
+    std::memcpy(err, msg.data(), n);
+    err[n] = 0;
+  }
+}
+
+// The base point's y is 4/5; x is the even root of (y^2 - 1)/(d y^2 + 1).
+// Both are the published RFC 8032 constants, computed rather than typed.
+Fe sqrt_ratio_even(const Fe &u, const Fe &v) {
+  // x = u v^3 (u v^7)^((p-5)/8), then fix by sqrt(-1) when needed (RFC 8032 5.1.3)
+  Fe v3 = fe_mul(fe_sq(v), v);
+  Fe v7 = fe_mul(fe_sq(v3), v);
+  Fe t = fe_mul(u, v7);
+  // (p-5)/8 = 2^252 - 3: t^(2^252-3) via the chain of t^(2^250-1)
+  Fe z2 = fe_sq(t);
+  Fe z8 = fe_sqn(z2, 2);
+  Fe z9 = fe_mul(t, z8);
+  Fe z11 = fe_mul(z2, z9);
+  Fe z22 = fe_sq(z11);
+  Fe z5 = fe_mul(z9, z22);
+  Fe z10 = fe_mul(fe_sqn(z5, 5), z5);
+  Fe z20 = fe_mul(fe_sqn(z10, 10), z10);
+  Fe z40 = fe_mul(fe_sqn(z20, 20), z20);
+  Fe z50 = fe_mul(fe_sqn(z40, 10), z10);
+  Fe z100 = fe_mul(fe_sqn(z50, 50), z50);
+  Fe z200 = fe_mul(fe_sqn(z100, 100), z100);
+  Fe z250 = fe_mul(fe_sqn(z200, 50), z50);
+  Fe pw = fe_mul(fe_sqn(z250, 2), t);  // t^(2^252 - 3)
+  Fe x = fe_mul(fe_mul(u, v3), pw);
+  Fe vx2 = fe_mul(v, fe_sq(x));
+  if (!fe_is_zero(fe_sub(vx2, u))) {
+    // sqrt(-1) = 2^((p-1)/4)
+    Fe two = fe_small(2);
+    Fe i = two;
+    // 2^((p-1)/4) = 2^(2^253 - 5): square-and-multiply over the exponent bits
+    // 2^253 - 5 = 0b1...1011 (253 bits)
+    Fe acc = fe_one();
+    for (int bit = 252; bit >= 0; --bit) {
+      acc = fe_sq(acc);
+      const bool set = !(bit == 2);  // 2^253 - 5 = all ones except bit 2
+      if (set) acc = fe_mul(acc, two);
+    }
+    i = acc;
+    x = fe_mul(x, i);
+  }
+  uint8_t b[32];
+  fe_to_bytes(b, x);
+  if (b[0] & 1) x = fe_sub(fe_zero(), x);
+  return fe_canon(x);
+}
+
+}  // namespace
+
+HostCurve::HostCurve() {
+  // d = -121665 / 121666 (RFC 8032 5.1)
+  d = fe_canon(fe_mul(fe_sub(fe_zero(), fe_small(121665)), fe_invert(fe_small(121666))));
+  two_d = fe_canon(fe_add(d, d));
+  Fe y = fe_canon(fe_mul(fe_small(4), fe_invert(fe_small(5))));
+  Fe y2 = fe_sq(y);
+  Fe x = sqrt_ratio_even(fe_sub(y2, fe_one()), fe_add(fe_mul(d, y2), fe_one()));
+  base.X = x;
+  base.Y = y;
+  base.Z = fe_one();
+  base.T = fe_canon(fe_mul(x, y));
+}
+
+Cached HostCurve::eight_cached() const {
+  Point p = pt_dbl(pt_dbl(pt_dbl(base)));
+  return pt_to_cached(p, two_d);
+}
+
+std::vector<uint32_t> HostCurve::offset_table(uint32_t half) const {
+  std::vector<uint32_t> out((size_t)half * 24);
+  if (!half) return out;
+  const Cached e8 = eight_cached();
+  std::vector<Point> pts(half);
+  Point q = pt_add(pt_identity(), e8);
+  for (uint32_t m = 0; m < half; ++m) {
+    pts[m] = q;
+    q = pt_add(q, e8);
+  }
+  // one inversion for the whole table (Montgomery's trick)
+  std::vector<Fe> pre(half);
+  Fe acc = fe_one();
+  for (uint32_t m = 0; m < half; ++m) {
+    pre[m] = acc;
+    acc = fe_mul(acc, pts[m].Z);
+  }
+  Fe inv = fe_invert(acc);
+  for (uint32_t m = half; m-- > 0;) {
+    Fe zinv = fe_mul(inv, pre[m]);
+    inv = fe_mul(inv, pts[m].Z);
+    Fe x = fe_canon(fe_mul(pts[m].X, zinv));
+    Fe y = fe_canon(fe_mul(pts[m].Y, zinv));
+    Fe dxy = fe_canon(fe_mul(d, fe_mul(x, y)));
+    for (int l = 0; l < 8; ++l) {
+      out[(size_t)m * 24 + l] = x.v[l];
+      out[(size_t)m * 24 + 8 + l] = y.v[l];
+      out[(size_t)m * 24 + 16 + l] = dxy.v[l];
+    }
+  }
+  return out;
+}
+
+Cached1 HostCurve::multiple_of_eight_cached1(uint64_t m) const {
+  Point p = pt_mul_u64(m, eight_cached());
+  Fe zinv = fe_invert(p.Z);
+  Fe x = fe_mul(p.X, zinv), y = fe_mul(p.Y, zinv);
+  Cached1 c;
+  c.ypx = fe_canon(fe_add(y, x));
+  c.ymx = fe_canon(fe_sub(y, x));
+  c.t2d = fe_canon(fe_mul(two_d, fe_mul(x, y)));
+  return c;
+}
+
+Point HostCurve::scalar_base_mult(const uint8_t scalar[32]) const {
+  const Cached bc = pt_to_cached(base, two_d);
+  Point acc = pt_identity();
+  for (int i = 255; i >= 0; --i) {
+    acc = pt_dbl(acc);
+    if ((scalar[i / 8] >> (i % 8)) & 1) acc = pt_add(acc, bc);
+  }
+  return acc;
+}
+
+void HostCurve::pack(const Point &p, uint8_t out[32]) const {
+  Fe zinv = fe_invert(p.Z);
+  Fe y = fe_mul(p.Y, zinv), x = fe_mul(p.X, zinv);
+  fe_to_bytes(out, y);
+  uint8_t xb[32];
+  fe_to_bytes(xb, x);
+  out[31] ^= (uint8_t)((xb[0] & 1) << 7);
+}
+
+const FilterSet *filterset_of(const og_filterset *s) { return s ? &s->set : nullptr; }
+
+static const HostCurve &curve() {
+  static const HostCurve c;
+  return c;
+}
+
+}  // namespace og
+
+using namespace og;
+
+extern "C" {
+
+og_filterset *og_filters_parse(const char *text, uint32_t index_bits, char *err, size_t errcap) {
+  if (!text) {
+    put_err(err, errcap, "null filter text");
+    return nullptr;
+  }
+  std::vector<std::string> lines;
+  std::string cur;
+  for (const char *p = text; *p; ++p) {
+    if (*p == '\n') {
+      lines.push_back(cur);
+      cur.clear();
+    } else {
+      cur += *p;
+    }
+  }
+  lines.push_back(cur);
+  og_filterset *s = new og_filterset();
+  std::string e = build_filter_set(lines, index_bits, s->set);
+  if (!e.empty() || s->set.filters.empty()) {
+    put_err(err, errcap, e.empty() ? "no filters" : e);
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
+
+void og_filters_free(og_filterset *s) { delete s; }
+
+size_t og_filters_count(const og_filterset *s) { return s ? s->set.filters.size() : 0; }
+
+uint32_t og_filters_index_words(const og_filterset *s, uint64_t *words, size_t cap) {
+  if (!s || !s->set.index_bits) return 0;
+  if (words && cap >= s->set.words.size())
+    std::memcpy(words, s->set.words.data(), s->set.words.size() * 8);
+  return s->set.index_bits;
+}
+
+int og_filters_match_key(const og_filterset *s, const uint8_t packed[32]) {
+  return s ? s->set.match_key(packed) : -1;
+}
+
+int og_filters_match_address(const og_filterset *s, const uint8_t address[35]) {
+  return s ? s->set.match_address(address) : -1;
+}
+
+void og_key_secret_scalar(const uint8_t seed[32], uint8_t sk[32]) {
+  uint8_t d[64];
+  sha512(seed, 32, d);
+  std::memcpy(sk, d, 32);
+  sk[0] &= 248;
+  sk[31] &= 63;
+  sk[31] |= 64;
+}
+
+void og_key_expanded_secret(const uint8_t seed[32], uint8_t out[64]) {
+  sha512(seed, 32, out);
+  out[0] &= 248;
+  out[31] &= 63;
+  out[31] |= 64;
+}
+
+void og_key_start_point(const uint8_t seed[32], uint8_t xyzt[128]) {
+  uint8_t sk[32];
+  og_key_secret_scalar(seed, sk);
+  Point p = curve().scalar_base_mult(sk);
+  fe_to_bytes(xyzt, p.X);
+  fe_to_bytes(xyzt + 32, p.Y);
+  fe_to_bytes(xyzt + 64, p.Z);
+  fe_to_bytes(xyzt + 96, p.T);
+}
+
+void og_key_public_from_scalar(const uint8_t scalar[32], uint8_t pk[32]) {
+  curve().pack(curve().scalar_base_mult(scalar), pk);
+}
+
+int og_key_expanded_secret_at_offset(const uint8_t seed[32], uint64_t offset, uint8_t out[64]) {
+  og_key_expanded_secret(seed, out);
+  unsigned __int128 carry = offset;
+  for (int i = 0; i < 32 && carry; ++i) {
+    carry += out[i];
+    out[i] = (uint8_t)carry;
+    carry >>= 8;
+  }
+  return (out[0] & 248) == out[0] && (((out[31] & 63) | 64) == out[31]);
+}
+
+void og_key_address_bytes(const uint8_t pk[32], uint8_t out[35]) {
+  uint32_t w[8];
+  for (int l = 0; l < 8; ++l)
+    w[l] = (uint32_t)pk[4 * l] | ((uint32_t)pk[4 * l + 1] << 8) | ((uint32_t)pk[4 * l + 2] << 16) |
+           ((uint32_t)pk[4 * l + 3] << 24);
+  const uint32_t chk = onion_checksum(w);
+  std::memcpy(out, pk, 32);
+  out[32] = (uint8_t)chk;
+  out[33] = (uint8_t)(chk >> 8);
+  out[34] = 0x03;
+}
+
+void og_key_address(const uint8_t pk[32], char out[57]) {
+  static const char alpha[] = "abcdefghijklmnopqrstuvwxyz234567";
+  uint8_t b[35];
+  og_key_address_bytes(pk, b);
+  unsigned acc = 0, bits = 0;
+  int o = 0;
+  for (int i = 0; i < 35; ++i) {
+    acc = (acc << 8) | b[i];
+    bits += 8;
+    while (bits >= 5) {
+      bits -= 5;
+      out[o++] = alpha[(acc >> bits) & 31];
+    }
+  }
+  out[o] = 0;
+}
+
+void og_run_block_seed(const uint8_t root[32], uint64_t index, uint8_t out[32]) {
+  uint8_t buf[58], d[64];
+  std::memcpy(buf, "onion-gen block v1", 18);
+  std::memcpy(buf + 18, root, 32);
+  for (int i = 0; i < 8; ++i) buf[50 + i] = (uint8_t)(index >> (8 * i));
+  sha512(buf, sizeof buf, d);
+  std::memcpy(out, d, 32);
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+"""Compiled filter sets (src/filter.rs FilterSet) over the C ABI."""
+import ctypes
+
+from ._lib import OgError, err_buf, lib
+
+
+class FilterSet:
+    """``FilterSet.parse(lines, index_bits=0)``: parse, dedupe, absorb, index.
+
+    Mirrors FilterSet::parse_all / new / with_index_bits (src/filter.rs:1930,
+    1326, 1344), ``index_words`` (:1857), ``match_key`` (:2098) and
+    ``match_whole_address`` (:1673).
+    """
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @classmethod
+    def parse(cls, lines, index_bits=0):
+        if isinstance(lines, str):
+            lines = [lines]
+        text = "\n".join(lines).encode()
+        e = err_buf()
+        h = lib().og_filters_parse(text, index_bits, e, len(e))
+        if not h:
+            raise OgError(e.value.decode())
+        return cls(ctypes.c_void_p(h))
+
+    def __len__(self):
+        return lib().og_filters_count(self.handle)
+
+    def index_words(self):
+        """(list of u64 words, bits) or None when no index was built."""
+        bits = lib().og_filters_index_words(self.handle, None, 0)
+        if not bits:
+            return None
+        n = ((1 << bits) + 63) // 64
+        buf = (ctypes.c_uint64 * n)()
+        lib().og_filters_index_words(self.handle, buf, n)
+        return list(buf), bits
+
+    def match_key(self, packed):
+        return lib().og_filters_match_key(self.handle, bytes(packed))
+
+    def match_address(self, address_bytes):
+        return lib().og_filters_match_address(self.handle, bytes(address_bytes))
+
+    def close(self):
+        if self.handle:
+            lib().og_filters_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,130 @@
+"""The device engine behind the reference's gpu::Engine boundary.
+
+Mirrors src/gpu/mod.rs: ``Survivor`` (:113-119), ``hit_room`` (:306-312),
+``Engine.new`` (:346-353), ``threads``/``batch`` (:446-467) and ``advance``
+(:470-476), over the C ABI of include/onion_gen_hip.h.
+"""
+import ctypes
+from dataclasses import dataclass
+
+from ._lib import OgError, err_buf, lib
+
+
+class _Survivor(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_uint64), ("packed", ctypes.c_uint8 * 32)]
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_uint32), ("half", ctypes.c_uint32),
+                ("rounds", ctypes.c_uint32), ("hit_cap", ctypes.c_uint32)]
+
+
+@dataclass(frozen=True)
+class Survivor:
+    """A candidate the device could not rule out (src/gpu/mod.rs:113-119).
+
+    ``offset`` is the scalar delta from the start scalar (8k for candidate k);
+    ``packed`` is the canonical y, sign bit clear.
+    """
+    offset: int
+    packed: bytes
+
+
+@dataclass(frozen=True)
+class Device:
+    index: int
+    name: str
+    memory: int
+    compute_units: int
+
+
+def device_count():
+    return lib().og_hip_device_count()
+
+
+def look():
+    """Devices the HIP path reaches (src/gpu/mod.rs:165-244 for this API)."""
+    out = []
+    for i in range(device_count()):
+        name = ctypes.create_string_buffer(256)
+        mem = ctypes.c_uint64()
+        cus = ctypes.c_int()
+        if lib().og_hip_device_info(i, name, 256, ctypes.byref(mem), ctypes.byref(cus)) == 0:
+            out.append(Device(i, name.value.decode(), mem.value, cus.value))
+    return out
+
+
+def hit_room(batch, bitmap_bits):
+    return lib().og_hit_room(batch, bitmap_bits)
+
+
+class Engine:
+    """One device, its chains and its buffers.
+
+    ``Engine.new(start, threads, bitmap, bitmap_bits)`` is the reference's
+    constructor; ``Engine.with_filters(start, filters, ...)`` is the extended one
+    that takes a compiled :class:`onion_gen_amd.filters.FilterSet`.
+    """
+
+    def __init__(self, handle):
+        self._h = handle
+        self._out = None
+
+    @classmethod
+    def new(cls, start_xyzt, threads, bitmap, bitmap_bits, device=0):
+        words = (ctypes.c_uint64 * len(bitmap))(*bitmap)
+        h = ctypes.c_void_p()
+        e = err_buf()
+        rc = lib().og_hip_engine_create(device, bytes(start_xyzt), threads, words, len(bitmap),
+                                        bitmap_bits, ctypes.byref(h), e, len(e))
+        if rc != 0:
+            raise OgError(e.value.decode())
+        return cls(h)
+
+    @classmethod
+    def with_filters(cls, start_xyzt, filters, threads=0, half=0, rounds=0, hit_cap=0, device=0):
+        opts = _Opts(threads, half, rounds, hit_cap)
+        h = ctypes.c_void_p()
+        e = err_buf()
+        rc = lib().og_hip_engine_create_ex(device, bytes(start_xyzt), ctypes.byref(opts),
+                                           filters.handle, ctypes.byref(h), e, len(e))
+        if rc != 0:
+            raise OgError(e.value.decode())
+        return cls(h)
+
+    def threads(self):
+        return lib().og_hip_engine_threads(self._h)
+
+    def batch(self):
+        return lib().og_hip_engine_batch(self._h)
+
+    def last_ms(self):
+        return lib().og_hip_engine_last_ms(self._h)
+
+    def advance(self, cap=1 << 18):
+        """Examines the next ``batch()`` candidates; returns the survivors."""
+        if self._out is None or len(self._out) < cap:
+            self._out = (_Survivor * cap)()
+        n = ctypes.c_uint32()
+        e = err_buf()
+        rc = lib().og_hip_engine_advance(self._h, self._out, cap, ctypes.byref(n), e, len(e))
+        if rc != 0:
+            raise OgError(e.value.decode())
+        return [Survivor(self._out[i].offset, bytes(self._out[i].packed)) for i in range(n.value)]
+
+    def close(self):
+        if self._h:
+            lib().og_hip_engine_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

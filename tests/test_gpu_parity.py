@@ -1,0 +1,189 @@
+"""Parity of the HIP path with the CPU oracle, through the C ABI.
+
+Bit-exact: the set of (k, packed key) the device reports must equal the
+oracle's hits for the same start scalar and filters (src/batch.rs,
+src/filter.rs, src/run.rs:1030-1116 semantics).  Every test prints the case
+it runs before touching the device.
+"""
+import ctypes
+import hashlib
+import json
+import os
+import random
+
+import pytest
+
+from onion_gen_amd import _lib, filters, gpu, key, workload
+from tests import ed25519_ref as ref
+from tests.oracle_lib import OracleFilters, digest, load, search
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+P = ref.P
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def clear_sign(pk):
+    return pk[:31] + bytes([pk[31] & 0x7F])
+
+
+def run_engine(start, fs, n, **shape):
+    """All survivors with k < n, sorted (k, packed)."""
+    eng = gpu.Engine.with_filters(start, fs, **shape)
+    got, done = [], 0
+    try:
+        while done < n:
+            for s in eng.advance():
+                assert s.offset % 8 == 0
+                if s.offset // 8 < n:
+                    got.append((s.offset // 8, s.packed))
+            done += eng.batch()
+    finally:
+        eng.close()
+    return sorted(got)
+
+
+def oracle_hits(start, lines, n, width=256):
+    return [(k, clear_sign(pk)) for k, pk, _ in search(start, 0, n, width, OracleFilters(lines))]
+
+
+def field_op(op, a, b):
+    n = len(a)
+    out = ctypes.create_string_buffer(32 * n)
+    e = _lib.err_buf()
+    rc = _lib.lib().og_hip_field_op(0, op, b"".join(x.to_bytes(32, "little") for x in a),
+                                    b"".join(x.to_bytes(32, "little") for x in b), out, n, e, len(e))
+    assert rc == 0, e.value
+    return [int.from_bytes(out.raw[32 * i:32 * i + 32], "little") for i in range(n)]
+
+
+def test_device_is_visible():
+    say("case: device discovery")
+    devs = gpu.look()
+    assert devs and "gfx950" in devs[0].name
+
+
+def test_field_kernels_match_big_integers():
+    say("case: field kernels, 20000 random + edge values")
+    rng = random.Random(5)
+    edges = [0, 1, 2, 19, P - 1, P, P + 1, 2 * P, 2 ** 255 - 1, 2 ** 255, 2 ** 256 - 1,
+             2 ** 256 - 38, 2 ** 256 - 39, 38]
+    a = edges + [rng.getrandbits(256) for _ in range(20000)]
+    b = list(reversed(edges)) + [rng.getrandbits(256) for _ in range(20000)]
+    assert field_op(0, a, b) == [x * y % P for x, y in zip(a, b)]
+    assert field_op(1, a, b) == [x * x % P for x in a]
+    assert field_op(2, a, b) == [(x + y) % P for x, y in zip(a, b)]
+    assert field_op(3, a, b) == [(x - y) % P for x, y in zip(a, b)]
+    assert field_op(5, a, b) == [x % P for x in a]
+    nz = [x for x in a if x % P][:2000]
+    assert field_op(4, nz, nz) == [pow(x, P - 2, P) for x in nz]
+
+
+def test_config1_golden_hits():
+    """BASELINE configuration 1 on the device: 2^24 candidates, the 4-symbol
+    prefix, the committed hit list."""
+    with open(os.path.join(GOLDEN, "config1_hits.json")) as f:
+        g = json.load(f)
+    say(f"case: config1 prefix={g['prefix']} n=2^24")
+    seed = bytes.fromhex(g["seed"])
+    start = key.start_point(seed)
+    got = run_engine(start, filters.FilterSet.parse(g["prefix"]), g["candidates"],
+                     threads=1 << 15, half=256)
+    want = [(h["k"], clear_sign(bytes.fromhex(h["pk"]))) for h in g["hits"]]
+    assert got == want
+    for k, _ in got:
+        pk, addr = key.finish_hit(seed, 8 * k)
+        assert addr.startswith(g["prefix"])
+
+
+@pytest.mark.parametrize("prefix,n,shape", [
+    ("2x", 1 << 18, dict(threads=256, half=64)),
+    ("2xe", 1 << 21, dict(threads=1024, half=128, rounds=2)),
+    ("q", 1 << 15, dict(threads=256, half=32, hit_cap=1 << 12)),
+])
+def test_prefix_hits_match_oracle(prefix, n, shape):
+    say(f"case: prefix {prefix} n={n} shape={shape}")
+    seed = hashlib.sha256(prefix.encode()).digest()
+    start = key.start_point(seed)
+    assert run_engine(start, filters.FilterSet.parse(prefix), n, **shape) == \
+        oracle_hits(start, prefix, n)
+
+
+def test_shape_does_not_change_results():
+    """Chain count, window width and rounds are performance knobs only
+    (src/batch.rs:235-264)."""
+    say("case: shape invariance, prefix 2x, n=2^17")
+    start = key.start_point(bytes(range(32)))
+    fs = filters.FilterSet.parse("2x")
+    base = run_engine(start, fs, 1 << 17, threads=256, half=16)
+    for shape in (dict(threads=512, half=100), dict(threads=256, half=511, rounds=3)):
+        assert run_engine(start, fs, 1 << 17, **shape) == base
+
+
+def test_dictionary_bitmap_and_exact_table():
+    say("case: dictionary 10000 prefixes 5..8, 2^24-bit bitmap, n=2^22")
+    words = workload.dictionary(10000, seed=2)
+    start = key.start_point(workload.chacha_stream(3, 32))
+    n = 1 << 22
+    got = run_engine(start, filters.FilterSet.parse(words, index_bits=24), n,
+                     threads=1 << 13, half=256)
+    want = oracle_hits(start, words, n)
+    assert got == want and len(want) > 0
+
+
+def test_reference_constructor_bitmap_only():
+    """Engine::new with the bitmap alone over-reports but never under-reports."""
+    say("case: bitmap-only engine, 4 prefixes, n=2^20")
+    lines = ["2xe", "abc", "zz7", "q2q"]
+    fs = filters.FilterSet.parse(lines)
+    words, bits = fs.index_words()
+    start = key.start_point(bytes(range(5, 37)))
+    eng = gpu.Engine.new(start, 1024, words, bits)
+    got, done, n = set(), 0, 1 << 20
+    while done < n:
+        got |= {(s.offset // 8, s.packed) for s in eng.advance() if s.offset // 8 < n}
+        done += eng.batch()
+    eng.close()
+    want = set(oracle_hits(start, lines, n))
+    assert want <= got
+    for k, packed in got:
+        slot = int.from_bytes(packed[:8], "big") >> (64 - bits)
+        assert (words[slot >> 6] >> (slot & 63)) & 1
+
+
+@pytest.mark.parametrize("lines,n", [
+    (["suffix:aqd"], 1 << 16),
+    (["contains:2xeq"], 1 << 16),
+    (["suffix:[a-e]id", "contains:zz7", "ab"], 1 << 15),
+])
+def test_checksum_forms_match_oracle(lines, n):
+    say(f"case: checksum forms {lines} n={n}")
+    start = key.start_point(hashlib.sha256(repr(lines).encode()).digest())
+    got = run_engine(start, filters.FilterSet.parse(lines), n, threads=256, half=64)
+    want = oracle_hits(start, lines, n)
+    assert got == want and len(want) > 0
+
+
+def test_overflow_is_an_error():
+    say("case: survivor overflow, prefix 'a', hit_cap 4096")
+    start = key.start_point(bytes(32))
+    eng = gpu.Engine.with_filters(start, filters.FilterSet.parse("a"), threads=1024, half=256,
+                                  hit_cap=4096)
+    with pytest.raises(_lib.OgError, match="survivors"):
+        eng.advance()
+    eng.close()
+
+
+def test_digest_over_2p20_matches_oracle():
+    """A non-vacuous check of every candidate: a 1-symbol prefix hits 1/32 of
+    the keys; their XOR and sum over 2^20 candidates equal the oracle's over the
+    same keys."""
+    say("case: digest, prefix '2', n=2^20")
+    start = key.start_point(workload.config1()["seed"])
+    got = run_engine(start, filters.FilterSet.parse("2"), 1 << 20, threads=1024, half=256,
+                     hit_cap=1 << 16)
+    want = oracle_hits(start, "2", 1 << 20)
+    assert got == want

@@ -1,0 +1,126 @@
+"""CPU tests of the library's host side: exports, key derivation, filter sets.
+
+The product's host code (onion_gen_amd/csrc/og_host.cpp, og_filters.cpp) is a
+second restatement of src/key.rs and src/filter.rs; the oracle is the checker.
+"""
+import hashlib
+import os
+import random
+import re
+
+import pytest
+
+from onion_gen_amd import _lib, filters, gpu, key, workload
+from tests.oracle_lib import OracleFilters, address as o_address, public_key as o_public_key, \
+    start_point as o_start_point
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    with open(os.path.join(ROOT, "include", "onion_gen_hip.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(og_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.lib()
+    names = header_functions()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    if gpu.device_count() > 0:
+        pytest.skip("a device is visible")
+    fs = filters.FilterSet.parse("abc")
+    with pytest.raises(_lib.OgError, match="no HIP device"):
+        gpu.Engine.with_filters(key.start_point(bytes(32)), fs)
+
+
+def test_key_derivation_matches_oracle():
+    rng = random.Random(31)
+    for i in range(30):
+        seed = bytes(rng.getrandbits(8) for _ in range(32))
+        assert key.start_point(seed) == o_start_point(seed), i
+        pk = key.public_from_scalar(key.secret_scalar(seed))
+        assert pk == o_public_key(seed), i
+        assert key.address(pk) == o_address(pk), i
+        assert key.secret_scalar(seed) == key.expanded_secret(seed)[:32]
+    root = bytes(range(32))
+    assert key.block_seed(root, 7) == hashlib.sha512(
+        b"onion-gen block v1" + root + (7).to_bytes(8, "little")).digest()[:32]
+
+
+def test_offset_secret_rederives_the_chain_key():
+    seed = bytes(range(32))
+    for k in (0, 1, 7, 2047, 123456):
+        sk = key.expanded_secret_at_offset(seed, 8 * k)
+        assert sk is not None
+        s0 = int.from_bytes(key.secret_scalar(seed), "little")
+        assert int.from_bytes(sk[:32], "little") == s0 + 8 * k
+        pk, addr = key.finish_hit(seed, 8 * k)
+        assert len(addr) == 56 and addr.endswith("d")
+
+
+def test_hit_room_bounds():
+    # src/gpu/mod.rs:521-553
+    for bits in range(8, 31):
+        for threads in (1 << 12, 1 << 16, 1 << 17):
+            batch = threads * 64
+            assert gpu.hit_room(batch, bits) >= (batch >> bits) * 4
+    assert gpu.hit_room(2 ** 64 - 1, 0) * 40 <= 16 * 1024 * 1024
+    assert gpu.hit_room(1 << 22, 30) >= 1024
+
+
+def test_filter_sets_match_oracle_structure():
+    cases = [
+        ["abc"],
+        ["abc", "abcd", "prefix:xyz", "suffix:ad", "contains:qq[a-c]"],
+        workload.dictionary(1000, seed=3),
+        workload.dictionary(10000, seed=2),
+    ]
+    for lines in cases:
+        mine = filters.FilterSet.parse(lines)
+        theirs = OracleFilters(lines)
+        assert len(mine) == len(theirs)
+        a, b = mine.index_words(), theirs.index()
+        assert (a is None) == (b is None)
+        if a:
+            assert a[1] == b[1] and a[0] == b[0]
+    fixed = filters.FilterSet.parse(workload.dictionary(10000, seed=2), index_bits=24)
+    assert fixed.index_words()[1] == 24
+    assert fixed.index_words()[0] == OracleFilters(workload.dictionary(10000, seed=2)).index(24)[0]
+
+
+def test_filter_matching_matches_oracle():
+    words = workload.dictionary(3000, seed=9)
+    general = ["suffix:[a-z]qd", "contains:a2?c", "prefix:ab?d", "suffix:aid"]
+    mine_p, theirs_p = filters.FilterSet.parse(words), OracleFilters(words)
+    mine_g, theirs_g = filters.FilterSet.parse(general), OracleFilters(general)
+    rng = random.Random(9)
+    for i in range(3000):
+        pk = bytearray(rng.getrandbits(8) for _ in range(32))
+        if i % 2:
+            w = words[rng.randrange(len(words))]
+            v = int("".join(format(workload.ALPHABET.index(c), "05b") for c in w), 2)
+            nb = 5 * len(w)
+            h = int.from_bytes(pk[:8], "big")
+            pk[:8] = ((h & ((1 << (64 - nb)) - 1)) | (v << (64 - nb))).to_bytes(8, "big")
+        pk = bytes(pk)
+        ab = key.address_bytes(pk)
+        assert ab == pk + hashlib.sha3_256(b".onion checksum" + pk + b"\x03").digest()[:2] + b"\x03"
+        clear = pk[:31] + bytes([pk[31] & 0x7F])
+        assert (mine_p.match_key(clear) >= 0) == (theirs_p.match_key(clear) >= 0)
+        assert (mine_p.match_address(ab) >= 0) == (theirs_p.match_address(ab) >= 0)
+        assert (mine_g.match_address(ab) >= 0) == (theirs_g.match_address(ab) >= 0)
+
+
+def test_bad_filters_raise():
+    for bad in ["ab0", "suffix:abc", "suffix:abd", "[", "a]", "[e-b]", "???", "wat:abc", "",
+                "regex:^ab"]:
+        with pytest.raises(_lib.OgError):
+            filters.FilterSet.parse(bad)
