@@ -97,6 +97,17 @@ void og_hip_engine_destroy(og_engine *e);
 int og_hip_field_op(int ordinal, int op, const uint8_t *a, const uint8_t *b, uint8_t *out,
                     size_t n, char *err, size_t errcap);
 
+/* The radix bench (no counterpart in the reference; it decides the field
+ * representation for gfx950): `threads` lanes each run two chains of `iters`
+ * dependent multiplications in radix 8 (8 x 32 saturated limbs, row-wise),
+ * 81 (8 x 32, product scanning on the multiply-add carry flag), 9 (9 x 29
+ * limbs, carry-free columns) or 10 (10 x 25.5 limbs, the shape of
+ * src/field4.rs:568-670).  Writes field
+ * multiplications per second and, when `results` is non-null, the 2*threads
+ * canonical chain ends (64 bytes per lane). */
+int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
+                       double *muls_per_s, uint8_t *results, char *err, size_t errcap);
+
 /* ---------------------------------------------------------------- filters */
 
 /* Replaces FilterSet::parse_all + FilterSet::new / with_index_bits
@@ -118,6 +129,26 @@ int og_filters_match_key(const og_filterset *s, const uint8_t packed[32]);
 /* Replaces FilterSet::match_whole_address (src/filter.rs:1673-1691): index of
  * a filter the 35 address bytes match, or -1. */
 int og_filters_match_address(const og_filterset *s, const uint8_t address[35]);
+
+/* ----------------------------------------------------------------- tuning */
+
+/* Replaces tuning::Shape (src/tuning.rs:27-39), extended with `rounds` and
+ * `radix`; first = pairs per window (half), second = chains (threads). */
+typedef struct {
+  uint32_t first;
+  uint32_t second;
+  double rate;
+  int64_t at;
+  uint32_t rounds;
+  uint32_t radix;
+} og_tuning_shape;
+
+/* Replaces tuning::path (src/tuning.rs:80-91); OG_TUNING_PATH overrides. */
+int og_tuning_path(char *out, size_t cap);
+/* Replaces tuning::get (src/tuning.rs:94-97): 1 when an entry exists. */
+int og_tuning_get(const char *key, og_tuning_shape *out);
+/* Replaces tuning::put (src/tuning.rs:103-126): atomic rewrite by rename. */
+int og_tuning_put(const char *key, const og_tuning_shape *shape);
 
 /* -------------------------------------------------------------------- key */
 

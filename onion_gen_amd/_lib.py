@@ -30,6 +30,9 @@ SIGNATURES = {
     "og_hip_engine_destroy": (None, [_c.c_void_p]),
     "og_hip_field_op": (_c.c_int, [_c.c_int, _c.c_int, _c.c_char_p, _c.c_char_p, _c.c_char_p,
                                    _c.c_size_t, _c.c_char_p, _c.c_size_t]),
+    "og_hip_radix_bench": (_c.c_int, [_c.c_int, _c.c_int, _c.c_uint32, _c.c_uint32,
+                                      _c.POINTER(_c.c_double), _c.c_char_p, _c.c_char_p,
+                                      _c.c_size_t]),
     "og_filters_parse": (_c.c_void_p, [_c.c_char_p, _c.c_uint32, _c.c_char_p, _c.c_size_t]),
     "og_filters_free": (None, [_c.c_void_p]),
     "og_filters_count": (_c.c_size_t, [_c.c_void_p]),
@@ -45,6 +48,19 @@ SIGNATURES = {
     "og_key_address": (None, [_c.c_char_p, _c.c_char_p]),
     "og_run_block_seed": (None, [_c.c_char_p, _c.c_uint64, _c.c_char_p]),
 }
+
+
+
+class TuningShape(_c.Structure):
+    _fields_ = [("first", _c.c_uint32), ("second", _c.c_uint32), ("rate", _c.c_double),
+                ("at", _c.c_int64), ("rounds", _c.c_uint32), ("radix", _c.c_uint32)]
+
+
+SIGNATURES.update({
+    "og_tuning_path": (_c.c_int, [_c.c_char_p, _c.c_size_t]),
+    "og_tuning_get": (_c.c_int, [_c.c_char_p, _c.POINTER(TuningShape)]),
+    "og_tuning_put": (_c.c_int, [_c.c_char_p, _c.POINTER(TuningShape)]),
+})
 
 _lib = None
 

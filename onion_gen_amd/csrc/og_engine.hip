@@ -16,6 +16,8 @@
 #include "../../include/onion_gen_hip.h"
 #include "og_filters.hpp"
 #include "og_kernels.hpp"
+#include "fe10.hpp"
+#include "fe9.hpp"
 #include "og_host.hpp"
 
 using namespace og;
@@ -279,6 +281,65 @@ __global__ __launch_bounds__(256) void og_field_kernel(int op, const uint32_t *a
   }
   r = fe_canon(r);
   for (int l = 0; l < 8; ++l) out[(size_t)i * 8 + l] = r.v[l];
+}
+
+// Radix bench: each lane runs two independent chains of `iters` dependent
+// multiplications x <- x*y in the chosen representation (8: 8 x 32 saturated,
+// 10: 10 x 25.5), then writes the canonical results.  Inputs are a fixed
+// function of the lane (radix_input), so the host can recompute the answer.
+__host__ __device__ inline uint32_t radix_input(uint32_t lane, uint32_t which, uint32_t l) {
+  uint32_t x = lane * 0x9E3779B9u + which * 0x85EBCA6Bu + l * 0xC2B2AE35u + 0x27220A95u;
+  x ^= x >> 15;
+  x *= 0x2C1B3C6Du;
+  x ^= x >> 12;
+  x *= 0x297A2D39u;
+  x ^= x >> 15;
+  return l == 7 ? (x & 0x7fffffffu) : x;
+}
+
+template <int RADIX>
+__global__ __launch_bounds__(256) void og_radix_kernel(uint32_t iters, uint32_t *out, uint32_t n) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  Fe a, b, y;
+  for (int l = 0; l < 8; ++l) {
+    a.v[l] = radix_input(g, 0, l);
+    b.v[l] = radix_input(g, 1, l);
+    y.v[l] = radix_input(g, 2, l);
+  }
+  if (RADIX == 8) {
+    for (uint32_t i = 0; i < iters; ++i) {
+      a = fe_mul(a, y);
+      b = fe_mul(b, y);
+    }
+  } else if (RADIX == 81) {
+    for (uint32_t i = 0; i < iters; ++i) {
+      a = fe_mul_madc(a, y);
+      b = fe_mul_madc(b, y);
+    }
+  } else if (RADIX == 9) {
+    Fe9 a9 = fe9_from(a), b9 = fe9_from(b), y9 = fe9_from(y);
+    for (uint32_t i = 0; i < iters; ++i) {
+      a9 = fe9_mul(a9, y9);
+      b9 = fe9_mul(b9, y9);
+    }
+    a = fe9_to(a9);
+    b = fe9_to(b9);
+  } else {
+    Fe10 a10 = fe10_from(a), b10 = fe10_from(b), y10 = fe10_from(y);
+    for (uint32_t i = 0; i < iters; ++i) {
+      a10 = fe10_mul(a10, y10);
+      b10 = fe10_mul(b10, y10);
+    }
+    a = fe10_to(a10);
+    b = fe10_to(b10);
+  }
+  a = fe_canon(a);
+  b = fe_canon(b);
+  for (int l = 0; l < 8; ++l) {
+    out[(size_t)g * 16 + l] = a.v[l];
+    out[(size_t)g * 16 + 8 + l] = b.v[l];
+  }
 }
 
 }  // namespace
@@ -650,6 +711,47 @@ int og_hip_engine_advance(og_engine *e, og_survivor *out, uint32_t cap, uint32_t
 }
 
 void og_hip_engine_destroy(og_engine *e) { free_engine(e); }
+
+int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
+                       double *muls_per_s, uint8_t *results, char *err, size_t errcap) {
+  if ((radix != 8 && radix != 81 && radix != 9 && radix != 10) || threads == 0 || threads % 256 || threads > (1u << 22)) {
+    put_err(err, errcap, "radix must be 8, 81, 9 or 10 and threads a multiple of 256 up to 2^22");
+    return -1;
+  }
+  OG_TRY(hipSetDevice(ordinal));
+  uint32_t *d = nullptr;
+  OG_TRY(hipMalloc(&d, (size_t)threads * 64));
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  auto launch = [&](uint32_t it) {
+    if (radix == 8)
+      hipLaunchKernelGGL(og_radix_kernel<8>, dim3(threads / 256), dim3(256), 0, 0, it, d, threads);
+    else if (radix == 81)
+      hipLaunchKernelGGL(og_radix_kernel<81>, dim3(threads / 256), dim3(256), 0, 0, it, d, threads);
+    else if (radix == 9)
+      hipLaunchKernelGGL(og_radix_kernel<9>, dim3(threads / 256), dim3(256), 0, 0, it, d, threads);
+    else
+      hipLaunchKernelGGL(og_radix_kernel<10>, dim3(threads / 256), dim3(256), 0, 0, it, d, threads);
+  };
+  launch(iters < 16 ? iters : 16);  // warm-up
+  (void)hipEventRecord(e0, 0);
+  launch(iters);
+  (void)hipEventRecord(e1, 0);
+  hipError_t e = hipEventSynchronize(e1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess && results) e = hipMemcpy(results, d, (size_t)threads * 64, hipMemcpyDeviceToHost);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(d);
+  if (e != hipSuccess) {
+    put_err(err, errcap, std::string("radix bench: ") + hipGetErrorString(e));
+    return -1;
+  }
+  if (muls_per_s) *muls_per_s = 2.0 * threads * (double)iters / (ms * 1e-3);
+  return 0;
+}
 
 int og_hip_field_op(int ordinal, int op, const uint8_t *a, const uint8_t *b, uint8_t *out,
                     size_t n, char *err, size_t errcap) {
