@@ -30,7 +30,8 @@ def field_muls_per_candidate(half, with_x=False):
     """Field multiplications (squarings included) the kernel performs per
     candidate: forward 7/pair (+4 for x), backward 5/pair (+2 for x), the centre
     (3, +2 for x), one inversion (254 sq + 11 mul) and the stride addition (7)
-    per window of 2*half+1 candidates."""
+    per window of 2*half+1 candidates.  Priced at 64 32x32-bit products each,
+    the radix-independent count of an 8 x 32 schoolbook."""
     pair = 12 + (6 if with_x else 0)
     window = pair * half + 3 + (2 if with_x else 0) + 265 + 7
     return window / (2 * half + 1)
@@ -60,12 +61,24 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="0: cover 2^34 candidates")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--threads", type=int, default=1 << 17)
+    ap.add_argument("--threads", type=int, default=1 << 18)
     ap.add_argument("--half", type=int, default=256)
-    ap.add_argument("--rounds", type=int, default=4)
+    ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="run only a 1 GiB device-to-device copy: the known-size kernel the "
+                         "FETCH_SIZE / WRITE_SIZE counters are scaled by")
     args = ap.parse_args()
+    if args.copy_calibrate:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the search has no cross-GPU step; run with --gpus 1",
               file=sys.stderr)

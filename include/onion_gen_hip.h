@@ -90,12 +90,16 @@ double og_hip_engine_last_ms(const og_engine *e);
 /* Replaces Drop for the engine (src/gpu/cuda.rs:103-122). */
 void og_hip_engine_destroy(og_engine *e);
 
-/* Elementwise GF(2^255-19) arithmetic on the device for parity tests of the
- * field itself (the device form of src/field.rs:82-255): op 0 mul, 1 sq, 2 add,
- * 3 sub, 4 invert, 5 canonical.  Inputs are n raw 32-byte little-endian words
- * (any value below 2^256); outputs are canonical encodings. */
+/* Elementwise GF(2^255-19) arithmetic for parity tests of the field itself
+ * (the device forms of src/field.rs:82-255): ops 0..5 in the 8 x 32 form and
+ * 10..15 in the 10 x 25.5 form the search kernels run in; each block is mul,
+ * sq, add, sub, invert, canonical; op 16 is (2x - y)(x - y) in the 10-limb
+ * form, the loosest operands the search multiplies.  Inputs are n raw 32-byte little-endian
+ * words (any value below 2^256); outputs are canonical encodings.  The _host
+ * variant runs the same code on the CPU. */
 int og_hip_field_op(int ordinal, int op, const uint8_t *a, const uint8_t *b, uint8_t *out,
                     size_t n, char *err, size_t errcap);
+int og_field_op_host(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, size_t n);
 
 /* The radix bench (no counterpart in the reference; it decides the field
  * representation for gfx950): `threads` lanes each run two chains of `iters`
@@ -129,6 +133,104 @@ int og_filters_match_key(const og_filterset *s, const uint8_t packed[32]);
 /* Replaces FilterSet::match_whole_address (src/filter.rs:1673-1691): index of
  * a filter the 35 address bytes match, or -1. */
 int og_filters_match_address(const og_filterset *s, const uint8_t address[35]);
+
+/* ------------------------------------------------------------------ batch */
+
+/* Many independent searches in one launch.  No counterpart in the reference,
+ * whose device path runs one search per engine (src/gpu/mod.rs:292-477); per
+ * search the candidate numbering, survivor contract and errors are
+ * Engine::advance's.  A batch has `slots` search slots of `chains` chains; one
+ * og_batch_step replays a captured HIP graph that advances every active slot by
+ * chains * (2*half+1) * rounds candidates. */
+typedef struct og_batch og_batch;
+
+typedef struct {
+  uint32_t slots;    /* search slots; 0 = 64 */
+  uint32_t chains;   /* chains per slot, a multiple of 256; 0 = 2048 */
+  uint32_t half;     /* pairs per window; 0 = 256 */
+  uint32_t rounds;   /* windows per chain per step; 0 = 1 */
+  uint32_t ring_cap; /* survivor records per step; 0 = 65536 */
+} og_batch_opts;
+
+typedef struct {
+  uint64_t tag;       /* the tag the search was admitted with */
+  uint64_t offset;    /* scalar delta from the search's s0: 8k */
+  uint32_t slot;
+  uint8_t packed[32]; /* canonical y, sign bit clear */
+} og_batch_hit;
+
+int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char *err,
+                    size_t errcap);
+/* Device bytes a search's filter tables take (the admission footprint). */
+size_t og_batch_footprint(const og_filterset *filters);
+/* Puts a search (start point, filters, caller's tag) into a free slot: its
+ * chains are set up on the device and its candidate index starts at 0.
+ * Returns -2 when no slot is free. */
+int og_batch_admit(og_batch *b, const uint8_t start_xyzt[128], const og_filterset *filters,
+                   uint64_t tag, int *slot, char *err, size_t errcap);
+/* Frees a slot; its search stops at the end of the last step. */
+int og_batch_retire(og_batch *b, int slot);
+/* One step for every active slot; writes the survivors.  Overflowing the
+ * ring or `cap` is an error. */
+int og_batch_step(og_batch *b, og_batch_hit *out, uint32_t cap, uint32_t *n, char *err,
+                  size_t errcap);
+int og_batch_slot_state(const og_batch *b, int slot, uint64_t *tag, uint64_t *examined,
+                        uint64_t *hits, int *active);
+/* Candidates one step examines per active slot. */
+uint64_t og_batch_per_step(const og_batch *b);
+/* Device time of the last step (HIP events around the graph launch). */
+double og_batch_last_ms(const og_batch *b);
+void og_batch_destroy(og_batch *b);
+
+/* ------------------------------------------------------------------ serve */
+
+/* A serve loop over one batch: requests arrive over time, wait in a bounded
+ * queue, are admitted into free slots first come first served while the
+ * device-table budget allows, and are retired the step their n-th hit
+ * arrives.  No counterpart in the reference (its nearest analogue is the
+ * master's lease queue, src/master/queue.rs:51-175). */
+typedef struct {
+  double arrival_s;    /* seconds after the start, non-decreasing */
+  uint8_t seed[32];    /* the search's seed: s0 = clamp(SHA-512(seed)[0..32]) */
+  char filter[64];     /* filter text, NUL-terminated */
+  uint32_t want_hits;  /* n: the request completes at its n-th hit */
+} og_request;
+
+enum {
+  OG_REQ_NOT_ARRIVED = 0,
+  OG_REQ_QUEUED = 1,
+  OG_REQ_RUNNING = 2,
+  OG_REQ_DONE = 3,
+  OG_REQ_REJECTED = 4
+};
+
+typedef struct {
+  uint64_t offset;
+  uint8_t packed[32];
+} og_found;
+
+typedef struct {
+  int32_t status;       /* OG_REQ_* at the end of the run */
+  uint32_t hits;
+  uint64_t examined;
+  double admitted_s, first_hit_s, done_s;  /* -1 when not reached */
+  og_found found[16];   /* the first 16 hits */
+} og_request_result;
+
+typedef struct {
+  double wall_s, keys_per_s, device_busy_s;
+  uint64_t examined, steps;
+  uint32_t completed, running, queued, rejected, not_arrived;
+  double p50_time_to_n, p99_time_to_n;          /* over completed requests */
+  double p50_time_to_first, p99_time_to_first;  /* over requests with a hit */
+} og_serve_stats;
+
+/* Runs the loop for `duration_s` seconds of wall time (or until every request
+ * is done).  max_queue bounds the waiting queue; table_budget bounds the
+ * device bytes of admitted filter tables (0 = unbounded). */
+int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs, size_t n,
+                 double duration_s, uint32_t max_queue, uint64_t table_budget,
+                 og_request_result *results, og_serve_stats *stats, char *err, size_t errcap);
 
 /* ----------------------------------------------------------------- tuning */
 

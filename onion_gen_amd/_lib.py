@@ -7,7 +7,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libonion_gen_hip.so")
+LIB_PATH = os.environ.get("OG_LIB_PATH") or os.path.join(_HERE, "libonion_gen_hip.so")
 
 # Every entry point include/onion_gen_hip.h declares, with (restype, argtypes).
 _c = ctypes
@@ -33,6 +33,8 @@ SIGNATURES = {
     "og_hip_radix_bench": (_c.c_int, [_c.c_int, _c.c_int, _c.c_uint32, _c.c_uint32,
                                       _c.POINTER(_c.c_double), _c.c_char_p, _c.c_char_p,
                                       _c.c_size_t]),
+    "og_field_op_host": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_char_p, _c.c_char_p,
+                                    _c.c_size_t]),
     "og_filters_parse": (_c.c_void_p, [_c.c_char_p, _c.c_uint32, _c.c_char_p, _c.c_size_t]),
     "og_filters_free": (None, [_c.c_void_p]),
     "og_filters_count": (_c.c_size_t, [_c.c_void_p]),
@@ -60,6 +62,29 @@ SIGNATURES.update({
     "og_tuning_path": (_c.c_int, [_c.c_char_p, _c.c_size_t]),
     "og_tuning_get": (_c.c_int, [_c.c_char_p, _c.POINTER(TuningShape)]),
     "og_tuning_put": (_c.c_int, [_c.c_char_p, _c.POINTER(TuningShape)]),
+})
+
+SIGNATURES.update({
+    "og_batch_create": (_c.c_int, [_c.c_int, _c.c_void_p, _c.POINTER(_c.c_void_p), _c.c_char_p,
+                                   _c.c_size_t]),
+    "og_batch_footprint": (_c.c_size_t, [_c.c_void_p]),
+    "og_batch_admit": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_void_p, _c.c_uint64,
+                                  _c.POINTER(_c.c_int), _c.c_char_p, _c.c_size_t]),
+    "og_batch_retire": (_c.c_int, [_c.c_void_p, _c.c_int]),
+    "og_batch_step": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_uint32, _c.POINTER(_c.c_uint32),
+                                 _c.c_char_p, _c.c_size_t]),
+    "og_batch_slot_state": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(_c.c_uint64),
+                                       _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint64),
+                                       _c.POINTER(_c.c_int)]),
+    "og_batch_per_step": (_c.c_uint64, [_c.c_void_p]),
+    "og_batch_last_ms": (_c.c_double, [_c.c_void_p]),
+    "og_batch_destroy": (None, [_c.c_void_p]),
+})
+
+SIGNATURES.update({
+    "og_serve_run": (_c.c_int, [_c.c_int, _c.c_void_p, _c.c_void_p, _c.c_size_t, _c.c_double,
+                                _c.c_uint32, _c.c_uint64, _c.c_void_p, _c.c_void_p, _c.c_char_p,
+                                _c.c_size_t]),
 })
 
 _lib = None

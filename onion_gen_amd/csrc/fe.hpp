@@ -76,37 +76,6 @@ OG_HD Fe fe_mul(const Fe &a, const Fe &b) {
   return fe_fold(w);
 }
 
-// Product scanning with the carry flag of v_mad_u64_u32: each partial product
-// is one multiply-add into the column's 64-bit accumulator and one add of the
-// carry-out into the column's third word.  Device only (inline assembly).
-__device__ __forceinline__ void madc_step(uint64_t &acc, uint32_t &c2, uint32_t a, uint32_t b) {
-  uint64_t cy;
-  asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\tv_addc_co_u32_e64 %2, %1, 0, %2, %1"
-      : "+v"(acc), "=&s"(cy), "+v"(c2)
-      : "v"(a), "v"(b));
-}
-
-__device__ __forceinline__ Fe fe_mul_madc(const Fe &a, const Fe &b) {
-  uint32_t w[16];
-  uint64_t acc = (uint64_t)a.v[0] * b.v[0];
-  w[0] = (uint32_t)acc;
-  acc >>= 32;
-#pragma unroll
-  for (int k = 1; k < 15; ++k) {
-    uint32_t c2 = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int j = k - i;
-      if (j < 0 || j > 7) continue;
-      madc_step(acc, c2, a.v[i], b.v[j]);
-    }
-    w[k] = (uint32_t)acc;
-    acc = (acc >> 32) | ((uint64_t)c2 << 32);
-  }
-  w[15] = (uint32_t)acc;
-  return fe_fold(w);
-}
-
 // Squaring: the 28 cross products once, doubled, plus the 8 squares.
 OG_HD Fe fe_sq(const Fe &a) {
   uint32_t w[16];

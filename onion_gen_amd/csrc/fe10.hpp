@@ -1,11 +1,22 @@
-// The radix-2^25.5 alternative (ten limbs of 26/25 bits, the shape of the
-// reference's src/field4.rs:568-670 lane multiply), kept for the radix bench
-// that decides which representation the search uses on gfx950.
+// GF(2^255 - 19) in radix 2^25.5: ten limbs of 26 / 25 bits (even / odd
+// index), the representation the device search runs in.
 //
-// Limb i holds bits [ceil(25.5 i), ceil(25.5 (i+1))): 26 bits for even i,
-// 25 for odd i.  A product needs 100 partial products of at most 32 x 32 bits,
-// each on v_mad_u64_u32 with a 64-bit column accumulator; wrapped columns are
-// scaled by 19 (2^255 = 19) and odd x odd terms by 2 (the half bit).
+// Why this radix on gfx950 (see the radix bench): v_mad_u64_u32 issues at
+// nearly full rate, so the 100 partial products of a 10 x 10 schoolbook cost
+// about what the 64 of an 8 x 32 product do, but here every column
+// accumulates in one 64-bit register with no carry flag (gfx950 needs two wait
+// states between a VALU carry write and its read), and the six or seven free
+// bits per limb let additions and subtractions skip carry chains entirely.
+// It is the shape of the reference's src/field4.rs:568-670 lane multiply.
+//
+// Limb i holds bits [a_i, a_{i+1}) with a_i = ceil(25.5 i).  Bounds, in units
+// of the limb's nominal width 2^w (w = 26 even, 25 odd):
+//   tight  : the result of fe10_mul / fe10_carry, below 1 (limb 0 below 1 + 2^-21)
+//   loose  : the sum of two tight values (< 2), a tight minus a tight (< 3), or
+//            a loose-2 minus a tight (< 4)
+// fe10_mul(f, g) accepts f limbs up to 5 units (its largest column then stays
+// below 2^63.6) and g limbs below 3.3 units: g is scaled by 19 in 32 bits
+// (19 * 3.3 * 2^26 < 2^32), so the looser operand always goes first.
 #pragma once
 #include "fe.hpp"
 
@@ -15,61 +26,41 @@ struct Fe10 {
   uint32_t v[10];
 };
 
-OG_HD Fe10 fe10_from(const Fe &a) {
-  Fe c = fe_canon(a);
-  uint64_t bits[4];
-  for (int i = 0; i < 4; ++i) bits[i] = (uint64_t)c.v[2 * i] | ((uint64_t)c.v[2 * i + 1] << 32);
+OG_HD constexpr int fe10_width(int i) { return (i & 1) ? 25 : 26; }
+
+OG_HD Fe10 fe10_zero() {
   Fe10 r;
-  int pos = 0;
-  for (int i = 0; i < 10; ++i) {
-    const int w = (i & 1) ? 25 : 26;
-    const int word = pos >> 6, sh = pos & 63;
-    uint64_t x = bits[word] >> sh;
-    if (sh + w > 64 && word + 1 < 4) x |= bits[word + 1] << (64 - sh);
-    r.v[i] = (uint32_t)(x & ((1ull << w) - 1));
-    pos += w;
-  }
+#pragma unroll
+  for (int i = 0; i < 10; ++i) r.v[i] = 0;
   return r;
 }
 
-// Carry into 26/25-bit limbs; the carry out of limb 9 re-enters limb 0 as 19.
-OG_HD void fe10_carry(uint64_t h[10], Fe10 &r) {
-  uint64_t c;
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int i = 0; i < 10; ++i) {
-      const int w = (i & 1) ? 25 : 26;
-      c = h[i] >> w;
-      h[i] &= (1ull << w) - 1;
-      if (i < 9) h[i + 1] += c;
-      else h[0] += 19 * c;
-    }
-  }
-  for (int i = 0; i < 10; ++i) r.v[i] = (uint32_t)h[i];
+OG_HD Fe10 fe10_one() {
+  Fe10 r = fe10_zero();
+  r.v[0] = 1;
+  return r;
 }
 
-OG_HD Fe fe10_to(const Fe10 &a) {
-  // Sum the limbs at their bit positions with full carries (limbs may be
-  // slightly wider than their nominal width), then fold 2^256 = 38.
-  uint32_t w[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  int pos = 0;
-  for (int i = 0; i < 10; ++i) {
-    const int width = (i & 1) ? 25 : 26;
-    uint64_t v = a.v[i];
-    int word = pos >> 5, sh = pos & 31;
-    uint64_t add = v << sh;  // spans at most three 32-bit words
-    uint64_t c = 0;
-    for (int k = word; k < 10; ++k) {
-      c += (uint64_t)w[k] + (uint32_t)add;
-      w[k] = (uint32_t)c;
-      c >>= 32;
-      add >>= 32;
-      if (!add && !c) break;
-    }
-    pos += width;
+// Carry the 64-bit column sums of a product into tight limbs; the carry out of
+// limb 9 re-enters limb 0 as 19 (2^255 = 19).  A second, short pass moves the
+// (at most 2^38) re-entered carry on by one limb, which leaves limb 0 below
+// 2^26 + 19*2^13 and every other limb below 2^w.
+OG_HD Fe10 fe10_carry64(uint64_t h[10]) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const int w = fe10_width(i);
+    h[i + 1] += h[i] >> w;
+    h[i] &= (1ull << w) - 1;
   }
-  uint32_t wide[16] = {0};
-  for (int i = 0; i < 10; ++i) wide[i] = w[i];
-  return fe_canon(fe_fold(wide));
+  const uint64_t c9 = h[9] >> 25;
+  h[9] &= (1ull << 25) - 1;
+  h[0] += 19 * c9;
+  h[1] += h[0] >> 26;
+  h[0] &= (1ull << 26) - 1;
+  Fe10 r;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) r.v[i] = (uint32_t)h[i];
+  return r;
 }
 
 OG_HD Fe10 fe10_mul(const Fe10 &f, const Fe10 &g) {
@@ -86,21 +77,125 @@ OG_HD Fe10 fe10_mul(const Fe10 &f, const Fe10 &g) {
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
       const int j = k - i;
+      // an odd limb times an odd limb lands half a bit high: doubled
       if (j >= 0) {
-        // odd x odd products carry the extra half bit
-        const uint32_t fi = ((i & 1) && (j & 1)) ? f2[i] : f.v[i];
-        acc = mad64(fi, g.v[j], acc);
+        acc = mad64(((i & 1) && (j & 1)) ? f2[i] : f.v[i], g.v[j], acc);
       } else {
         const int jj = j + 10;
-        const uint32_t fi = ((i & 1) && (jj & 1)) ? f2[i] : f.v[i];
-        acc = mad64(fi, g19[jj], acc);
+        acc = mad64(((i & 1) && (jj & 1)) ? f2[i] : f.v[i], g19[jj], acc);
       }
     }
     h[k] = acc;
   }
+  return fe10_carry64(h);
+}
+
+OG_HD Fe10 fe10_sq(const Fe10 &f) { return fe10_mul(f, f); }
+
+OG_HD Fe10 fe10_add(const Fe10 &a, const Fe10 &b) {
   Fe10 r;
-  fe10_carry(h, r);
+#pragma unroll
+  for (int i = 0; i < 10; ++i) r.v[i] = a.v[i] + b.v[i];
   return r;
+}
+
+// a - b + 2p, limb by limb: every limb of 2p (2^27 - 38, 2^26 - 2, 2^27 - 2, ...)
+// exceeds the corresponding limb of a tight b, so nothing goes negative.
+OG_HD Fe10 fe10_sub(const Fe10 &a, const Fe10 &b) {
+  Fe10 r;
+  r.v[0] = a.v[0] + ((1u << 27) - 38u) - b.v[0];
+#pragma unroll
+  for (int i = 1; i < 10; ++i)
+    r.v[i] = a.v[i] + ((i & 1) ? ((1u << 26) - 2u) : ((1u << 27) - 2u)) - b.v[i];
+  return r;
+}
+
+// Tighten a loose value (limbs below 2^31).
+OG_HD Fe10 fe10_carry(const Fe10 &a) {
+  uint64_t h[10];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) h[i] = a.v[i];
+  return fe10_carry64(h);
+}
+
+// The low 32 bits of the integer a tight value stands for.  Its canonical
+// form is that integer, or that integer minus p (when it is at least p), whose
+// low 32 bits are those of the integer plus 19: a filter on the leading 32
+// bits of the key asks both and confirms on the canonical form.
+OG_HD uint32_t fe10_low32(const Fe10 &a) { return a.v[0] + (a.v[1] << 26); }
+
+// The canonical representative below p as eight 32-bit words.  q = [x >= p]
+// comes from propagating x + 19 through the limbs; then x + 19q - 2^255 q is
+// carried out exactly and packed (the technique of ref10's fe_tobytes).
+OG_HD Fe fe10_canon(const Fe10 &a) {
+  Fe10 t = fe10_carry(a);
+  uint32_t q = (t.v[0] + 19u) >> 26;
+#pragma unroll
+  for (int i = 1; i < 10; ++i) q = (t.v[i] + q) >> fe10_width(i);
+  uint32_t h[10];
+  h[0] = t.v[0] + 19u * q;
+#pragma unroll
+  for (int i = 1; i < 10; ++i) h[i] = t.v[i];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const int w = fe10_width(i);
+    h[i + 1] += h[i] >> w;
+    h[i] &= (1u << w) - 1;
+  }
+  h[9] &= (1u << 25) - 1;  // drops 2^255 q
+  Fe r;
+  r.v[0] = h[0] | (h[1] << 26);
+  r.v[1] = (h[1] >> 6) | (h[2] << 19);
+  r.v[2] = (h[2] >> 13) | (h[3] << 13);
+  r.v[3] = (h[3] >> 19) | (h[4] << 6);
+  r.v[4] = h[5] | (h[6] << 25);
+  r.v[5] = (h[6] >> 7) | (h[7] << 19);
+  r.v[6] = (h[7] >> 13) | (h[8] << 12);
+  r.v[7] = (h[8] >> 20) | (h[9] << 6);
+  return r;
+}
+
+// From any 8-word value below 2^256 (canonicalised first).
+OG_HD Fe10 fe10_from(const Fe &x) {
+  const Fe c = fe_canon(x);
+  Fe10 r;
+  r.v[0] = c.v[0] & 0x3ffffff;
+  r.v[1] = ((c.v[0] >> 26) | (c.v[1] << 6)) & 0x1ffffff;
+  r.v[2] = ((c.v[1] >> 19) | (c.v[2] << 13)) & 0x3ffffff;
+  r.v[3] = ((c.v[2] >> 13) | (c.v[3] << 19)) & 0x1ffffff;
+  r.v[4] = (c.v[3] >> 6) & 0x3ffffff;
+  r.v[5] = c.v[4] & 0x1ffffff;
+  r.v[6] = ((c.v[4] >> 25) | (c.v[5] << 7)) & 0x3ffffff;
+  r.v[7] = ((c.v[5] >> 19) | (c.v[6] << 13)) & 0x1ffffff;
+  r.v[8] = ((c.v[6] >> 12) | (c.v[7] << 20)) & 0x3ffffff;
+  r.v[9] = (c.v[7] >> 6) & 0x1ffffff;
+  return r;
+}
+
+OG_HD Fe fe10_to(const Fe10 &a) { return fe10_canon(a); }
+
+OG_HD Fe10 fe10_sqn(Fe10 a, int n) {
+  for (int i = 0; i < n; ++i) a = fe10_sq(a);
+  return a;
+}
+
+// x^(p-2): the ref10 chain, 254 squarings and 11 multiplications
+// (src/field.rs:163-220).
+OG_HD Fe10 fe10_invert(const Fe10 &z1) {
+  Fe10 z2 = fe10_sq(z1);
+  Fe10 z8 = fe10_sqn(z2, 2);
+  Fe10 z9 = fe10_mul(z1, z8);
+  Fe10 z11 = fe10_mul(z2, z9);
+  Fe10 z22 = fe10_sq(z11);
+  Fe10 z5 = fe10_mul(z9, z22);
+  Fe10 z10 = fe10_mul(fe10_sqn(z5, 5), z5);
+  Fe10 z20 = fe10_mul(fe10_sqn(z10, 10), z10);
+  Fe10 z40 = fe10_mul(fe10_sqn(z20, 20), z20);
+  Fe10 z50 = fe10_mul(fe10_sqn(z40, 10), z10);
+  Fe10 z100 = fe10_mul(fe10_sqn(z50, 50), z50);
+  Fe10 z200 = fe10_mul(fe10_sqn(z100, 100), z100);
+  Fe10 t = fe10_mul(fe10_sqn(z200, 50), z50);
+  return fe10_mul(fe10_sqn(t, 5), z11);
 }
 
 }  // namespace og

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <string>
 
+#include "fe10.hpp"
 #include "keccak.hpp"
 #include "sha512.hpp"
 
@@ -126,6 +127,47 @@ std::vector<uint32_t> HostCurve::offset_table(uint32_t half) const {
     }
   }
   return out;
+}
+
+std::vector<uint32_t> HostCurve::offset_table10(uint32_t half) const {
+  const std::vector<uint32_t> t8 = offset_table(half);
+  std::vector<uint32_t> out((size_t)half * 30);
+  for (uint32_t m = 0; m < half; ++m)
+    for (int c = 0; c < 3; ++c) {
+      Fe x;
+      for (int l = 0; l < 8; ++l) x.v[l] = t8[(size_t)m * 24 + 8 * c + l];
+      put_fe10(&out[(size_t)m * 30 + 10 * c], x);
+    }
+  return out;
+}
+
+SetupArgs HostCurve::setup_args(const uint8_t start_xyzt[128], uint32_t *state, uint32_t threads,
+                                uint32_t first, uint32_t count, uint32_t half) const {
+  SetupArgs sa;
+  std::memset(&sa, 0, sizeof sa);
+  sa.state = state;
+  sa.threads = threads;
+  sa.first = first;
+  sa.count = count;
+  sa.half = half;
+  for (int c = 0; c < 4; ++c) {
+    Fe v = fe_from_bytes(start_xyzt + 32 * c);
+    for (int l = 0; l < 8; ++l) sa.start[8 * c + l] = v.v[l];
+  }
+  const Cached eight = eight_cached();
+  for (int l = 0; l < 8; ++l) {
+    sa.eight[l] = eight.ypx.v[l];
+    sa.eight[8 + l] = eight.ymx.v[l];
+    sa.eight[16 + l] = eight.Z.v[l];
+    sa.eight[24 + l] = eight.t2d.v[l];
+    sa.two_d[l] = two_d.v[l];
+  }
+  return sa;
+}
+
+void put_fe10(uint32_t out[10], const Fe &x) {
+  const Fe10 r = fe10_from(x);
+  for (int l = 0; l < 10; ++l) out[l] = r.v[l];
 }
 
 Cached1 HostCurve::multiple_of_eight_cached1(uint64_t m) const {

@@ -7,6 +7,7 @@
 #include "../../include/onion_gen_hip.h"
 #include "ge.hpp"
 #include "og_filters.hpp"
+#include "og_kernels.hpp"
 
 namespace og {
 
@@ -18,6 +19,12 @@ struct HostCurve {
   Cached eight_cached() const;  // 8G
   // Q_m = m * 8G for m = 1..half as (x, y, d*x*y), canonical, 24 words each.
   std::vector<uint32_t> offset_table(uint32_t half) const;
+  // The same table in the 10-limb form the kernels read: 30 words per entry.
+  std::vector<uint32_t> offset_table10(uint32_t half) const;
+  // Set-up arguments for chains [first, first + count) of a start point given
+  // as 128 bytes xyzt, in a state buffer of SoA stride `threads`.
+  SetupArgs setup_args(const uint8_t start_xyzt[128], uint32_t *state, uint32_t threads,
+                       uint32_t first, uint32_t count, uint32_t half) const;
   // m * 8G with Z scaled to one.
   Cached1 multiple_of_eight_cached1(uint64_t m) const;
   // scalar * B, 256-bit scalar, double-and-add from the top bit
@@ -27,5 +34,8 @@ struct HostCurve {
 };
 
 const FilterSet *filterset_of(const og_filterset *s);
+
+// Writes the 10-limb form of an 8-word value.
+void put_fe10(uint32_t out[10], const Fe &x);
 
 }  // namespace og

@@ -87,3 +87,55 @@ def dictionary(count=10000, seed=2):
             seen.add(word)
             out.append(word)
     return out
+
+
+# ---- the batched and served mixes
+
+def _suffix(stream, i, length):
+    body = [ALPHABET[b & 31] for b in stream[i:i + length]]
+    body[-1] = "d"                              # the version byte fixes the last symbol
+    body[-2] = "aiqy"[stream[i + length] & 3]   # and leaves four values before it
+    return "".join(body)
+
+
+def search_mix(count, seed, hits="loguniform"):
+    """``count`` searches: (seed32, filter text, wanted hits).  Filter kind is
+    uniform over {prefix of 4-7 symbols, suffix of 3-5 symbols, substring of 5};
+    wanted hits are log-uniform in 1..1000 (``hits="loguniform"``) or uniform
+    over {1, 10, 100} (``hits="choice"``)."""
+    import math
+    stream = chacha_stream(seed, count * 64)
+    out = []
+    for r in range(count):
+        s = stream[64 * r:64 * r + 64]
+        kind = s[0] % 3
+        if kind == 0:
+            text = symbols(s, 4 + s[1] % 4, 8)
+        elif kind == 1:
+            text = "suffix:" + _suffix(s, 8, 3 + s[1] % 3)
+        else:
+            text = "contains:" + symbols(s, 5, 8)
+        if hits == "loguniform":
+            u = int.from_bytes(s[2:6], "little") / 2 ** 32
+            want = max(1, min(1000, int(round(math.exp(u * math.log(1000))))))
+        else:
+            want = (1, 10, 100)[s[2] % 3]
+        out.append((s[32:64], text, want))
+    return out
+
+
+def poisson_arrivals(rate, duration, seed):
+    """Arrival times of a Poisson process of ``rate`` per second over
+    ``duration`` seconds, from ChaCha20(seed) uniforms."""
+    import math
+    times, t, i = [], 0.0, 0
+    stream = chacha_stream(seed, int(rate * duration * 3 + 64) * 8)
+    while True:
+        if 8 * i + 8 > len(stream):
+            stream += chacha_stream(seed + 1_000_003 * (i + 1), len(stream))
+        u = (int.from_bytes(stream[8 * i:8 * i + 8], "little") + 0.5) / 2 ** 64
+        i += 1
+        t += -math.log(u) / rate
+        if t >= duration:
+            return times
+        times.append(t)

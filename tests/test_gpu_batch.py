@@ -1,0 +1,97 @@
+"""The slot engine against the oracle: each search in a mixed batch reports
+exactly its own hits, the same as it reports alone, across graph replays,
+retirement and re-admission."""
+import hashlib
+
+import pytest
+
+from onion_gen_amd import batch, filters, gpu, key
+from tests.oracle_lib import OracleFilters, search
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def clear_sign(pk):
+    return pk[:31] + bytes([pk[31] & 0x7F])
+
+
+def oracle(seed, lines, n):
+    start = key.start_point(seed)
+    return sorted((k, clear_sign(pk)) for k, pk, _ in search(start, 0, n, 256, OracleFilters(lines)))
+
+
+MIX = [
+    (b"prefix", ["2x"]),
+    (b"suffix", ["suffix:aqd"]),
+    (b"dict", ["2xe", "abc", "zz7", "q2q", "77", "a2b3"]),
+    (b"contains", ["contains:abcd"]),
+]
+
+
+def test_mixed_batch_matches_each_search_alone():
+    say("case: batch 4 slots x 512 chains, half 64, mixed filter kinds")
+    b = batch.Batch(slots=4, chains=512, half=64, rounds=1)
+    seeds = {tag: hashlib.sha256(tag).digest() for tag, _ in MIX}
+    fsets = {tag: filters.FilterSet.parse(lines) for tag, lines in MIX}
+    slots = {}
+    for i, (tag, lines) in enumerate(MIX):
+        slots[tag] = b.admit(key.start_point(seeds[tag]), fsets[tag], i)
+    assert sorted(slots.values()) == [0, 1, 2, 3]
+    assert b.admit(key.start_point(b"x" * 32), fsets[b"prefix"], 99) is None  # full
+    got = {tag: [] for tag, _ in MIX}
+    n = 1 << 18
+    steps = -(-n // b.per_step())
+    for _ in range(steps):
+        for h in b.step():
+            tag = MIX[h.tag][0]
+            assert h.slot == slots[tag]
+            if h.offset // 8 < n:
+                got[tag].append((h.offset // 8, h.packed))
+    for tag, lines in MIX:
+        want = oracle(seeds[tag], lines, n)
+        assert sorted(got[tag]) == want and want, (tag, len(got[tag]), len(want))
+    # alone: the single-search engine over the same range gives the same set
+    eng = gpu.Engine.with_filters(key.start_point(seeds[b"prefix"]), fsets[b"prefix"],
+                                  threads=512, half=64)
+    alone, done = [], 0
+    while done < n:
+        alone += [(s.offset // 8, s.packed) for s in eng.advance() if s.offset // 8 < n]
+        done += eng.batch()
+    eng.close()
+    assert sorted(alone) == sorted(got[b"prefix"])
+    b.close()
+
+
+def test_retire_and_readmit_starts_a_search_from_zero():
+    say("case: retire slot, admit a new search into it")
+    b = batch.Batch(slots=2, chains=256, half=32, rounds=1)
+    s1, s2 = hashlib.sha256(b"one").digest(), hashlib.sha256(b"two").digest()
+    f = filters.FilterSet.parse("2x")
+    slot_a = b.admit(key.start_point(s1), f, 1)
+    slot_b = b.admit(key.start_point(s2), f, 2)
+    for _ in range(3):
+        b.step()
+    assert b.slot_state(slot_a)["examined"] == 3 * b.per_step()
+    b.retire(slot_a)
+    assert not b.slot_state(slot_a)["active"]
+    s3 = hashlib.sha256(b"three").digest()
+    slot_c = b.admit(key.start_point(s3), f, 3)
+    assert slot_c == slot_a
+    n = 1 << 16
+    got = {2: [], 3: []}
+    steps = -(-n // b.per_step())
+    for _ in range(steps):
+        for h in b.step():
+            if h.tag in got and h.offset // 8 < n + 3 * b.per_step():
+                got[h.tag].append((h.offset // 8, h.packed))
+    want3 = oracle(s3, ["2x"], n)
+    assert sorted(x for x in got[3] if x[0] < n) == want3
+    # the search in the other slot simply continued: its k kept counting
+    want2 = oracle(s2, ["2x"], 3 * b.per_step() + n)
+    assert sorted(x for x in got[2] if x[0] >= 3 * b.per_step()) == \
+        [x for x in want2 if x[0] >= 3 * b.per_step()]
+    b.close()

@@ -80,6 +80,14 @@ def test_field_kernels_match_big_integers():
     assert field_op(5, a, b) == [x % P for x in a]
     nz = [x for x in a if x % P][:2000]
     assert field_op(4, nz, nz) == [pow(x, P - 2, P) for x in nz]
+    # the 10 x 25.5 form the search kernels run in
+    assert field_op(10, a, b) == [x * y % P for x, y in zip(a, b)]
+    assert field_op(11, a, b) == [x * x % P for x in a]
+    assert field_op(12, a, b) == [(x + y) % P for x, y in zip(a, b)]
+    assert field_op(13, a, b) == [(x - y) % P for x, y in zip(a, b)]
+    assert field_op(15, a, b) == [x % P for x in a]
+    assert field_op(14, nz, nz) == [pow(x, P - 2, P) for x in nz]
+    assert field_op(16, a, b) == [(x - y) * (2 * x - y) % P for x, y in zip(a, b)]
 
 
 def test_config1_golden_hits():

@@ -66,6 +66,42 @@ def test_offset_secret_rederives_the_chain_key():
         assert len(addr) == 56 and addr.endswith("d")
 
 
+def host_field_op(op, a, b):
+    import ctypes
+    n = len(a)
+    out = ctypes.create_string_buffer(32 * n)
+    rc = _lib.lib().og_field_op_host(op, b"".join(x.to_bytes(32, "little") for x in a),
+                                     b"".join(x.to_bytes(32, "little") for x in b), out, n)
+    assert rc == 0
+    return [int.from_bytes(out.raw[32 * i:32 * i + 32], "little") for i in range(n)]
+
+
+def test_field_arithmetic_both_forms_on_the_host():
+    """The device field code (8 x 32 and 10 x 25.5) compiled for the host,
+    against Python integers, including the loosest operands the search feeds
+    its multiplications (op 16)."""
+    P = 2 ** 255 - 19
+    rng = random.Random(17)
+    edges = [0, 1, 2, 19, P - 1, P, P + 1, 2 * P, 2 ** 255 - 1, 2 ** 255, 2 ** 256 - 1,
+             2 ** 256 - 38, 2 ** 256 - 39, 38, 2 ** 26 - 1, 2 ** 26, 2 ** 51]
+    a = edges + [rng.getrandbits(256) for _ in range(2000)]
+    b = list(reversed(edges)) + [rng.getrandbits(256) for _ in range(2000)]
+    for base in (0, 10):
+        assert host_field_op(base + 0, a, b) == [x * y % P for x, y in zip(a, b)]
+        assert host_field_op(base + 1, a, b) == [x * x % P for x in a]
+        assert host_field_op(base + 2, a, b) == [(x + y) % P for x, y in zip(a, b)]
+        assert host_field_op(base + 3, a, b) == [(x - y) % P for x, y in zip(a, b)]
+        assert host_field_op(base + 5, a, b) == [x % P for x in a]
+        nz = [x for x in a if x % P][:150]
+        assert host_field_op(base + 4, nz, nz) == [pow(x, P - 2, P) for x in nz]
+    assert host_field_op(16, a, b) == [(x - y) * (2 * x - y) % P for x, y in zip(a, b)]
+    # the largest limbs a tight value can carry: all-ones limbs
+    big = [sum(((1 << (26 if i % 2 == 0 else 25)) - 1) << [0, 26, 51, 77, 102, 128, 153, 179, 204, 230][i]
+               for i in range(10))]
+    assert host_field_op(16, big * 3, [0, 1, big[0]]) == \
+        [(big[0] - y) * (2 * big[0] - y) % P for y in [0, 1, big[0]]]
+
+
 def test_hit_room_bounds():
     # src/gpu/mod.rs:521-553
     for bits in range(8, 31):
