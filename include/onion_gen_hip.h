@@ -139,14 +139,16 @@ int og_filters_match_address(const og_filterset *s, const uint8_t address[35]);
 /* Many independent searches in one launch.  No counterpart in the reference,
  * whose device path runs one search per engine (src/gpu/mod.rs:292-477); per
  * search the candidate numbering, survivor contract and errors are
- * Engine::advance's.  A batch has `slots` search slots of `chains` chains; one
- * og_batch_step replays a captured HIP graph that advances every active slot by
+ * Engine::advance's.  A batch keeps any number of resident searches on the
+ * device (chain state, filter tables, next candidate) and runs `slots` of them
+ * per step, round-robin, each on `chains` chains; one og_batch_step replays a
+ * captured HIP graph that advances each scheduled search by
  * chains * (2*half+1) * rounds candidates. */
 typedef struct og_batch og_batch;
 
 typedef struct {
-  uint32_t slots;    /* search slots; 0 = 64 */
-  uint32_t chains;   /* chains per slot, a multiple of 256; 0 = 2048 */
+  uint32_t slots;    /* searches per step; 0 = 64 */
+  uint32_t chains;   /* chains per search, a multiple of 256; 0 = 2048 */
   uint32_t half;     /* pairs per window; 0 = 256 */
   uint32_t rounds;   /* windows per chain per step; 0 = 1 */
   uint32_t ring_cap; /* survivor records per step; 0 = 65536 */
@@ -155,29 +157,31 @@ typedef struct {
 typedef struct {
   uint64_t tag;       /* the tag the search was admitted with */
   uint64_t offset;    /* scalar delta from the search's s0: 8k */
-  uint32_t slot;
+  uint32_t search;    /* the search's id */
   uint8_t packed[32]; /* canonical y, sign bit clear */
 } og_batch_hit;
 
 int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char *err,
                     size_t errcap);
-/* Device bytes a search's filter tables take (the admission footprint). */
-size_t og_batch_footprint(const og_filterset *filters);
-/* Puts a search (start point, filters, caller's tag) into a free slot: its
- * chains are set up on the device and its candidate index starts at 0.
- * Returns -2 when no slot is free. */
+/* Device bytes a resident search takes: its chain state and filter tables
+ * (the admission footprint).  b may be null for the default chain count. */
+size_t og_batch_footprint(const og_batch *b, const og_filterset *filters);
+/* Makes a search (start point, filters, caller's tag) resident: its chains are
+ * set up on the device and its candidate index starts at 0.  Writes its id. */
 int og_batch_admit(og_batch *b, const uint8_t start_xyzt[128], const og_filterset *filters,
-                   uint64_t tag, int *slot, char *err, size_t errcap);
-/* Frees a slot; its search stops at the end of the last step. */
-int og_batch_retire(og_batch *b, int slot);
-/* One step for every active slot; writes the survivors.  Overflowing the
- * ring or `cap` is an error. */
+                   uint64_t tag, int *id, char *err, size_t errcap);
+/* Frees a resident search's device memory; it is not scheduled again. */
+int og_batch_retire(og_batch *b, int id);
+/* One step: the next `slots` resident searches (round-robin) each advance;
+ * writes the survivors.  Overflowing the ring or `cap` is an error. */
 int og_batch_step(og_batch *b, og_batch_hit *out, uint32_t cap, uint32_t *n, char *err,
                   size_t errcap);
-int og_batch_slot_state(const og_batch *b, int slot, uint64_t *tag, uint64_t *examined,
-                        uint64_t *hits, int *active);
-/* Candidates one step examines per active slot. */
+int og_batch_search_state(const og_batch *b, int id, uint64_t *tag, uint64_t *examined,
+                          uint64_t *hits, int *live);
+/* Candidates one step examines per scheduled search. */
 uint64_t og_batch_per_step(const og_batch *b);
+/* Device bytes held by resident searches. */
+uint64_t og_batch_resident_bytes(const og_batch *b);
 /* Device time of the last step (HIP events around the graph launch). */
 double og_batch_last_ms(const og_batch *b);
 void og_batch_destroy(og_batch *b);
@@ -185,10 +189,11 @@ void og_batch_destroy(og_batch *b);
 /* ------------------------------------------------------------------ serve */
 
 /* A serve loop over one batch: requests arrive over time, wait in a bounded
- * queue, are admitted into free slots first come first served while the
- * device-table budget allows, and are retired the step their n-th hit
- * arrives.  No counterpart in the reference (its nearest analogue is the
- * master's lease queue, src/master/queue.rs:51-175). */
+ * queue, become resident first come first served while their device footprint
+ * (chain state + filter tables) fits the budget, share the slots round-robin,
+ * and are retired the step their n-th hit arrives.  No counterpart in the
+ * reference (its nearest analogue is the master's lease queue,
+ * src/master/queue.rs:51-175). */
 typedef struct {
   double arrival_s;    /* seconds after the start, non-decreasing */
   uint8_t seed[32];    /* the search's seed: s0 = clamp(SHA-512(seed)[0..32]) */
@@ -220,16 +225,17 @@ typedef struct {
 typedef struct {
   double wall_s, keys_per_s, device_busy_s;
   uint64_t examined, steps;
+  uint32_t max_resident;
   uint32_t completed, running, queued, rejected, not_arrived;
   double p50_time_to_n, p99_time_to_n;          /* over completed requests */
   double p50_time_to_first, p99_time_to_first;  /* over requests with a hit */
 } og_serve_stats;
 
 /* Runs the loop for `duration_s` seconds of wall time (or until every request
- * is done).  max_queue bounds the waiting queue; table_budget bounds the
- * device bytes of admitted filter tables (0 = unbounded). */
+ * is done).  max_queue bounds the waiting queue; device_budget bounds the
+ * device bytes of resident searches (0 = 4 GiB). */
 int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs, size_t n,
-                 double duration_s, uint32_t max_queue, uint64_t table_budget,
+                 double duration_s, uint32_t max_queue, uint64_t device_budget,
                  og_request_result *results, og_serve_stats *stats, char *err, size_t errcap);
 
 /* ----------------------------------------------------------------- tuning */

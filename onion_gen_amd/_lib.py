@@ -67,15 +67,16 @@ SIGNATURES.update({
 SIGNATURES.update({
     "og_batch_create": (_c.c_int, [_c.c_int, _c.c_void_p, _c.POINTER(_c.c_void_p), _c.c_char_p,
                                    _c.c_size_t]),
-    "og_batch_footprint": (_c.c_size_t, [_c.c_void_p]),
+    "og_batch_footprint": (_c.c_size_t, [_c.c_void_p, _c.c_void_p]),
     "og_batch_admit": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_void_p, _c.c_uint64,
                                   _c.POINTER(_c.c_int), _c.c_char_p, _c.c_size_t]),
     "og_batch_retire": (_c.c_int, [_c.c_void_p, _c.c_int]),
     "og_batch_step": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_uint32, _c.POINTER(_c.c_uint32),
                                  _c.c_char_p, _c.c_size_t]),
-    "og_batch_slot_state": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(_c.c_uint64),
-                                       _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint64),
-                                       _c.POINTER(_c.c_int)]),
+    "og_batch_search_state": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(_c.c_uint64),
+                                         _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint64),
+                                         _c.POINTER(_c.c_int)]),
+    "og_batch_resident_bytes": (_c.c_uint64, [_c.c_void_p]),
     "og_batch_per_step": (_c.c_uint64, [_c.c_void_p]),
     "og_batch_last_ms": (_c.c_double, [_c.c_void_p]),
     "og_batch_destroy": (None, [_c.c_void_p]),

@@ -1,4 +1,7 @@
-"""Many searches in one launch: the slot engine of include/onion_gen_hip.h."""
+"""Many searches in one launch: the slot engine of include/onion_gen_hip.h.
+
+Resident searches keep their chain state and filter tables on the device and
+take the ``slots`` of each step round-robin."""
 import ctypes
 from dataclasses import dataclass
 
@@ -12,14 +15,14 @@ class _Opts(ctypes.Structure):
 
 
 class _Hit(ctypes.Structure):
-    _fields_ = [("tag", ctypes.c_uint64), ("offset", ctypes.c_uint64), ("slot", ctypes.c_uint32),
-                ("packed", ctypes.c_uint8 * 32)]
+    _fields_ = [("tag", ctypes.c_uint64), ("offset", ctypes.c_uint64),
+                ("search", ctypes.c_uint32), ("packed", ctypes.c_uint8 * 32)]
 
 
 @dataclass(frozen=True)
 class BatchHit:
     tag: int
-    slot: int
+    search: int
     offset: int
     packed: bytes
 
@@ -34,7 +37,7 @@ class Batch:
         if rc != 0:
             raise OgError(e.value.decode())
         self._h = h
-        self._out = (_Hit * (1 << 16))()
+        self._out = (_Hit * (ring_cap or (1 << 16)))()
 
     def per_step(self):
         return lib().og_batch_per_step(self._h)
@@ -42,21 +45,25 @@ class Batch:
     def last_ms(self):
         return lib().og_batch_last_ms(self._h)
 
+    def resident_bytes(self):
+        return lib().og_batch_resident_bytes(self._h)
+
+    def footprint(self, filters):
+        return lib().og_batch_footprint(self._h, filters.handle)
+
     def admit(self, start_xyzt, filters, tag):
-        """Slot index of the admitted search, or None when every slot is busy."""
-        slot = ctypes.c_int()
+        """Makes a search resident; returns its id."""
+        sid = ctypes.c_int()
         e = err_buf()
         rc = lib().og_batch_admit(self._h, bytes(start_xyzt), filters.handle, tag,
-                                  ctypes.byref(slot), e, len(e))
-        if rc == -2:
-            return None
+                                  ctypes.byref(sid), e, len(e))
         if rc != 0:
             raise OgError(e.value.decode())
-        return slot.value
+        return sid.value
 
-    def retire(self, slot):
-        if lib().og_batch_retire(self._h, slot) != 0:
-            raise OgError(f"could not retire slot {slot}")
+    def retire(self, sid):
+        if lib().og_batch_retire(self._h, sid) != 0:
+            raise OgError(f"could not retire search {sid}")
 
     def step(self):
         n = ctypes.c_uint32()
@@ -64,16 +71,16 @@ class Batch:
         rc = lib().og_batch_step(self._h, self._out, len(self._out), ctypes.byref(n), e, len(e))
         if rc != 0:
             raise OgError(e.value.decode())
-        return [BatchHit(self._out[i].tag, self._out[i].slot, self._out[i].offset,
+        return [BatchHit(self._out[i].tag, self._out[i].search, self._out[i].offset,
                          bytes(self._out[i].packed)) for i in range(n.value)]
 
-    def slot_state(self, slot):
+    def search_state(self, sid):
         tag, examined, hits = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
-        active = ctypes.c_int()
-        lib().og_batch_slot_state(self._h, slot, ctypes.byref(tag), ctypes.byref(examined),
-                                  ctypes.byref(hits), ctypes.byref(active))
+        live = ctypes.c_int()
+        lib().og_batch_search_state(self._h, sid, ctypes.byref(tag), ctypes.byref(examined),
+                                    ctypes.byref(hits), ctypes.byref(live))
         return {"tag": tag.value, "examined": examined.value, "hits": hits.value,
-                "active": bool(active.value)}
+                "live": bool(live.value)}
 
     def close(self):
         if self._h:
@@ -91,7 +98,3 @@ class Batch:
             self.close()
         except Exception:
             pass
-
-
-def footprint(filters):
-    return lib().og_batch_footprint(filters.handle)

@@ -1,20 +1,25 @@
 // The slot engine: many independent searches in one launch.
 //
-// A batch holds `slots` search slots of `chains` chains each.  One step is a
-// captured HIP graph - clear the survivor counter, run every chain of every
-// active slot for `rounds` windows, advance each active slot's candidate base,
-// copy the counter back - replayed as is, so admitting or retiring a search
-// never re-captures anything: a slot's descriptor in device memory says
-// whether its blocks work and which filter they apply.  Searches of different
-// filter kinds share the launch; each block reads its slot's descriptor once.
+// A batch keeps any number of *resident* searches on the device - each with
+// its own chain state (chains x 160 B in HBM), filter tables and next
+// candidate index - and runs `slots` of them per step.  One step is a captured
+// HIP graph: clear the survivor counter, run every chain of every scheduled
+// search for `rounds` windows, copy the counter back.  Before each launch the
+// host writes the per-slot descriptors (which search, its state, its filter,
+// its k base), so scheduling never re-captures the graph.  Resident searches
+// take the slots round-robin: with more searches than slots the device is
+// time-sliced between them, and no search waits for another to finish.
+// Searches of different filter kinds share the launch; each block reads its
+// slot's descriptor once.
 //
 // This is the batching the reference does not have (its device path runs one
-// search per engine, src/gpu/mod.rs:292-477); per slot the arithmetic, the
+// search per engine, src/gpu/mod.rs:292-477); per search the arithmetic, the
 // candidate numbering and the survivor contract are the single engine's.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <string>
 #include <vector>
@@ -29,27 +34,26 @@ using namespace og;
 namespace {
 
 struct SlotDesc {
-  DevFilterArgs f;      // f.mode == 0: inactive
-  uint64_t k_base;      // next candidate of the slot's search
+  DevFilterArgs f;      // f.mode == 0: no search in this slot this step
+  uint64_t k_base;      // first candidate of this step for the slot's search
+  uint32_t *state;      // the search's chains: [4 coords][10 limbs][chains]
 };
 
 struct BatchArgs {
-  uint32_t *state;
   const uint32_t *table;
-  uint32_t *scratch;
-  SlotDesc *slots;
+  uint32_t *scratch;    // [2*half][10][slots*chains], then the same for x
+  const SlotDesc *slots;
   uint32_t nslots, chains, half, rounds;
   uint32_t stride[30];
   uint32_t *counter;
   HitRec *hits;
   uint32_t cap;
-  uint32_t *slot_hits;
 };
 
 template <uint32_t MODE>
 __device__ __forceinline__ void batch_body(const BatchArgs &a, const SlotDesc &d, uint32_t slot,
-                                           uint32_t local, uint32_t g, uint32_t T) {
-  Point10 P = ld_point(a.state, T, g);
+                                           uint32_t local) {
+  Point10 P = ld_point(d.state, a.chains, local);
   Cached10 st;
 #pragma unroll
   for (int l = 0; l < 10; ++l) {
@@ -57,26 +61,25 @@ __device__ __forceinline__ void batch_body(const BatchArgs &a, const SlotDesc &d
     st.ymx.v[l] = a.stride[10 + l];
     st.t2d.v[l] = a.stride[20 + l];
   }
-  HitSink sink{a.counter, a.hits, a.cap, slot, a.slot_hits};
-  search_chain<MODE>(P, st, (cptr)a.table, a.scratch, T, g, a.half, a.rounds, a.chains, local,
-                     d.k_base, d.f, sink);
-  st_point(a.state, T, g, P);
+  HitSink sink{a.counter, a.hits, a.cap, slot, nullptr};
+  search_chain<MODE>(P, st, (cptr)a.table, a.scratch, a.nslots * a.chains,
+                     slot * a.chains + local, a.half, a.rounds, a.chains, local, d.k_base, d.f,
+                     sink);
+  st_point(d.state, a.chains, local, P);
 }
 
 __global__ __launch_bounds__(256) OG_SEARCH_ATTR void og_batch_kernel(BatchArgs a) {
   const uint32_t blocks_per_slot = a.chains / 256;
   const uint32_t slot = blockIdx.x / blocks_per_slot;
   const uint32_t local = (blockIdx.x % blocks_per_slot) * 256 + threadIdx.x;
-  const uint32_t T = a.nslots * a.chains;
-  const uint32_t g = slot * a.chains + local;
   const SlotDesc d = a.slots[slot];  // block-uniform
   switch (d.f.mode) {
-    case kModePrefix: batch_body<kModePrefix>(a, d, slot, local, g, T); break;
-    case kModeBitmap: batch_body<kModeBitmap>(a, d, slot, local, g, T); break;
-    case kModePattern: batch_body<kModePattern>(a, d, slot, local, g, T); break;
-    case kModePrefix | kModePattern: batch_body<kModePrefix | kModePattern>(a, d, slot, local, g, T); break;
-    case kModeBitmap | kModePattern: batch_body<kModeBitmap | kModePattern>(a, d, slot, local, g, T); break;
-    default: break;  // inactive slot
+    case kModePrefix: batch_body<kModePrefix>(a, d, slot, local); break;
+    case kModeBitmap: batch_body<kModeBitmap>(a, d, slot, local); break;
+    case kModePattern: batch_body<kModePattern>(a, d, slot, local); break;
+    case kModePrefix | kModePattern: batch_body<kModePrefix | kModePattern>(a, d, slot, local); break;
+    case kModeBitmap | kModePattern: batch_body<kModeBitmap | kModePattern>(a, d, slot, local); break;
+    default: break;  // an empty slot
   }
 }
 
@@ -84,12 +87,6 @@ __global__ __launch_bounds__(256) OG_SEARCH_ATTR void og_batch_kernel(BatchArgs 
 // is replayed many times).
 __global__ void og_batch_clear(uint32_t *counter) {
   if (threadIdx.x == 0) counter[0] = 0;
-}
-
-// Moves every active slot on by one step's candidates.
-__global__ void og_batch_advance(SlotDesc *slots, uint32_t nslots, uint64_t per_step) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < nslots && slots[s].f.mode) slots[s].k_base += per_step;
 }
 
 __global__ __launch_bounds__(256) void og_batch_setup(SetupArgs a) {
@@ -115,12 +112,15 @@ void put_err(char *err, size_t cap, const std::string &msg) {
     }                                                                                \
   } while (0)
 
-struct HostSlot {
-  bool active = false;
+struct Search {
+  bool live = false;
   uint64_t tag = 0;
+  uint64_t k_next = 0;    // next candidate index of this search
   uint64_t examined = 0;
   uint64_t hits = 0;
+  uint32_t *d_state = nullptr;
   DevFilterHost filt;
+  size_t bytes = 0;       // device bytes held (state + tables)
 };
 
 }  // namespace
@@ -132,38 +132,47 @@ struct og_batch {
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   og_batch_opts o{};
-  uint64_t per_step = 0;  // candidates one step examines per active slot
-  uint32_t *d_state = nullptr, *d_table = nullptr, *d_scratch = nullptr;
-  SlotDesc *d_slots = nullptr;
-  uint32_t *d_counter = nullptr, *d_slot_hits = nullptr;
+  uint64_t per_step = 0;  // candidates one step examines per scheduled search
+  uint32_t *d_table = nullptr, *d_scratch = nullptr;
+  SlotDesc *d_slots = nullptr, *h_slots = nullptr;
+  uint32_t *d_counter = nullptr;
   HitRec *d_hits = nullptr;
   uint32_t *h_counter = nullptr;
   HitRec *h_hits = nullptr;
-  std::vector<HostSlot> slots;
-  std::vector<uint8_t> start_cache;  // not used by the device; kept for diagnostics
+  std::vector<Search> searches;       // indexed by search id
+  std::vector<uint32_t> free_ids;
+  std::deque<uint32_t> runnable;      // round-robin order
+  std::vector<int64_t> assigned;      // search id per slot in the last step, -1 empty
   BatchArgs args;
   HostCurve curve;
   float last_ms = 0;
   uint64_t steps = 0;
-  size_t table_bytes_budget = 0, table_bytes_used = 0;
+  size_t resident_bytes = 0;
 };
 
 namespace {
+
+void drop_search(Search &s) {
+  (void)hipFree(s.d_state);
+  s.d_state = nullptr;
+  s.filt.free_device();
+  s.live = false;
+}
 
 void free_batch(og_batch *b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  for (HostSlot &s : b->slots) s.filt.free_device();
+  for (Search &s : b->searches)
+    if (s.live) drop_search(s);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  (void)hipFree(b->d_state);
   (void)hipFree(b->d_table);
   (void)hipFree(b->d_scratch);
   (void)hipFree(b->d_slots);
   (void)hipFree(b->d_counter);
-  (void)hipFree(b->d_slot_hits);
   (void)hipFree(b->d_hits);
+  (void)hipHostFree(b->h_slots);
   (void)hipHostFree(b->h_counter);
   (void)hipHostFree(b->h_hits);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -207,7 +216,7 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
   std::unique_ptr<og_batch, void (*)(og_batch *)> b(new og_batch(), free_batch);
   b->device = ordinal;
   b->o = o;
-  b->slots.resize(o.slots);
+  b->assigned.assign(o.slots, -1);
   const uint64_t S = 2ull * o.half + 1;
   b->per_step = (uint64_t)o.chains * S * o.rounds;
   const size_t T = (size_t)o.slots * o.chains;
@@ -216,27 +225,24 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
   OG_TRY(hipEventCreate(&b->ev0));
   OG_TRY(hipEventCreate(&b->ev1));
   const std::vector<uint32_t> table = b->curve.offset_table10(o.half);
-  OG_TRY(hipMalloc(&b->d_state, T * 40 * 4));
   OG_TRY(hipMalloc(&b->d_table, std::max<size_t>(table.size(), 30) * 4));
   OG_TRY(hipMalloc(&b->d_scratch, (size_t)2 * o.half * 10 * T * 2 * 4));
   OG_TRY(hipMalloc(&b->d_slots, o.slots * sizeof(SlotDesc)));
+  OG_TRY(hipHostMalloc(&b->h_slots, o.slots * sizeof(SlotDesc), hipHostMallocDefault));
   OG_TRY(hipMalloc(&b->d_counter, 4));
-  OG_TRY(hipMalloc(&b->d_slot_hits, o.slots * 4));
   OG_TRY(hipMalloc(&b->d_hits, (size_t)o.ring_cap * sizeof(HitRec)));
   OG_TRY(hipHostMalloc(&b->h_counter, 4, hipHostMallocDefault));
   OG_TRY(hipHostMalloc(&b->h_hits, (size_t)o.ring_cap * sizeof(HitRec), hipHostMallocDefault));
   if (!table.empty())
     OG_TRY(hipMemcpyAsync(b->d_table, table.data(), table.size() * 4, hipMemcpyHostToDevice,
                           b->stream));
-  std::vector<SlotDesc> zero(o.slots);
-  std::memset(zero.data(), 0, zero.size() * sizeof(SlotDesc));
-  OG_TRY(hipMemcpyAsync(b->d_slots, zero.data(), zero.size() * sizeof(SlotDesc),
+  std::memset(b->h_slots, 0, o.slots * sizeof(SlotDesc));
+  OG_TRY(hipMemcpyAsync(b->d_slots, b->h_slots, o.slots * sizeof(SlotDesc),
                         hipMemcpyHostToDevice, b->stream));
   OG_TRY(hipStreamSynchronize(b->stream));
 
   BatchArgs &a = b->args;
   std::memset(&a, 0, sizeof a);
-  a.state = b->d_state;
   a.table = b->d_table;
   a.scratch = b->d_scratch;
   a.slots = b->d_slots;
@@ -251,14 +257,11 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
   a.counter = b->d_counter;
   a.hits = b->d_hits;
   a.cap = o.ring_cap;
-  a.slot_hits = b->d_slot_hits;
 
   // Capture one step.
   OG_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(og_batch_clear, dim3(1), dim3(64), 0, b->stream, b->d_counter);
   hipLaunchKernelGGL(og_batch_kernel, dim3((unsigned)(T / 256)), dim3(256), 0, b->stream, a);
-  hipLaunchKernelGGL(og_batch_advance, dim3((o.slots + 255) / 256), dim3(256), 0, b->stream,
-                     b->d_slots, o.slots, b->per_step);
   (void)hipMemcpyAsync(b->h_counter, b->d_counter, 4, hipMemcpyDeviceToHost, b->stream);
   OG_TRY(hipStreamEndCapture(b->stream, &b->graph));
   OG_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -266,70 +269,79 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
   return 0;
 }
 
-size_t og_batch_footprint(const og_filterset *filters) {
+size_t og_batch_footprint(const og_batch *b, const og_filterset *filters) {
   const FilterSet *fs = filterset_of(filters);
   if (!fs) return 0;
   DevFilterHost h;
   if (!h.compile(*fs).empty()) return 0;
-  return h.device_bytes();
+  const size_t chains = b ? b->o.chains : 2048;
+  return h.device_bytes() + chains * 160;
 }
 
 int og_batch_admit(og_batch *b, const uint8_t start_xyzt[128], const og_filterset *filters,
-                   uint64_t tag, int *slot_out, char *err, size_t errcap) {
-  if (!b || !start_xyzt || !filters || !slot_out) {
+                   uint64_t tag, int *id_out, char *err, size_t errcap) {
+  if (!b || !start_xyzt || !filters || !id_out) {
     put_err(err, errcap, "null argument");
     return -1;
   }
-  int slot = -1;
-  for (size_t i = 0; i < b->slots.size(); ++i)
-    if (!b->slots[i].active) {
-      slot = (int)i;
-      break;
-    }
-  if (slot < 0) {
-    put_err(err, errcap, "no free slot");
-    return -2;
+  uint32_t id;
+  if (!b->free_ids.empty()) {
+    id = b->free_ids.back();
+    b->free_ids.pop_back();
+  } else {
+    id = (uint32_t)b->searches.size();
+    b->searches.emplace_back();
   }
-  HostSlot &hs = b->slots[slot];
-  hs = HostSlot();
-  std::string msg = hs.filt.compile(*filterset_of(filters));
+  Search &s = b->searches[id];
+  s = Search();
+  std::string msg = s.filt.compile(*filterset_of(filters));
   if (!msg.empty()) {
+    b->free_ids.push_back(id);
     put_err(err, errcap, msg);
     return -1;
   }
   OG_TRY(hipSetDevice(b->device));
-  OG_TRY(hs.filt.to_device(b->stream));
-  SlotDesc d;
-  std::memset(&d, 0, sizeof d);
-  d.f = hs.filt.args();
-  d.k_base = 0;
-  SetupArgs sa = b->curve.setup_args(start_xyzt, b->d_state, b->o.slots * b->o.chains,
-                                     (uint32_t)slot * b->o.chains, b->o.chains, b->o.half);
+  hipError_t e = hipMalloc(&s.d_state, (size_t)b->o.chains * 160);
+  if (e == hipSuccess) e = s.filt.to_device(b->stream);
+  if (e != hipSuccess) {
+    drop_search(s);
+    b->free_ids.push_back(id);
+    put_err(err, errcap, std::string("admission: ") + hipGetErrorString(e));
+    return -1;
+  }
+  SetupArgs sa = b->curve.setup_args(start_xyzt, s.d_state, b->o.chains, 0, b->o.chains,
+                                     b->o.half);
   hipLaunchKernelGGL(og_batch_setup, dim3(b->o.chains / 256), dim3(256), 0, b->stream, sa);
-  OG_TRY(hipGetLastError());
-  uint32_t zero = 0;
-  OG_TRY(hipMemcpyAsync(b->d_slot_hits + slot, &zero, 4, hipMemcpyHostToDevice, b->stream));
-  OG_TRY(hipMemcpyAsync(b->d_slots + slot, &d, sizeof d, hipMemcpyHostToDevice, b->stream));
-  OG_TRY(hipStreamSynchronize(b->stream));
-  hs.active = true;
-  hs.tag = tag;
-  *slot_out = slot;
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) {
+    drop_search(s);
+    b->free_ids.push_back(id);
+    put_err(err, errcap, std::string("chain set-up: ") + hipGetErrorString(e));
+    return -1;
+  }
+  s.live = true;
+  s.tag = tag;
+  s.bytes = (size_t)b->o.chains * 160 + s.filt.device_bytes();
+  b->resident_bytes += s.bytes;
+  b->runnable.push_back(id);
+  *id_out = (int)id;
   return 0;
 }
 
-int og_batch_retire(og_batch *b, int slot) {
-  if (!b || slot < 0 || (size_t)slot >= b->slots.size()) return -1;
-  HostSlot &hs = b->slots[slot];
-  if (!hs.active) return 0;
+int og_batch_retire(og_batch *b, int id) {
+  if (!b || id < 0 || (size_t)id >= b->searches.size()) return -1;
+  Search &s = b->searches[id];
+  if (!s.live) return 0;
   (void)hipSetDevice(b->device);
-  SlotDesc d;
-  std::memset(&d, 0, sizeof d);
-  if (hipMemcpyAsync(b->d_slots + slot, &d, sizeof d, hipMemcpyHostToDevice, b->stream) !=
-      hipSuccess)
-    return -1;
   if (hipStreamSynchronize(b->stream) != hipSuccess) return -1;
-  hs.filt.free_device();
-  hs.active = false;
+  b->resident_bytes -= s.bytes;
+  drop_search(s);
+  b->runnable.erase(std::remove(b->runnable.begin(), b->runnable.end(), (uint32_t)id),
+                    b->runnable.end());
+  for (int64_t &a : b->assigned)
+    if (a == id) a = -1;
+  b->free_ids.push_back((uint32_t)id);
   return 0;
 }
 
@@ -341,14 +353,33 @@ int og_batch_step(og_batch *b, og_batch_hit *out, uint32_t cap, uint32_t *n, cha
   }
   *n = 0;
   OG_TRY(hipSetDevice(b->device));
+  // Round-robin: the next `slots` resident searches take the slots.
+  const size_t take = std::min<size_t>(b->o.slots, b->runnable.size());
+  std::memset(b->h_slots, 0, b->o.slots * sizeof(SlotDesc));
+  for (size_t s = 0; s < b->o.slots; ++s) b->assigned[s] = -1;
+  for (size_t s = 0; s < take; ++s) {
+    const uint32_t id = b->runnable.front();
+    b->runnable.pop_front();
+    b->runnable.push_back(id);
+    Search &sr = b->searches[id];
+    b->assigned[s] = id;
+    b->h_slots[s].f = sr.filt.args();
+    b->h_slots[s].k_base = sr.k_next;
+    b->h_slots[s].state = sr.d_state;
+  }
+  OG_TRY(hipMemcpyAsync(b->d_slots, b->h_slots, b->o.slots * sizeof(SlotDesc),
+                        hipMemcpyHostToDevice, b->stream));
   OG_TRY(hipEventRecord(b->ev0, b->stream));
   OG_TRY(hipGraphLaunch(b->exec, b->stream));
   OG_TRY(hipEventRecord(b->ev1, b->stream));
   OG_TRY(hipStreamSynchronize(b->stream));
   OG_TRY(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
   b->steps++;
-  for (HostSlot &s : b->slots)
-    if (s.active) s.examined += b->per_step;
+  for (size_t s = 0; s < take; ++s) {
+    Search &sr = b->searches[b->assigned[s]];
+    sr.k_next += b->per_step;
+    sr.examined += b->per_step;
+  }
   const uint32_t found = *b->h_counter;
   const uint32_t kept = std::min(found, b->o.ring_cap);
   if (kept) {
@@ -366,32 +397,37 @@ int og_batch_step(og_batch *b, og_batch_hit *out, uint32_t cap, uint32_t *n, cha
                              std::to_string(kept) + " were found");
     return -3;
   }
+  uint32_t m = 0;
   for (uint32_t i = 0; i < kept; ++i) {
     const HitRec &h = b->h_hits[i];
-    og_batch_hit &o = out[i];
-    o.slot = h.slot;
-    o.tag = h.slot < b->slots.size() ? b->slots[h.slot].tag : 0;
+    if (h.slot >= b->o.slots || b->assigned[h.slot] < 0) continue;
+    Search &sr = b->searches[b->assigned[h.slot]];
+    og_batch_hit &o = out[m++];
+    o.search = (uint32_t)b->assigned[h.slot];
+    o.tag = sr.tag;
     o.offset = h.k * 8;
     for (int l = 0; l < 8; ++l)
       for (int k = 0; k < 4; ++k) o.packed[4 * l + k] = (uint8_t)(h.y[l] >> (8 * k));
-    if (h.slot < b->slots.size()) b->slots[h.slot].hits++;
+    sr.hits++;
   }
-  *n = kept;
+  *n = m;
   return 0;
 }
 
-int og_batch_slot_state(const og_batch *b, int slot, uint64_t *tag, uint64_t *examined,
-                        uint64_t *hits, int *active) {
-  if (!b || slot < 0 || (size_t)slot >= b->slots.size()) return -1;
-  const HostSlot &s = b->slots[slot];
+int og_batch_search_state(const og_batch *b, int id, uint64_t *tag, uint64_t *examined,
+                          uint64_t *hits, int *live) {
+  if (!b || id < 0 || (size_t)id >= b->searches.size()) return -1;
+  const Search &s = b->searches[id];
   if (tag) *tag = s.tag;
   if (examined) *examined = s.examined;
   if (hits) *hits = s.hits;
-  if (active) *active = s.active ? 1 : 0;
+  if (live) *live = s.live ? 1 : 0;
   return 0;
 }
 
 uint64_t og_batch_per_step(const og_batch *b) { return b ? b->per_step : 0; }
+
+uint64_t og_batch_resident_bytes(const og_batch *b) { return b ? b->resident_bytes : 0; }
 
 double og_batch_last_ms(const og_batch *b) { return b ? b->last_ms : 0.0; }
 

@@ -1,12 +1,14 @@
 // The serve loop: searches arrive over time and are served by one slot engine.
 //
-// Continuous batching: every iteration admits queued requests into free slots
-// (first come, first served, while the device-table budget allows), runs one
-// batch step for everything admitted, credits each survivor to its request,
-// and retires a request the step its n-th hit arrives, freeing the slot for
-// the next one.  A request waits in a bounded queue; an arrival that finds the
-// queue full is rejected (the back-pressure policy).  Chain state and filter
-// tables stay on the device for a request's whole life.
+// Continuous batching: an arriving request becomes resident on the device at
+// once (its chain state and filter tables are set up) when its footprint fits
+// the device budget, else it waits in a bounded FIFO queue, and an arrival
+// that finds the queue full is rejected (the back-pressure policy).  Every
+// iteration runs one batch step, in which the resident requests take the
+// slots round-robin, credits each survivor to its request, and retires a
+// request the step its n-th hit arrives, which frees its memory for the queue.
+// A request's chain state and filter tables stay on the device for its whole
+// life; no request waits for another to finish.
 //
 // There is no serving layer in the reference; its nearest analogue is the
 // master's lease queue (src/master/queue.rs:51-175).  Each request's search is
@@ -42,7 +44,6 @@ double percentile(std::vector<double> v, double q) {
 
 struct Live {
   size_t req;
-  int slot;
   og_filterset *fs;
   size_t bytes;
 };
@@ -52,7 +53,7 @@ struct Live {
 extern "C" {
 
 int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs, size_t n,
-                 double duration_s, uint32_t max_queue, uint64_t table_budget,
+                 double duration_s, uint32_t max_queue, uint64_t device_budget,
                  og_request_result *results, og_serve_stats *stats, char *err, size_t errcap) {
   if (!reqs || !results || !stats) {
     put_err(err, errcap, "null argument");
@@ -66,8 +67,8 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
   og_batch *b = nullptr;
   if (og_batch_create(ordinal, opts, &b, err, errcap) != 0) return -1;
   og_batch_opts o = opts ? *opts : og_batch_opts{};
-  const uint32_t nslots = o.slots ? o.slots : 64;
   const uint32_t ring = o.ring_cap ? o.ring_cap : (1u << 16);
+  const uint64_t budget = device_budget ? device_budget : (4ull << 30);
   std::vector<og_batch_hit> hits(ring);
   for (size_t i = 0; i < n; ++i) {
     results[i] = og_request_result{};
@@ -75,8 +76,9 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
     results[i].status = OG_REQ_NOT_ARRIVED;
   }
   std::deque<size_t> queue;
-  std::vector<Live> live(nslots, Live{0, -1, nullptr, 0});
-  uint64_t table_used = 0;
+  std::vector<Live> live;          // indexed by search id
+  size_t resident = 0, max_resident = 0;
+  uint64_t used = 0;
   size_t next = 0;
   uint64_t steps = 0, examined = 0;
   double busy_ms = 0.0;
@@ -85,19 +87,9 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
   auto now = [&] {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   };
-  for (;;) {
-    const double t = now();
-    if (t >= duration_s) break;
-    // admission: first come, first served, while a slot and the budget allow
-    auto admit_pending = [&] {
+  // admission: first come, first served, while the footprint fits the budget
+  auto admit_pending = [&] {
     while (!queue.empty()) {
-      int free_slot = -1;
-      for (uint32_t s = 0; s < nslots; ++s)
-        if (live[s].slot < 0) {
-          free_slot = (int)s;
-          break;
-        }
-      if (free_slot < 0) break;
       const size_t r = queue.front();
       char e2[256];
       og_filterset *fs = og_filters_parse(reqs[r].filter, 0, e2, sizeof e2);
@@ -106,44 +98,48 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
         queue.pop_front();
         continue;
       }
-      const size_t bytes = og_batch_footprint(fs);
-      if (table_budget && table_used + bytes > table_budget) {
+      const size_t bytes = og_batch_footprint(b, fs);
+      if (used + bytes > budget) {
         og_filters_free(fs);
-        break;  // wait for a running request to free device memory
+        break;  // wait for a resident request to free device memory
       }
       uint8_t start[128];
       og_key_start_point(reqs[r].seed, start);
-      int slot = -1;
-      if (og_batch_admit(b, start, fs, (uint64_t)r, &slot, e2, sizeof e2) != 0) {
+      int id = -1;
+      if (og_batch_admit(b, start, fs, (uint64_t)r, &id, e2, sizeof e2) != 0) {
         og_filters_free(fs);
         results[r].status = OG_REQ_REJECTED;
         queue.pop_front();
         continue;
       }
       queue.pop_front();
-      live[slot] = Live{r, slot, fs, bytes};
-      table_used += bytes;
+      if ((size_t)id >= live.size()) live.resize(id + 1, Live{0, nullptr, 0});
+      live[id] = Live{r, fs, bytes};
+      used += bytes;
+      resident++;
+      max_resident = std::max(max_resident, resident);
       results[r].status = OG_REQ_RUNNING;
       results[r].admitted_s = now();
     }
-    };
-    // arrivals: each is admitted at once when a slot is free, else queued
-    // while the queue has room, else rejected
+  };
+  for (;;) {
+    const double t = now();
+    if (t >= duration_s) break;
     while (next < n && reqs[next].arrival_s <= t) {
-      if (queue.size() >= max_queue) {
-        results[next].status = OG_REQ_REJECTED;
-      } else {
-        results[next].status = OG_REQ_QUEUED;
-        queue.push_back(next);
-        admit_pending();
+      // the arrival joins the FIFO and admission runs; only requests still
+      // waiting after that count against the queue bound
+      results[next].status = OG_REQ_QUEUED;
+      queue.push_back(next);
+      admit_pending();
+      if (queue.size() > max_queue) {
+        results[queue.back()].status = OG_REQ_REJECTED;
+        queue.pop_back();
       }
       ++next;
     }
     admit_pending();
-    bool any = false;
-    for (const Live &l : live) any = any || l.slot >= 0;
-    if (!any) {
-      if (next >= n && queue.empty()) break;  // nothing running, nothing left to arrive
+    if (resident == 0) {
+      if (next >= n && queue.empty()) break;  // nothing resident, nothing left to arrive
       if (next < n) {
         const double wait = std::min(reqs[next].arrival_s - now(), 0.001);
         if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
@@ -158,15 +154,12 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
     steps++;
     busy_ms += og_batch_last_ms(b);
     const double t_step = now();
-    for (uint32_t s = 0; s < nslots; ++s)
-      if (live[s].slot >= 0) {
-        examined += og_batch_per_step(b);
-        results[live[s].req].examined += og_batch_per_step(b);
-      }
+    const uint64_t scheduled = std::min<uint64_t>(resident, o.slots ? o.slots : 64);
+    examined += scheduled * og_batch_per_step(b);
     for (uint32_t i = 0; i < nh; ++i) {
       const og_batch_hit &h = hits[i];
-      if (h.slot >= nslots || live[h.slot].slot < 0) continue;
-      og_request_result &res = results[live[h.slot].req];
+      if (h.search >= live.size() || !live[h.search].fs) continue;
+      og_request_result &res = results[live[h.search].req];
       if (res.hits == 0) res.first_hit_s = t_step;
       if (res.hits < 16) {
         res.found[res.hits].offset = h.offset;
@@ -174,30 +167,35 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
       }
       res.hits++;
     }
-    // completion: retire the slot the step its n-th hit arrives
-    for (uint32_t s = 0; s < nslots; ++s) {
-      Live &l = live[s];
-      if (l.slot < 0) continue;
+    // completion: retire a request the step its n-th hit arrives
+    for (size_t id = 0; id < live.size(); ++id) {
+      Live &l = live[id];
+      if (!l.fs) continue;
       og_request_result &res = results[l.req];
+      uint64_t ex = 0;
+      og_batch_search_state(b, (int)id, nullptr, &ex, nullptr, nullptr);
+      res.examined = ex;
       if (res.hits >= reqs[l.req].want_hits) {
         res.done_s = t_step;
         res.status = OG_REQ_DONE;
-        og_batch_retire(b, l.slot);
+        og_batch_retire(b, (int)id);
         og_filters_free(l.fs);
-        table_used -= l.bytes;
-        l = Live{0, -1, nullptr, 0};
+        used -= l.bytes;
+        resident--;
+        l = Live{0, nullptr, 0};
       }
     }
   }
   const double wall = now();
   for (Live &l : live)
-    if (l.slot >= 0) og_filters_free(l.fs);
+    if (l.fs) og_filters_free(l.fs);
   og_batch_destroy(b);
 
   og_serve_stats st{};
   st.wall_s = wall;
   st.steps = steps;
   st.examined = examined;
+  st.max_resident = (uint32_t)max_resident;
   st.keys_per_s = wall > 0 ? (double)examined / wall : 0.0;
   st.device_busy_s = busy_ms * 1e-3;
   std::vector<double> ttn, ttf;

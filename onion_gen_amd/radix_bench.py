@@ -16,8 +16,8 @@ from . import gpu
 
 P = 2 ** 255 - 19
 VERSION = "0.1.0"
-RADIXES = {8: "8 x 32 saturated, row-wise", 81: "8 x 32 saturated, carry-flag product scan",
-           9: "9 x 29, carry-free columns", 10: "10 x 25.5 (ref10 shape)"}
+RADIXES = {8: "8 x 32 saturated, row-wise", 9: "9 x 29, carry-free columns",
+           10: "10 x 25.5 (ref10 shape)"}
 
 
 def radix_input(lane, which, limb):

@@ -24,7 +24,8 @@ class Result(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("wall_s", ctypes.c_double), ("keys_per_s", ctypes.c_double),
                 ("device_busy_s", ctypes.c_double), ("examined", ctypes.c_uint64),
-                ("steps", ctypes.c_uint64), ("completed", ctypes.c_uint32),
+                ("steps", ctypes.c_uint64), ("max_resident", ctypes.c_uint32),
+                ("completed", ctypes.c_uint32),
                 ("running", ctypes.c_uint32), ("queued", ctypes.c_uint32),
                 ("rejected", ctypes.c_uint32), ("not_arrived", ctypes.c_uint32),
                 ("p50_time_to_n", ctypes.c_double), ("p99_time_to_n", ctypes.c_double),
@@ -35,7 +36,7 @@ STATUS = {0: "not_arrived", 1: "queued", 2: "running", 3: "done", 4: "rejected"}
 
 
 def run(requests, duration, slots=64, chains=2048, half=256, rounds=1, max_queue=256,
-        table_budget=0, device=0):
+        device_budget=0, device=0):
     """``requests``: list of (arrival_s, seed32, filter_text, want_hits), sorted
     by arrival.  Returns (stats dict, list of result dicts)."""
     n = len(requests)
@@ -49,7 +50,7 @@ def run(requests, duration, slots=64, chains=2048, half=256, rounds=1, max_queue
     stats = Stats()
     e = err_buf()
     rc = lib().og_serve_run(device, ctypes.byref(_Opts(slots, chains, half, rounds, 0)), reqs, n,
-                            duration, max_queue, table_budget, results, ctypes.byref(stats), e,
+                            duration, max_queue, device_budget, results, ctypes.byref(stats), e,
                             len(e))
     if rc != 0:
         raise OgError(e.value.decode())

@@ -37,18 +37,17 @@ def test_mixed_batch_matches_each_search_alone():
     b = batch.Batch(slots=4, chains=512, half=64, rounds=1)
     seeds = {tag: hashlib.sha256(tag).digest() for tag, _ in MIX}
     fsets = {tag: filters.FilterSet.parse(lines) for tag, lines in MIX}
-    slots = {}
+    ids = {}
     for i, (tag, lines) in enumerate(MIX):
-        slots[tag] = b.admit(key.start_point(seeds[tag]), fsets[tag], i)
-    assert sorted(slots.values()) == [0, 1, 2, 3]
-    assert b.admit(key.start_point(b"x" * 32), fsets[b"prefix"], 99) is None  # full
+        ids[tag] = b.admit(key.start_point(seeds[tag]), fsets[tag], i)
+    assert sorted(ids.values()) == [0, 1, 2, 3]
     got = {tag: [] for tag, _ in MIX}
     n = 1 << 18
     steps = -(-n // b.per_step())
     for _ in range(steps):
         for h in b.step():
             tag = MIX[h.tag][0]
-            assert h.slot == slots[tag]
+            assert h.search == ids[tag]
             if h.offset // 8 < n:
                 got[tag].append((h.offset // 8, h.packed))
     for tag, lines in MIX:
@@ -67,20 +66,20 @@ def test_mixed_batch_matches_each_search_alone():
 
 
 def test_retire_and_readmit_starts_a_search_from_zero():
-    say("case: retire slot, admit a new search into it")
+    say("case: retire a search, admit a new one in its place")
     b = batch.Batch(slots=2, chains=256, half=32, rounds=1)
     s1, s2 = hashlib.sha256(b"one").digest(), hashlib.sha256(b"two").digest()
     f = filters.FilterSet.parse("2x")
-    slot_a = b.admit(key.start_point(s1), f, 1)
-    slot_b = b.admit(key.start_point(s2), f, 2)
+    id_a = b.admit(key.start_point(s1), f, 1)
+    id_b = b.admit(key.start_point(s2), f, 2)
     for _ in range(3):
         b.step()
-    assert b.slot_state(slot_a)["examined"] == 3 * b.per_step()
-    b.retire(slot_a)
-    assert not b.slot_state(slot_a)["active"]
+    assert b.search_state(id_a)["examined"] == 3 * b.per_step()
+    b.retire(id_a)
+    assert not b.search_state(id_a)["live"]
     s3 = hashlib.sha256(b"three").digest()
-    slot_c = b.admit(key.start_point(s3), f, 3)
-    assert slot_c == slot_a
+    id_c = b.admit(key.start_point(s3), f, 3)
+    assert id_c == id_a
     n = 1 << 16
     got = {2: [], 3: []}
     steps = -(-n // b.per_step())
@@ -94,4 +93,25 @@ def test_retire_and_readmit_starts_a_search_from_zero():
     want2 = oracle(s2, ["2x"], 3 * b.per_step() + n)
     assert sorted(x for x in got[2] if x[0] >= 3 * b.per_step()) == \
         [x for x in want2 if x[0] >= 3 * b.per_step()]
+    b.close()
+
+
+def test_more_searches_than_slots_share_the_device_round_robin():
+    """Six resident searches on two slots: each step runs two of them, every
+    search advances in turn, and each one's hits over what it examined are
+    exactly its oracle's."""
+    say("case: 6 resident searches on 2 slots x 256 chains, half 32")
+    b = batch.Batch(slots=2, chains=256, half=32, rounds=1)
+    f = filters.FilterSet.parse(["2x", "suffix:aqd"])
+    seeds = [hashlib.sha256(b"rr%d" % i).digest() for i in range(6)]
+    ids = [b.admit(key.start_point(s), f, i) for i, s in enumerate(seeds)]
+    got = {i: [] for i in range(6)}
+    for _ in range(9):  # three turns each
+        for h in b.step():
+            got[h.tag].append((h.offset // 8, h.packed))
+    for i, sid in enumerate(ids):
+        st = b.search_state(sid)
+        assert st["examined"] == 3 * b.per_step()
+        want = oracle(seeds[i], ["2x", "suffix:aqd"], st["examined"])
+        assert sorted(got[i]) == want and want, i
     b.close()

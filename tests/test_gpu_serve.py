@@ -6,7 +6,7 @@ import re
 
 import pytest
 
-from onion_gen_amd import key, serve, workload
+from onion_gen_amd import batch, filters, key, serve, workload
 from tests.oracle_lib import OracleFilters, search
 
 pytestmark = pytest.mark.gpu
@@ -52,11 +52,39 @@ def test_requests_complete_with_real_hits():
 
 
 def test_bounded_queue_rejects_overflow():
-    say("case: serve 10 simultaneous arrivals, 1 slot, queue of 2")
+    """A device budget that holds one search: the first arrival runs, two wait
+    in the queue of 2, the other seven are turned away."""
+    say("case: serve 10 simultaneous arrivals, budget for one search, queue of 2")
     reqs = [(0.0, hashlib.sha256(b"q%d" % i).digest(), "2", 1) for i in range(10)]
-    st, res = serve.run(reqs, duration=30.0, slots=1, chains=256, half=32, max_queue=2)
+    with batch.Batch(slots=1, chains=256, half=32) as b:
+        one = b.footprint(filters.FilterSet.parse("2"))
+    assert one > 0
+    st, res = serve.run(reqs, duration=30.0, slots=1, chains=256, half=32, max_queue=2,
+                        device_budget=one)
     assert st["rejected"] == 7 and st["completed"] == 3, st
+    assert st["max_resident"] == 1, st
     assert [r["status"] for r in res].count("rejected") == 7
+
+
+def test_resident_searches_outnumber_slots():
+    """With room on the device every arrival becomes resident at once and the
+    slots are time-shared: ten searches on two slots all finish."""
+    say("case: serve 10 simultaneous arrivals, 2 slots, ample budget")
+    reqs = [(0.0, hashlib.sha256(b"r%d" % i).digest(), "2xe", 2) for i in range(10)]
+    # a queue of 0: nothing may wait, so every arrival must fit at once
+    st, res = serve.run(reqs, duration=60.0, slots=2, chains=256, half=32, max_queue=0)
+    assert st["completed"] == 10 and st["rejected"] == 0, st
+    assert st["max_resident"] == 10, st
+    for (_, seed, text, want), r in zip(reqs, res):
+        assert r["status"] == "done" and r["hits"] >= want
+        got = sorted((o // 8, p) for o, p in r["found"])
+        oracle_hits = search(key.start_point(seed), 0, r["examined"], 256, OracleFilters(text))
+        want_hits = [(k, clear_sign(pk)) for k, pk, _ in oracle_hits]
+        assert len(want_hits) == r["hits"]
+        # found keeps the first 16 hits in ring order, not the 16 smallest k
+        assert set(got) <= set(want_hits)
+        if r["hits"] <= 16:
+            assert got == want_hits
 
 
 def test_mix_generators_are_reachable_filters():
