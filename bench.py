@@ -28,13 +28,14 @@ VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 
 def field_muls_per_candidate(half, with_x=False):
     """Field multiplications (squarings included) the kernel performs per
-    candidate: forward 7/pair (+4 for x), backward 5/pair (+2 for x), the centre
-    (3, +2 for x), one inversion (254 sq + 11 mul) and the stride addition (7)
-    per window of 2*half+1 candidates.  Priced at 64 32x32-bit products each,
-    the radix-independent count of an 8 x 32 schoolbook."""
-    pair = 12 + (6 if with_x else 0)
-    window = pair * half + 3 + (2 if with_x else 0) + 265 + 7
-    return window / (2 * half + 1)
+    candidate: 3 per pair forward (C = T dq and two prefix products), 8 per
+    pair backward (two inverses peeled, two products A and B, two y), the
+    centre (4), one inversion (254 sq + 11 mul) and the stride addition (7)
+    per window of 2*half+1 candidates; +2 per candidate when the sign of x is
+    needed.  Priced at 64 32x32-bit products each, the radix-independent count
+    of an 8 x 32 schoolbook."""
+    window = 11 * half + 4 + 265 + 7
+    return window / (2 * half + 1) + (2 if with_x else 0)
 
 
 def cpu_baseline(prefix, seconds):

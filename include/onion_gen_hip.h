@@ -104,8 +104,7 @@ int og_field_op_host(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, s
 /* The radix bench (no counterpart in the reference; it decides the field
  * representation for gfx950): `threads` lanes each run two chains of `iters`
  * dependent multiplications in radix 8 (8 x 32 saturated limbs, row-wise),
- * 81 (8 x 32, product scanning on the multiply-add carry flag), 9 (9 x 29
- * limbs, carry-free columns) or 10 (10 x 25.5 limbs, the shape of
+ * 9 (9 x 29 limbs, carry-free columns) or 10 (10 x 25.5 limbs, the shape of
  * src/field4.rs:568-670).  Writes field
  * multiplications per second and, when `results` is non-null, the 2*threads
  * canonical chain ends (64 bytes per lane). */
@@ -133,6 +132,15 @@ int og_filters_match_key(const og_filterset *s, const uint8_t packed[32]);
 /* Replaces FilterSet::match_whole_address (src/filter.rs:1673-1691): index of
  * a filter the 35 address bytes match, or -1. */
 int og_filters_match_address(const og_filterset *s, const uint8_t address[35]);
+/* The device matcher of the general forms (suffix, contains, class prefixes)
+ * run on the host for one key: the two stages of FilterSet::examine_key
+ * (src/filter.rs:1625-1691) with Filter::key_prefilter (:782-810).  packed is
+ * the canonical y (sign bit clear), sign the sign of x.  key_hit: a placement
+ * inside symbols 0..48 matched; need: the key pass left a placement reaching
+ * past symbol 48 possible; hit: the verdict on the whole address.  Returns 0,
+ * or -1 when the set has no general filter or does not compile. */
+int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int sign,
+                         int *key_hit, int *need, int *hit);
 
 /* ------------------------------------------------------------------ batch */
 

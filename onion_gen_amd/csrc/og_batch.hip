@@ -41,7 +41,7 @@ struct SlotDesc {
 
 struct BatchArgs {
   const uint32_t *table;
-  uint32_t *scratch;    // [2*half][10][slots*chains], then the same for x
+  uint32_t *scratch;    // [kPlanesPerPair*half][10][slots*chains]
   const SlotDesc *slots;
   uint32_t nslots, chains, half, rounds;
   uint32_t stride[30];
@@ -52,7 +52,7 @@ struct BatchArgs {
 
 template <uint32_t MODE>
 __device__ __forceinline__ void batch_body(const BatchArgs &a, const SlotDesc &d, uint32_t slot,
-                                           uint32_t local) {
+                                           uint32_t local, const ScanBank *banks) {
   Point10 P = ld_point(d.state, a.chains, local);
   Cached10 st;
 #pragma unroll
@@ -64,7 +64,7 @@ __device__ __forceinline__ void batch_body(const BatchArgs &a, const SlotDesc &d
   HitSink sink{a.counter, a.hits, a.cap, slot, nullptr};
   search_chain<MODE>(P, st, (cptr)a.table, a.scratch, a.nslots * a.chains,
                      slot * a.chains + local, a.half, a.rounds, a.chains, local, d.k_base, d.f,
-                     sink);
+                     banks, sink);
   st_point(d.state, a.chains, local, P);
 }
 
@@ -73,12 +73,18 @@ __global__ __launch_bounds__(256) OG_SEARCH_ATTR void og_batch_kernel(BatchArgs 
   const uint32_t slot = blockIdx.x / blocks_per_slot;
   const uint32_t local = (blockIdx.x % blocks_per_slot) * 256 + threadIdx.x;
   const SlotDesc d = a.slots[slot];  // block-uniform
+  __shared__ ScanBank banks[kMaxBanks];
+  if (d.f.mode & kModePattern) load_banks(banks, d.f);
   switch (d.f.mode) {
-    case kModePrefix: batch_body<kModePrefix>(a, d, slot, local); break;
-    case kModeBitmap: batch_body<kModeBitmap>(a, d, slot, local); break;
-    case kModePattern: batch_body<kModePattern>(a, d, slot, local); break;
-    case kModePrefix | kModePattern: batch_body<kModePrefix | kModePattern>(a, d, slot, local); break;
-    case kModeBitmap | kModePattern: batch_body<kModeBitmap | kModePattern>(a, d, slot, local); break;
+    case kModePrefix: batch_body<kModePrefix>(a, d, slot, local, banks); break;
+    case kModeBitmap: batch_body<kModeBitmap>(a, d, slot, local, banks); break;
+    case kModePattern: batch_body<kModePattern>(a, d, slot, local, banks); break;
+    case kModePrefix | kModePattern:
+      batch_body<kModePrefix | kModePattern>(a, d, slot, local, banks);
+      break;
+    case kModeBitmap | kModePattern:
+      batch_body<kModeBitmap | kModePattern>(a, d, slot, local, banks);
+      break;
     default: break;  // an empty slot
   }
 }
@@ -226,7 +232,7 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
   OG_TRY(hipEventCreate(&b->ev1));
   const std::vector<uint32_t> table = b->curve.offset_table10(o.half);
   OG_TRY(hipMalloc(&b->d_table, std::max<size_t>(table.size(), 30) * 4));
-  OG_TRY(hipMalloc(&b->d_scratch, (size_t)2 * o.half * 10 * T * 2 * 4));
+  OG_TRY(hipMalloc(&b->d_scratch, (size_t)kPlanesPerPair * o.half * 10 * T * 4));
   OG_TRY(hipMalloc(&b->d_slots, o.slots * sizeof(SlotDesc)));
   OG_TRY(hipHostMalloc(&b->h_slots, o.slots * sizeof(SlotDesc), hipHostMallocDefault));
   OG_TRY(hipMalloc(&b->d_counter, 4));

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -13,6 +14,51 @@
 
 namespace og {
 
+// The general filters of a set packed into automaton banks (og_scan.hpp):
+// first fit, each pattern on consecutive bits of one bank.  Empty string on
+// success.
+inline std::string build_banks(const FilterSet &fs, std::vector<ScanBank> &out) {
+  out.clear();
+  std::vector<int> used;
+  for (uint32_t idx : fs.general) {
+    const Filter &f = fs.filters[idx];
+    const int len = (int)f.sets.size();
+    const int kept = std::min(len, kBankBits);
+    size_t k = 0;
+    while (k < out.size() && used[k] + kept > kBankBits) ++k;
+    if (k == out.size()) {
+      if (out.size() == (size_t)kMaxBanks) return "the general filters need more than 8 automaton banks";
+      ScanBank b;
+      std::memset(&b, 0, sizeof b);
+      b.jstart = kAddressLen;
+      out.push_back(b);
+      used.push_back(0);
+    }
+    ScanBank &b = out[k];
+    const int off = used[k];
+    used[k] += kept;
+    for (int i = 0; i < kept; ++i)
+      for (int c = 0; c < 32; ++c)
+        if ((f.sets[i] >> c) & 1u) b.b[c] |= 1u << (off + i);
+    const uint32_t first = 1u << off, last = 1u << (off + kept - 1);
+    if (f.anchor == kStart) b.istart |= first;
+    else b.iany |= first;
+    if (len > kBankBits) b.flong |= last;
+    else if (f.anchor == kEnd) b.fend |= last;
+    else b.fany |= last;
+    const uint32_t js = f.anchor == kEnd ? (uint32_t)(kAddressLen - len) : 0u;
+    b.jstart = std::min(b.jstart, js);
+  }
+  for (ScanBank &b : out) {
+    for (int c = 0; c < 32; ++c) {
+      b.u51[c >> 4] |= b.b[c];
+      b.uall |= b.b[c];
+    }
+    b.u54 = b.b[0] | b.b[8] | b.b[16] | b.b[24];  // a, i, q, y
+  }
+  return "";
+}
+
 struct DevFilterHost {
   uint32_t mode = 0;
   uint32_t want[8] = {0}, mask[8] = {0};
@@ -21,12 +67,14 @@ struct DevFilterHost {
   std::vector<uint32_t> buckets;  // [offset, count, mask_lo, mask_hi]
   std::vector<uint64_t> keys;
   std::vector<uint32_t> pats;     // kPatWords per pattern
+  std::vector<ScanBank> banks;    // the automaton over the same patterns
   // device copies
-  uint32_t *d_bitmap = nullptr, *d_buckets = nullptr, *d_pats = nullptr;
+  uint32_t *d_bitmap = nullptr, *d_buckets = nullptr, *d_pats = nullptr, *d_banks = nullptr;
   uint64_t *d_keys = nullptr;
 
   size_t device_bytes() const {
-    return bitmap.size() * 4 + buckets.size() * 4 + keys.size() * 8 + pats.size() * 4;
+    return bitmap.size() * 4 + buckets.size() * 4 + keys.size() * 8 + pats.size() * 4 +
+           banks.size() * sizeof(ScanBank);
   }
 
   // Empty string on success.
@@ -58,7 +106,7 @@ struct DevFilterHost {
     }
     if (!fs.general.empty()) {
       if (fs.general.size() > (size_t)kMaxPatterns)
-        return "at most 8 general (suffix, contains, class) filters per search";
+        return "at most 64 general (suffix, contains, class) filters per search";
       mode |= kModePattern;
       for (uint32_t i : fs.general) {
         DevPattern p = dev_pattern(fs.filters[i]);
@@ -68,6 +116,8 @@ struct DevFilterHost {
         pats.push_back(p.key_last);
         for (int s = 0; s < kAddressLen; ++s) pats.push_back(p.sets[s]);
       }
+      const std::string e = build_banks(fs, banks);
+      if (!e.empty()) return e;
     }
     if (!mode) return "the filter set is empty";
     return "";
@@ -94,6 +144,13 @@ struct DevFilterHost {
     if (e == hipSuccess) e = upload(&d_buckets, buckets, s);
     if (e == hipSuccess) e = upload(&d_keys, keys, s);
     if (e == hipSuccess) e = upload(&d_pats, pats, s);
+    if (e == hipSuccess && !banks.empty()) {
+      std::vector<uint32_t> words(banks.size() * kBankWords);
+      std::memcpy(words.data(), banks.data(), words.size() * 4);
+      e = upload(&d_banks, words, s);
+      // the staging vector dies here: wait for the copy that reads it
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
     return e;
   }
 
@@ -102,7 +159,8 @@ struct DevFilterHost {
     (void)hipFree(d_buckets);
     (void)hipFree(d_keys);
     (void)hipFree(d_pats);
-    d_bitmap = d_buckets = d_pats = nullptr;
+    (void)hipFree(d_banks);
+    d_bitmap = d_buckets = d_pats = d_banks = nullptr;
     d_keys = nullptr;
   }
 
@@ -117,6 +175,8 @@ struct DevFilterHost {
     a.nbuckets = (uint32_t)(buckets.size() / 4);
     a.buckets = d_buckets;
     a.keys = d_keys;
+    a.nbanks = (uint32_t)banks.size();
+    a.banks = d_banks;
     a.npat = (uint32_t)(pats.size() / kPatWords);
     a.pats = d_pats;
     return a;

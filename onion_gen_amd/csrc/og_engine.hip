@@ -38,9 +38,11 @@ __global__ __launch_bounds__(256) OG_SEARCH_ATTR void og_search_kernel(SearchArg
     st.ymx.v[l] = a.stride[10 + l];
     st.t2d.v[l] = a.stride[20 + l];
   }
+  __shared__ ScanBank banks[(MODE & kModePattern) ? kMaxBanks : 1];
+  if (MODE & kModePattern) load_banks(banks, a.f);
   HitSink sink{a.counter, a.hits, a.cap, 0, nullptr};
   search_chain<MODE>(P, st, (cptr)a.table, a.scratch, T, g, a.half, a.rounds, T, g, a.k_base, a.f,
-                     sink);
+                     banks, sink);
   st_point(a.state, T, g, P);
 }
 
@@ -300,8 +302,7 @@ int create_engine(int ordinal, const uint8_t start_xyzt[128], const og_engine_op
   put_fe10(a.stride + 20, stride.t2d);
 
   // ---- buffers
-  const bool wantx = (e->filt.mode & kModePattern) != 0;
-  const size_t scratch_words = (size_t)2 * o.half * 10 * o.threads * (wantx ? 2 : 1);
+  const size_t scratch_words = (size_t)kPlanesPerPair * o.half * 10 * o.threads;
   e->cap = o.hit_cap ? o.hit_cap
                      : og_hit_room((uint64_t)o.threads * S * o.rounds, e->filt.bits ? e->filt.bits : 24);
   OG_TRY(hipMalloc(&e->d_state, (size_t)40 * o.threads * 4));

@@ -7,6 +7,8 @@
 
 #include "fe10.hpp"
 #include "keccak.hpp"
+#include "og_devfilter.hpp"
+#include "og_scan.hpp"
 #include "sha512.hpp"
 
 struct og_filterset {
@@ -256,6 +258,40 @@ int og_filters_match_key(const og_filterset *s, const uint8_t packed[32]) {
 
 int og_filters_match_address(const og_filterset *s, const uint8_t address[35]) {
   return s ? s->set.match_address(address) : -1;
+}
+
+int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int sign,
+                         int *key_hit, int *need, int *hit) {
+  if (!s || !packed || !key_hit || !need || !hit) return -1;
+  std::vector<ScanBank> banks;
+  if (s->set.general.empty() || !build_banks(s->set, banks).empty()) return -1;
+  Fe c;
+  for (int l = 0; l < 8; ++l) {
+    c.v[l] = (uint32_t)packed[4 * l] | ((uint32_t)packed[4 * l + 1] << 8) |
+             ((uint32_t)packed[4 * l + 2] << 16) | ((uint32_t)packed[4 * l + 3] << 24);
+  }
+  uint32_t kw[8];
+  key_words(c, kw);
+  bool kh, nd;
+  scan_key(kw, banks.data(), (uint32_t)banks.size(), kh, nd);
+  uint32_t pk[8];
+  for (int l = 0; l < 8; ++l) pk[l] = c.v[l];
+  pk[7] |= (uint32_t)(sign & 1) << 31;
+  uint32_t w[9];
+  address_words(kw, (uint32_t)(sign & 1), onion_checksum(pk), w);
+  uint32_t r = scan_address(w, banks.data(), (uint32_t)banks.size());
+  if (r == 2) {
+    // a truncated pattern's kept part matched: the symbol-wise check
+    uint8_t addr[35];
+    for (int i = 0; i < 35; ++i) addr[i] = (uint8_t)(w[i / 4] >> (24 - 8 * (i % 4)));
+    r = 0;
+    for (uint32_t g : s->set.general)
+      if (s->set.filters[g].matches_address(addr)) r = 1;
+  }
+  *key_hit = kh;
+  *need = nd;
+  *hit = r == 1;
+  return 0;
 }
 
 void og_key_secret_scalar(const uint8_t seed[32], uint8_t sk[32]) {

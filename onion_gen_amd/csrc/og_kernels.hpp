@@ -13,17 +13,26 @@
 // reference's pair idea (src/pairs.rs:1-19, cuda-kernel/src/lib.rs:118-121)
 // with the complete formula in place of its dedicated one.
 //
-// Montgomery's trick is folded (src/pairs.rs:182-207): the forward pass stores
-// s_i = n_i * (d_0...d_{i-1}) and keeps only the running product; after one
-// inversion the backward pass recomputes each d_i from P and the table (one
-// multiplication) instead of storing it, so a candidate costs one field element
-// of scratch written once and read once.  Per pair: 7 mul forward, 5 backward.
-// The x coordinates (needed only by the checksum forms) share the denominators:
-// x(P + Q) = (X yq + Y xq)/(Z + T dq) has the denominator of y(P - Q).
+// Montgomery's trick runs over the denominators in pair order.  The forward
+// pass computes C = T*dq, dp = Z - C, dm = Z + C and the running product, and
+// stores three field elements per pair: the prefix D0 before the pair, D1 =
+// D0*dp and C.  After one inversion of the whole product (times Z, for the
+// centre) the backward pass peels the inverses off:
+//     1/dm = inv*D1,  inv *= dm,  1/dp = inv*D0,  inv *= dp
+// and forms y(P - Q) = (A - B)/dm, y(P + Q) = (A + B)/dp with A = Y*yq,
+// B = X*xq.  Per pair: 3 multiplications forward, 8 backward; a candidate
+// costs 60 bytes of scratch written once and read once.  Keeping C costs one
+// element per pair and saves recomputing it (the reference recomputes its
+// denominators, src/pairs.rs:182-207, which suits a CPU's cache; here HBM has
+// the room and the bandwidth, and the multiply issue rate is the bound).
+// Because 1/dm and 1/dp are explicit, the x coordinate of a candidate costs
+// two more multiplications only where a filter needs the sign of x:
+//     x(P + Q) = (X*yq + Y*xq)/dm,   x(P - Q) = (X*yq - Y*xq)/dp.
 #pragma once
 #include "fe10.hpp"
 #include "ge.hpp"
 #include "keccak.hpp"
+#include "og_scan.hpp"
 
 namespace og {
 
@@ -38,11 +47,14 @@ struct HitRec {
 enum : uint32_t {
   kModePrefix = 1,   // masked compare of the key words (one literal prefix)
   kModeBitmap = 2,   // prefix bitmap, then the exact table when present
-  kModePattern = 4,  // general forms on the full address (sign + SHA3-256)
+  kModePattern = 4,  // general forms: the bank automaton of og_scan.hpp
 };
 
+// Scratch planes (field elements of 10 limbs) per pair: D0, D1, C.
+constexpr uint32_t kPlanesPerPair = 3;
+
 constexpr int kMaxBuckets = 16;
-constexpr int kMaxPatterns = 8;
+constexpr int kMaxPatterns = 64;  // symbol-wise records; the banks bound the set
 constexpr int kPatWords = 4 + 56;  // DevPattern: len, first, last, key_last, sets[56]
 
 struct DevFilterArgs {
@@ -53,14 +65,16 @@ struct DevFilterArgs {
   uint32_t nbuckets;              // exact table behind the bitmap
   const uint32_t *buckets;        // [nbuckets][offset, count, mask_lo, mask_hi]
   const uint64_t *keys;
-  uint32_t npat;                  // kModePattern
+  uint32_t nbanks;                // kModePattern: the automaton
+  const uint32_t *banks;          // nbanks records of kBankWords
+  uint32_t npat;                  // the same patterns, for the symbol-wise check
   const uint32_t *pats;           // npat records of kPatWords
 };
 
 struct SearchArgs {
   uint32_t *state;        // [4 coords][10 limbs][threads]
   const uint32_t *table;  // [half][xq, yq, dq][10]
-  uint32_t *scratch;      // [2*half][10][threads], then the same again for x
+  uint32_t *scratch;      // [kPlanesPerPair*half][10][threads]
   uint32_t threads, half, rounds;
   uint64_t k_base;        // first candidate of this launch
   uint32_t stride[30];    // (y+x, y-x, 2dxy) of threads*(2H+1)*8G, Z = 1

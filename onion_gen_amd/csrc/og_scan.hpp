@@ -1,0 +1,139 @@
+// The general filter forms (suffix, substring, class prefix) as one
+// bit-parallel Shift-And automaton over the 56 address symbols, for the host
+// and for gfx950.
+//
+// The reference decides these forms in two stages (src/filter.rs:782-810,
+// 1044-1125): what the key alone settles, and a prefilter that says whether a
+// placement reaching into the checksum is still possible; only then does it
+// pay for the sign of x and a SHA3-256.  Here both stages read one table:
+//
+//   * each pattern owns consecutive bits of a 32-bit state word (a "bank");
+//     b[c] has bit (offset + i) set when symbol c is allowed at position i;
+//   * per address symbol t_j:  D = ((D << 1) | inject_j) & b[t_j];  a match
+//     ends at j when a pattern's last bit is set in D.
+//
+// Key pass (canonical y, sign clear): symbols 0..48 are exact, so a match
+// ending there is a hit (src/filter.rs:120-134).  Symbols 49..55 continue as a
+// prefilter: 49 accepts both values of the deferred sign bit (either_sign,
+// src/filter.rs:158-164), 50 is exact, 51 has one known bit (y bit 248), 52
+// and 53 come from the checksum, 54 is one of a/i/q/y and 55 is always d
+// (src/key.rs:309-327).  A lane whose prefilter fires computes the sign and
+// the checksum and runs the exact pass over all 56 symbols.
+//
+// Patterns longer than one bank (32 symbols) keep their first 32 positions;
+// reaching the last of those marks the lane for the symbol-by-symbol check on
+// the full address (`flong`), which such a match makes rare.
+#pragma once
+#include "fe.hpp"
+
+namespace og {
+
+constexpr int kBankBits = 32;
+constexpr int kMaxBanks = 8;
+constexpr int kBankWords = 48;
+
+// One bank, 48 words (the device copy is these words back to back).
+struct ScanBank {
+  uint32_t b[32];   // per symbol value: the positions that accept it
+  uint32_t istart;  // first bits of start-anchored patterns (injected at 0)
+  uint32_t iany;    // first bits of the others (injected at every symbol)
+  uint32_t fany;    // last bits that end a match anywhere
+  uint32_t fend;    // last bits that count only at symbol 55
+  uint32_t flong;   // last kept bits of truncated patterns
+  uint32_t u51[2];  // symbol 51 by its known top bit (y bit 248)
+  uint32_t uall;    // any symbol (52, 53)
+  uint32_t u54;     // a, i, q or y
+  uint32_t jstart;  // no pattern of the bank starts before this symbol
+  uint32_t pad[6];
+};
+static_assert(sizeof(ScanBank) == kBankWords * 4, "bank layout");
+
+// Symbol j of the address bit string held as big-endian words (eight for the
+// key, nine with the checksum and version).
+OG_HD uint32_t sym_at(const uint32_t *w, int j) {
+  const int bit = 5 * j, i = bit >> 5, sh = bit & 31;
+  if (sh <= 27) return (w[i] >> (27 - sh)) & 31u;
+  return ((w[i] << (sh - 27)) | (w[i + 1] >> (59 - sh))) & 31u;
+}
+
+OG_HD void key_words(const Fe &c, uint32_t w[8]) {
+#pragma unroll
+  for (int l = 0; l < 8; ++l) {
+    const uint32_t x = c.v[l];
+    w[l] = (x >> 24) | ((x >> 8) & 0xff00u) | ((x << 8) & 0xff0000u) | (x << 24);
+  }
+}
+
+// The key pass.  `hit`: a placement inside symbols 0..48 matched.  `need`:
+// some placement reaching past 48 is still possible (or a truncated pattern
+// reached its last kept bit), so the full address decides.
+OG_HD void scan_key(const uint32_t w[8], const ScanBank *banks, uint32_t nbanks, bool &hit,
+                    bool &need) {
+  hit = false;
+  need = false;
+  const uint32_t y248 = w[7] & 1u;  // stream bit 255: the top bit of symbol 51
+  for (uint32_t k = 0; k < nbanks; ++k) {
+    const ScanBank &s = banks[k];
+    const uint32_t js = s.jstart, iany = s.iany;
+    uint32_t D = 0, acc = 0;
+#pragma unroll
+    for (int j = 0; j < 49; ++j) {
+      if ((uint32_t)j >= js) {
+        D = ((D << 1) | (j == 0 ? (s.istart | iany) : iany)) & s.b[sym_at(w, j)];
+        acc |= D;
+      }
+    }
+    hit |= (acc & s.fany) != 0;
+    uint32_t e = acc & s.flong;
+    const uint32_t t49 = sym_at(w, 49);  // sign bit clear: value 2 of the symbol
+    D = ((D << 1) | iany) & (s.b[t49] | s.b[t49 | 2u]);
+    e |= D & (s.fany | s.flong);
+    D = ((D << 1) | iany) & s.b[sym_at(w, 50)];
+    e |= D & (s.fany | s.flong);
+    D = ((D << 1) | iany) & s.u51[y248];
+    e |= D & (s.fany | s.flong);
+    D = ((D << 1) | iany) & s.uall;
+    e |= D & (s.fany | s.flong);
+    D = ((D << 1) | iany) & s.uall;
+    e |= D & (s.fany | s.flong);
+    D = ((D << 1) | iany) & s.u54;
+    e |= D & (s.fany | s.flong);
+    D = ((D << 1) | iany) & s.b[3];  // 'd'
+    e |= D & (s.fany | s.flong | s.fend);
+    need |= e != 0;
+  }
+}
+
+// The exact pass over the 35-byte address as nine big-endian words (the last
+// holds checksum[0], checksum[1], version 3 and a zero byte).  Returns 1 for a
+// match, 2 when only a truncated pattern's kept part matched (the caller
+// checks those symbol by symbol), 0 otherwise.
+OG_HD uint32_t scan_address(const uint32_t w[9], const ScanBank *banks, uint32_t nbanks) {
+  uint32_t out = 0;
+  for (uint32_t k = 0; k < nbanks; ++k) {
+    const ScanBank &s = banks[k];
+    const uint32_t js = s.jstart, iany = s.iany;
+    uint32_t D = 0, acc = 0;
+#pragma unroll
+    for (int j = 0; j < 56; ++j) {
+      if ((uint32_t)j >= js) {
+        D = ((D << 1) | (j == 0 ? (s.istart | iany) : iany)) & s.b[sym_at(w, j)];
+        acc |= D;
+      }
+    }
+    if ((acc & s.fany) | (D & s.fend)) return 1;
+    if (acc & s.flong) out = 2;
+  }
+  return out;
+}
+
+// The address words of a key: canonical y (sign clear), the sign of x and the
+// two checksum bytes.
+OG_HD void address_words(const uint32_t kw[8], uint32_t sign, uint32_t chk, uint32_t w[9]) {
+#pragma unroll
+  for (int l = 0; l < 8; ++l) w[l] = kw[l];
+  w[7] |= sign << 7;  // the top bit of key byte 31
+  w[8] = ((chk & 0xffu) << 24) | (((chk >> 8) & 0xffu) << 16) | (0x03u << 8);
+}
+
+}  // namespace og

@@ -29,17 +29,27 @@ __device__ __forceinline__ Fe10 ld10_const(cptr p) {
 
 // Element `plane` of a structure-of-arrays buffer of 10-limb values: limb l of
 // chain g sits at (plane*10 + l)*T + g, so a wave touches 64 consecutive words.
+// The plane base is wave-uniform and the in-plane index fits 32 bits, so each
+// access is a scalar base plus a 32-bit lane offset (no 64-bit address math
+// per load).
+__device__ __forceinline__ const uint32_t *soa_at(const uint32_t *base, size_t plane, uint32_t T,
+                                                  uint32_t g, int l) {
+  const char *pl = reinterpret_cast<const char *>(base + plane * 10 * T);
+  return reinterpret_cast<const uint32_t *>(pl + (uint32_t)((l * T + g) * 4u));
+}
+
 __device__ __forceinline__ Fe10 ld10(const uint32_t *base, size_t plane, uint32_t T, uint32_t g) {
   Fe10 r;
 #pragma unroll
-  for (int l = 0; l < 10; ++l) r.v[l] = __builtin_nontemporal_load(base + (plane * 10 + l) * T + g);
+  for (int l = 0; l < 10; ++l) r.v[l] = __builtin_nontemporal_load(soa_at(base, plane, T, g, l));
   return r;
 }
 
 __device__ __forceinline__ void st10(uint32_t *base, size_t plane, uint32_t T, uint32_t g,
                                      const Fe10 &v) {
 #pragma unroll
-  for (int l = 0; l < 10; ++l) __builtin_nontemporal_store(v.v[l], base + (plane * 10 + l) * T + g);
+  for (int l = 0; l < 10; ++l)
+    __builtin_nontemporal_store(v.v[l], const_cast<uint32_t *>(soa_at(base, plane, T, g, l)));
 }
 
 __device__ __forceinline__ uint32_t bswap32(uint32_t x) { return __builtin_bswap32(x); }
@@ -93,33 +103,15 @@ __device__ __forceinline__ bool exact_key(const Fe &c, const DevFilterArgs &f) {
   return false;
 }
 
-// Symbol m (0..55) of the address held as nine big-endian words.
-__device__ __forceinline__ uint32_t addr_symbol(const uint32_t w[9], uint32_t m) {
-  const uint32_t bit = 5 * m, i = bit >> 5, sh = bit & 31;
-  const uint32_t hi = w[i];
-  const uint32_t lo = i + 1 < 9 ? w[i + 1] : 0u;
-  const uint64_t both = ((uint64_t)hi << 32) | lo;
-  return (uint32_t)(both >> (59 - sh)) & 31u;
-}
-
-// The general forms on the full address: pk = canonical y with the sign of x
-// in bit 255, then SHA3-256 for the two checksum bytes (src/key.rs:106-132).
-__device__ __forceinline__ bool pattern_test(const Fe &cy, const Fe &cx, const DevFilterArgs &f) {
-  uint32_t pk[8];
-#pragma unroll
-  for (int l = 0; l < 8; ++l) pk[l] = cy.v[l];
-  pk[7] |= (cx.v[0] & 1u) << 31;
-  const uint32_t chk = onion_checksum(pk);
-  uint32_t w[9];
-#pragma unroll
-  for (int l = 0; l < 8; ++l) w[l] = bswap32(pk[l]);
-  w[8] = ((chk & 0xffu) << 24) | (((chk >> 8) & 0xffu) << 16) | (0x03u << 8);
+// The general forms symbol by symbol on the full address: the exact check for
+// patterns longer than one automaton bank.
+__device__ __forceinline__ bool pattern_test(const uint32_t w[9], const DevFilterArgs &f) {
   for (uint32_t p = 0; p < f.npat; ++p) {
     const uint32_t *rec = f.pats + p * kPatWords;
     const uint32_t len = rec[0], first = rec[1], last = rec[2];
     for (uint32_t s = first; s <= last; ++s) {
       bool ok = true;
-      for (uint32_t i = 0; i < len && ok; ++i) ok = (rec[4 + i] >> addr_symbol(w, s + i)) & 1u;
+      for (uint32_t i = 0; i < len && ok; ++i) ok = (rec[4 + i] >> sym_at(w, (int)(s + i))) & 1u;
       if (ok) return true;
     }
   }
@@ -156,25 +148,7 @@ struct HitSink {
   uint32_t *slot_hits;  // per-slot hit counters, or null
 };
 
-template <uint32_t MODE>
-__device__ __forceinline__ void test_candidate(const Fe10 &y, const Fe10 &x, uint64_t k,
-                                               const DevFilterArgs &f, const HitSink &sink) {
-  constexpr bool kKey = (MODE & (kModePrefix | kModeBitmap)) != 0;
-  constexpr bool kPat = (MODE & kModePattern) != 0;
-  bool hit = false;
-  Fe c;
-  if (kPat) {
-    c = fe10_canon(y);
-    if (kKey) hit = exact_key<MODE>(c, f);
-    if (!hit) hit = pattern_test(c, fe10_canon(x), f);
-  } else {
-    const bool maybe = maybe_key<MODE>(y, f);
-    if (__ballot(maybe) == 0) return;  // the common case: no lane needs more
-    if (maybe) {
-      c = fe10_canon(y);
-      hit = exact_key<MODE>(c, f);
-    }
-  }
+__device__ __forceinline__ void report(bool hit, uint64_t k, const Fe &c, const HitSink &sink) {
   if (sink.slot_hits) {
     const uint64_t m = __ballot(hit);
     if (m && __lane_id() == (uint32_t)__ffsll((unsigned long long)m) - 1)
@@ -183,71 +157,155 @@ __device__ __forceinline__ void test_candidate(const Fe10 &y, const Fe10 &x, uin
   append(hit, sink.slot, k, c, sink.counter, sink.hits, sink.cap);
 }
 
-// `rounds` windows of one chain.  Every multiplication takes its looser
-// operand first (fe10_mul's bounds).  g indexes the SoA buffers (stride T); the
-// chain's windows are numbered w = r*chains + local from k_base.
+// Key forms only: the low-32-bit probe, and the exact test on the lanes the
+// wave ballot names.
+template <uint32_t MODE>
+__device__ __forceinline__ void test_key_only(const Fe10 &y, uint64_t k, const DevFilterArgs &f,
+                                              const HitSink &sink) {
+  const bool maybe = maybe_key<MODE>(y, f);
+  if (__ballot(maybe) == 0) return;  // the common case: no lane needs more
+  bool hit = false;
+  Fe c;
+  if (maybe) {
+    c = fe10_canon(y);
+    hit = exact_key<MODE>(c, f);
+  }
+  report(hit, k, c, sink);
+}
+
+// The key-derived part of a candidate under the general forms (and the key
+// forms when the set has both): `hit` is final, `need` asks for the address.
+template <uint32_t MODE>
+__device__ __forceinline__ void key_stage(const Fe &c, const uint32_t w[8], const DevFilterArgs &f,
+                                          const ScanBank *banks, bool &hit, bool &need) {
+  scan_key(w, banks, f.nbanks, hit, need);
+  if (MODE & (kModePrefix | kModeBitmap)) hit |= exact_key<MODE>(c, f);
+  need &= !hit;
+}
+
+// The address stage for the lanes that need it: the sign of x (bit 0 of its
+// canonical form), SHA3-256 for the checksum, the exact automaton pass.
+__device__ __forceinline__ bool address_stage(const uint32_t kw[8], const Fe10 &x,
+                                              const DevFilterArgs &f, const ScanBank *banks) {
+  const uint32_t sign = fe10_canon(x).v[0] & 1u;
+  uint32_t pk[8];
+#pragma unroll
+  for (int l = 0; l < 7; ++l) pk[l] = __builtin_bswap32(kw[l]);
+  pk[7] = __builtin_bswap32(kw[7]) | (sign << 31);
+  uint32_t w[9];
+  address_words(kw, sign, onion_checksum(pk), w);
+  const uint32_t r = scan_address(w, banks, f.nbanks);
+  return r == 1 || (r == 2 && pattern_test(w, f));
+}
+
+// `rounds` windows of one chain (the method is described in og_kernels.hpp).
+// Every multiplication takes its looser operand first (fe10_mul's bounds).  g
+// indexes the SoA buffers (stride T); the chain's windows are numbered
+// w = r*chains + local from k_base.  `banks` is the automaton in LDS.
 template <uint32_t MODE>
 __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cptr table,
                                              uint32_t *scratch, uint32_t T, uint32_t g,
                                              uint32_t H, uint32_t rounds, uint32_t chains,
                                              uint32_t local, uint64_t k_base,
-                                             const DevFilterArgs &f, const HitSink &sink) {
-  constexpr bool kX = (MODE & kModePattern) != 0;
+                                             const DevFilterArgs &f, const ScanBank *banks,
+                                             const HitSink &sink) {
+  constexpr bool kPat = (MODE & kModePattern) != 0;
   const uint64_t S = 2ull * H + 1;
-  uint32_t *xs = scratch + (size_t)2 * H * 10 * T;
   for (uint32_t r = 0; r < rounds; ++r) {
     const uint64_t w = (uint64_t)r * chains + local;
     const uint64_t kc = k_base + w * S + H;
     Fe10 D = fe10_one();
     for (uint32_t m = 0; m < H; ++m) {
-      const Fe10 xq = ld10_const(table + m * 30);
-      const Fe10 yq = ld10_const(table + m * 30 + 10);
       const Fe10 dq = ld10_const(table + m * 30 + 20);
-      const Fe10 A = fe10_mul(P.Y, yq);
-      const Fe10 B = fe10_mul(P.X, xq);
       const Fe10 C = fe10_mul(P.T, dq);
       const Fe10 dp = fe10_sub(P.Z, C);
       const Fe10 dm = fe10_add(P.Z, C);
-      const Fe10 D0 = D;
-      st10(scratch, 2 * m, T, g, fe10_mul(fe10_add(A, B), D));
+      st10(scratch, kPlanesPerPair * m, T, g, D);
       D = fe10_mul(dp, D);
-      st10(scratch, 2 * m + 1, T, g, fe10_mul(fe10_sub(A, B), D));
-      if (kX) {
-        const Fe10 XY = fe10_mul(P.X, yq);
-        const Fe10 YX = fe10_mul(P.Y, xq);
-        st10(xs, 2 * m, T, g, fe10_mul(fe10_add(XY, YX), D));
-        st10(xs, 2 * m + 1, T, g, fe10_mul(fe10_sub(XY, YX), D0));
-      }
+      st10(scratch, kPlanesPerPair * m + 1, T, g, D);
+      st10(scratch, kPlanesPerPair * m + 2, T, g, C);
       D = fe10_mul(dm, D);
     }
-    const Fe10 sc = fe10_mul(P.Y, D);
-    const Fe10 tc = kX ? fe10_mul(P.X, D) : D;
-    D = fe10_mul(D, P.Z);
-    Fe10 inv = fe10_invert(D);
+    // inv = 1/(D Z): the centre takes 1/Z = inv D, and inv Z = 1/D starts
+    // the backward pass
+    Fe10 inv = fe10_invert(fe10_mul(D, P.Z));
     {
-      const Fe10 y = fe10_mul(sc, inv);
-      const Fe10 x = kX ? fe10_mul(tc, inv) : y;
-      test_candidate<MODE>(y, x, kc, f, sink);
+      const Fe10 iz = fe10_mul(inv, D);
+      const Fe10 y = fe10_mul(P.Y, iz);
+      if (kPat) {
+        const Fe c = fe10_canon(y);
+        uint32_t kw[8];
+        key_words(c, kw);
+        bool hit, need;
+        key_stage<MODE>(c, kw, f, banks, hit, need);
+        if (__ballot(need)) {
+          const Fe10 x = fe10_mul(P.X, iz);
+          if (need) hit = address_stage(kw, x, f, banks);
+        }
+        report(hit, kc, c, sink);
+      } else {
+        test_key_only<MODE>(y, kc, f, sink);
+      }
     }
     inv = fe10_mul(inv, P.Z);
+    // Software-pipelined by one pair: the three elements of pair m-1 are
+    // loaded as soon as pair m's are consumed, so their latency hides behind
+    // the four multiplications that form pair m's candidates.
+    Fe10 D0 = ld10(scratch, kPlanesPerPair * (H - 1), T, g);
+    Fe10 D1 = ld10(scratch, kPlanesPerPair * (H - 1) + 1, T, g);
+    Fe10 C = ld10(scratch, kPlanesPerPair * (H - 1) + 2, T, g);
     for (uint32_t m = H; m-- > 0;) {
-      const Fe10 dq = ld10_const(table + m * 30 + 20);
-      const Fe10 C = fe10_mul(P.T, dq);
       const Fe10 dp = fe10_sub(P.Z, C);
       const Fe10 dm = fe10_add(P.Z, C);
-      // entry: inv = 1/(D0 dp dm).  y(P-Q) = s_m inv and x(P+Q) = t_p inv;
-      // after inv *= dm: y(P+Q) = s_p inv and x(P-Q) = t_m inv.
-      const Fe10 ym = fe10_mul(ld10(scratch, 2 * m + 1, T, g), inv);
-      const Fe10 xp = kX ? fe10_mul(ld10(xs, 2 * m, T, g), inv) : ym;
+      // entry: inv = 1/(D0 dp dm)
+      const Fe10 idm = fe10_mul(D1, inv);
       inv = fe10_mul(dm, inv);
-      const Fe10 xm = kX ? fe10_mul(ld10(xs, 2 * m + 1, T, g), inv) : ym;
-      test_candidate<MODE>(ym, xm, kc - (m + 1), f, sink);
-      const Fe10 yp = fe10_mul(ld10(scratch, 2 * m, T, g), inv);
-      test_candidate<MODE>(yp, xp, kc + (m + 1), f, sink);
+      const Fe10 idp = fe10_mul(D0, inv);
       inv = fe10_mul(dp, inv);
+      {
+        const size_t next = kPlanesPerPair * (m ? m - 1 : 0);  // pair 0 reloads itself
+        D0 = ld10(scratch, next, T, g);
+        D1 = ld10(scratch, next + 1, T, g);
+        C = ld10(scratch, next + 2, T, g);
+      }
+      const Fe10 xq = ld10_const(table + m * 30);
+      const Fe10 yq = ld10_const(table + m * 30 + 10);
+      const Fe10 A = fe10_mul(P.Y, yq);
+      const Fe10 B = fe10_mul(P.X, xq);
+      const Fe10 ym = fe10_mul(fe10_sub(A, B), idm);  // y(P - Q)
+      const Fe10 yp = fe10_mul(fe10_add(A, B), idp);  // y(P + Q)
+      const uint64_t km = kc - (m + 1), kp = kc + (m + 1);
+      if (kPat) {
+        const Fe cm = fe10_canon(ym), cp = fe10_canon(yp);
+        uint32_t wm[8], wp[8];
+        key_words(cm, wm);
+        key_words(cp, wp);
+        bool hm, nm, hp, np;
+        key_stage<MODE>(cm, wm, f, banks, hm, nm);
+        key_stage<MODE>(cp, wp, f, banks, hp, np);
+        if (__ballot(nm | np)) {
+          // x(P - Q) has the denominator dp, x(P + Q) the denominator dm
+          const Fe10 XY = fe10_mul(P.X, yq);
+          const Fe10 YX = fe10_mul(P.Y, xq);
+          if (nm) hm = address_stage(wm, fe10_mul(fe10_sub(XY, YX), idp), f, banks);
+          if (np) hp = address_stage(wp, fe10_mul(fe10_add(XY, YX), idm), f, banks);
+        }
+        report(hm, km, cm, sink);
+        report(hp, kp, cp, sink);
+      } else {
+        test_key_only<MODE>(ym, km, f, sink);
+        test_key_only<MODE>(yp, kp, f, sink);
+      }
     }
     P = pt10_add1(P, st);
   }
+}
+
+// The automaton banks of a filter into this workgroup's LDS copy.
+__device__ __forceinline__ void load_banks(ScanBank *lds, const DevFilterArgs &f) {
+  uint32_t *dst = reinterpret_cast<uint32_t *>(lds);
+  for (uint32_t i = threadIdx.x; i < f.nbanks * kBankWords; i += blockDim.x) dst[i] = f.banks[i];
+  __syncthreads();
 }
 
 __device__ __forceinline__ Point10 ld_point(const uint32_t *state, uint32_t T, uint32_t g) {

@@ -45,6 +45,15 @@ class FilterSet:
     def match_address(self, address_bytes):
         return lib().og_filters_match_address(self.handle, bytes(address_bytes))
 
+    def scan(self, packed, sign):
+        """The device matcher of the general forms, run on the host for one
+        key: (key_hit, need, hit) as og_filters_scan_host defines them."""
+        kh, nd, hit = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        if lib().og_filters_scan_host(self.handle, bytes(packed), sign, ctypes.byref(kh),
+                                      ctypes.byref(nd), ctypes.byref(hit)) != 0:
+            raise OgError("the set has no general filter or does not compile")
+        return bool(kh.value), bool(nd.value), bool(hit.value)
+
     def close(self):
         if self.handle:
             lib().og_filters_free(self.handle)

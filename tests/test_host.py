@@ -160,3 +160,71 @@ def test_bad_filters_raise():
                 "regex:^ab"]:
         with pytest.raises(_lib.OgError):
             filters.FilterSet.parse(bad)
+
+
+def _address_text(pk):
+    ab = key.address_bytes(pk)
+    bits = "".join(format(b, "08b") for b in ab)
+    return "".join(workload.ALPHABET[int(bits[5 * i:5 * i + 5], 2)] for i in range(56))
+
+
+def _scan_cases(rng):
+    """General filter sets planted on real addresses: every placement from 0 to
+    the last, the sign symbol 49, the checksum symbols 51-55, '?' classes,
+    several banks, and patterns longer than one bank (32 symbols)."""
+    for i in range(400):
+        pk = bytes(rng.getrandbits(8) for _ in range(32))
+        text = _address_text(pk)
+        kind = i % 5
+        if kind == 0:
+            L = rng.randint(3, 12)
+            s = rng.randint(0, 56 - L)
+            lines = ["contains:" + text[s:s + L]]
+        elif kind == 1:
+            L = rng.randint(3, 9)
+            lines = ["suffix:" + text[56 - L:]]
+        elif kind == 2:
+            L = rng.randint(2, 6)
+            s = rng.choice([44, 45, 46, 47, 48, 49, 50, 51])
+            L = min(L, 56 - s)
+            body = list(text[s:s + L])
+            body[rng.randrange(L)] = "?"
+            if all(c == "?" for c in body):
+                body[0] = text[s]
+            lines = ["contains:" + "".join(body)]
+        elif kind == 3:
+            L = rng.randint(33, 50)
+            s = rng.randint(0, 56 - L)
+            lines = ["contains:" + text[s:s + L], "prefix:" + text[:3] + "?" + text[4:40]]
+        else:
+            lines = ["contains:" + "".join(rng.choice(workload.ALPHABET) for _ in range(5))
+                     for _ in range(12)]
+            lines += ["suffix:" + text[52:], "contains:" + text[20:25]]
+        yield pk, lines
+
+
+def test_device_matcher_agrees_with_oracle_on_every_placement():
+    """The automaton the kernels run (key pass, tail prefilter, address pass),
+    executed on the host: its verdict equals the oracle's whole-address match;
+    a key-pass hit is a hit whatever the sign; a hit the key pass did not
+    settle is always flagged for the address stage."""
+    rng = random.Random(2024)
+    checked = hits = needed = 0
+    for planted, lines in _scan_cases(rng):
+        mine, theirs = filters.FilterSet.parse(lines), OracleFilters(lines)
+        others = [bytes(rng.getrandbits(8) for _ in range(32)) for _ in range(6)]
+        for pk in [planted] + others:
+            sign = pk[31] >> 7
+            clear = pk[:31] + bytes([pk[31] & 0x7F])
+            kh, need, hit = mine.scan(clear, sign)
+            want = theirs.match_address(key.address_bytes(pk)) >= 0
+            assert hit == want, (lines, pk.hex())
+            assert mine.scan(clear, 1 - sign)[0] == kh
+            if kh:
+                assert hit
+            if hit and not kh:
+                assert need, (lines, pk.hex())
+            checked += 1
+            hits += hit
+            needed += need
+    assert hits >= 400 and needed > 0 and checked == 400 * 7
