@@ -38,6 +38,109 @@ def field_muls_per_candidate(half, with_x=False):
     return window / (2 * half + 1) + (2 if with_x else 0)
 
 
+def _pct(xs, q):
+    if not xs:
+        return None
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(q * len(xs)))]
+
+
+def run_mix(searches, seconds, slots, chains, half, policy, ring_cap=1 << 20):
+    """The searches (seed32, filter text, wanted hits) resident on one slot
+    engine for `seconds` of wall time.  A search retires at its target.
+    Returns aggregate candidates/s and per-search first-hit / done times."""
+    from onion_gen_amd import batch, filters, key
+    import numpy as np
+    b = batch.Batch(slots=slots, chains=chains, half=half, rounds=1, ring_cap=ring_cap,
+                    policy=policy)
+    tag_of = {}
+    for i, (seed, text, want) in enumerate(searches):
+        sid = b.admit(key.start_point(seed), filters.FilterSet.parse(text), i)
+        b.set_want(sid, want)
+        tag_of[sid] = i
+    per = b.per_step()
+    hits = [0] * len(searches)
+    first, done = [None] * len(searches), [None] * len(searches)
+    live = set(tag_of)
+    examined = steps = 0
+    device_ms = 0.0
+    t0 = time.perf_counter()
+    while live and time.perf_counter() - t0 < seconds:
+        ran = min(len(live), slots)
+        tags = b.step_tags()
+        now = time.perf_counter() - t0
+        steps += 1
+        examined += ran * per
+        device_ms += b.last_ms()
+        if len(tags):
+            for t, c in zip(*np.unique(tags, return_counts=True)):
+                hits[int(t)] += int(c)
+                if first[int(t)] is None:
+                    first[int(t)] = now
+        for sid in [s for s in live if hits[tag_of[s]] >= searches[tag_of[s]][2]]:
+            done[tag_of[sid]] = now
+            b.retire(sid)
+            live.discard(sid)
+    wall = time.perf_counter() - t0
+    b.close()
+    got = [t for t in first if t is not None]
+    return {"keys_per_s": examined / wall, "device_keys_per_s": examined / (device_ms * 1e-3),
+            "window_s": wall, "steps": steps, "completed": sum(d is not None for d in done),
+            "with_hit": len(got), "p50_first_hit_s": _pct(got, 0.5),
+            "p99_first_hit_s": _pct(got, 0.99)}
+
+
+def batched_leg(seconds):
+    """The 64-search mix in one launch per step, against one search per
+    launch: at the same per-search shape, and at the full-GPU shape."""
+    from onion_gen_amd import batch, workload
+    mix = workload.search_mix(64, 4)
+    kinds = {}
+    for _, text, _ in mix:
+        k = text.split(":")[0] if ":" in text else "prefix"
+        kinds[k] = kinds.get(k, 0) + 1
+    batched = run_mix(mix, seconds, 64, 2048, 256, batch.LEAST_SERVED)
+    plain = run_mix(mix, seconds, 1, 2048, 256, batch.ROUND_ROBIN)
+    full = run_mix(mix, seconds, 1, 1 << 18, 256, batch.ROUND_ROBIN, ring_cap=1 << 21)
+    return {
+        "workload": "64 searches (search_mix seed 4: " + ", ".join(f"{v} {k}" for k, v in
+                                                                   sorted(kinds.items())) +
+                    "; wanted hits log-uniform 1..1000), fixed wall-time window",
+        "batched": batched,
+        "plain_per_search_shape": plain,
+        "plain_full_gpu": full,
+        "speedup_vs_plain": batched["keys_per_s"] / plain["keys_per_s"],
+        "speedup_vs_plain_full_gpu": batched["keys_per_s"] / full["keys_per_s"],
+    }
+
+
+def served_leg(seconds, rate=200.0):
+    """Poisson arrivals at `rate`/s for `seconds`, each request a filter of
+    the batched mix wanting 1, 10 or 100 hits: the continuous-batching loop
+    against one search at a time on the whole GPU (first come, first served)."""
+    from onion_gen_amd import batch, filters, serve, workload
+    arrivals = workload.poisson_arrivals(rate, seconds, 5)
+    mix = workload.search_mix(len(arrivals), 6, hits="choice")
+    reqs = [(t, seed, text, want) for t, (seed, text, want) in zip(arrivals, mix)]
+    keep = ("keys_per_s", "completed", "running", "queued", "rejected", "max_resident",
+            "p50_time_to_n", "p99_time_to_n", "p50_time_to_first", "p99_time_to_first")
+    st, res = serve.run(reqs, duration=seconds, slots=64, chains=2048, half=256,
+                        max_queue=1 << 16, policy=batch.LEAST_SERVED)
+    cont = {k: st[k] for k in keep}
+    with batch.Batch(slots=1, chains=1 << 18, half=256) as b:
+        one = max(b.footprint(filters.FilterSet.parse(t)) for t in {r[2] for r in reqs})
+    st2, _ = serve.run(reqs, duration=seconds, slots=1, chains=1 << 18, half=256,
+                       max_queue=1 << 16, device_budget=one, policy=batch.ROUND_ROBIN)
+    fifo = {k: st2[k] for k in keep}
+    return {
+        "workload": f"Poisson {rate:.0f} req/s for {seconds:.0f} s ({len(reqs)} requests), "
+                    "filters of the batched mix, n in {1, 10, 100}",
+        "continuous": cont,
+        "plain_fifo_full_gpu": fifo,
+        "keys_per_s_vs_plain": cont["keys_per_s"] / fifo["keys_per_s"] if fifo["keys_per_s"] else None,
+    }
+
+
 def cpu_baseline(prefix, seconds):
     from tests.oracle_lib import OracleFilters, load
     import ctypes
@@ -67,6 +170,10 @@ def main():
     ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mix-seconds", type=float, default=5.0,
+                    help="wall-time window of each batched-mix run (0 skips the leg)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="arrival window of each served run (0 skips the leg)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="run only a 1 GiB device-to-device copy: the known-size kernel the "
                          "FETCH_SIZE / WRITE_SIZE counters are scaled by")
@@ -148,6 +255,10 @@ def main():
         },
     }
     eng.close()
+    if args.mix_seconds > 0:
+        out["batched"] = batched_leg(args.mix_seconds)
+    if args.serve_seconds > 0:
+        out["served"] = served_leg(args.serve_seconds)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prefix, args.cpu_seconds)
     print(json.dumps(out), flush=True)

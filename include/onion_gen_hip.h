@@ -160,7 +160,16 @@ typedef struct {
   uint32_t half;     /* pairs per window; 0 = 256 */
   uint32_t rounds;   /* windows per chain per step; 0 = 1 */
   uint32_t ring_cap; /* survivor records per step; 0 = 65536 */
+  uint32_t policy;   /* which resident searches take the slots: OG_POLICY_* */
 } og_batch_opts;
+
+enum {
+  /* the searches that have examined the fewest candidates (ties: first
+   * admitted first), so a new search runs at the next step */
+  OG_POLICY_LEAST_SERVED = 0,
+  /* in turn */
+  OG_POLICY_ROUND_ROBIN = 1
+};
 
 typedef struct {
   uint64_t tag;       /* the tag the search was admitted with */
@@ -178,9 +187,15 @@ size_t og_batch_footprint(const og_batch *b, const og_filterset *filters);
  * set up on the device and its candidate index starts at 0.  Writes its id. */
 int og_batch_admit(og_batch *b, const uint8_t start_xyzt[128], const og_filterset *filters,
                    uint64_t tag, int *id, char *err, size_t errcap);
+/* The search records at most `want` hits in all (default: every hit).  Hits
+ * past it are still found and counted (og_batch_search_found) but not written
+ * to the ring, so a search that needs n hits cannot flood it. */
+int og_batch_set_want(og_batch *b, int id, uint64_t want);
+/* Hits the search has found, recorded or not. */
+uint64_t og_batch_search_found(const og_batch *b, int id);
 /* Frees a resident search's device memory; it is not scheduled again. */
 int og_batch_retire(og_batch *b, int id);
-/* One step: the next `slots` resident searches (round-robin) each advance;
+/* One step: `slots` resident searches, chosen by the policy, each advance;
  * writes the survivors.  Overflowing the ring or `cap` is an error. */
 int og_batch_step(og_batch *b, og_batch_hit *out, uint32_t cap, uint32_t *n, char *err,
                   size_t errcap);
@@ -224,10 +239,11 @@ typedef struct {
 
 typedef struct {
   int32_t status;       /* OG_REQ_* at the end of the run */
-  uint32_t hits;
+  uint32_t hits;        /* hits recorded: at most want_hits */
   uint64_t examined;
   double admitted_s, first_hit_s, done_s;  /* -1 when not reached */
-  og_found found[16];   /* the first 16 hits */
+  og_found found[16];   /* the first 16 recorded hits */
+  uint64_t seen;        /* hits found over `examined`, recorded or not */
 } og_request_result;
 
 typedef struct {

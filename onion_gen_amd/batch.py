@@ -1,7 +1,8 @@
 """Many searches in one launch: the slot engine of include/onion_gen_hip.h.
 
-Resident searches keep their chain state and filter tables on the device and
-take the ``slots`` of each step round-robin."""
+Resident searches keep their chain state and filter tables on the device; at
+each step the policy (least served first, or round-robin) picks which of them
+take the ``slots``."""
 import ctypes
 from dataclasses import dataclass
 
@@ -11,7 +12,10 @@ from ._lib import OgError, err_buf, lib
 class _Opts(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_uint32), ("chains", ctypes.c_uint32),
                 ("half", ctypes.c_uint32), ("rounds", ctypes.c_uint32),
-                ("ring_cap", ctypes.c_uint32)]
+                ("ring_cap", ctypes.c_uint32), ("policy", ctypes.c_uint32)]
+
+
+LEAST_SERVED, ROUND_ROBIN = 0, 1
 
 
 class _Hit(ctypes.Structure):
@@ -28,11 +32,12 @@ class BatchHit:
 
 
 class Batch:
-    def __init__(self, slots=64, chains=2048, half=256, rounds=1, ring_cap=0, device=0):
+    def __init__(self, slots=64, chains=2048, half=256, rounds=1, ring_cap=0,
+                 policy=LEAST_SERVED, device=0):
         h = ctypes.c_void_p()
         e = err_buf()
         rc = lib().og_batch_create(device, ctypes.byref(_Opts(slots, chains, half, rounds,
-                                                               ring_cap)),
+                                                               ring_cap, policy)),
                                    ctypes.byref(h), e, len(e))
         if rc != 0:
             raise OgError(e.value.decode())
@@ -61,6 +66,15 @@ class Batch:
             raise OgError(e.value.decode())
         return sid.value
 
+    def set_want(self, sid, want):
+        """The search records at most `want` hits in all; later hits are
+        counted (found) but not reported."""
+        if lib().og_batch_set_want(self._h, sid, want) != 0:
+            raise OgError(f"no resident search {sid}")
+
+    def found(self, sid):
+        return lib().og_batch_search_found(self._h, sid)
+
     def retire(self, sid):
         if lib().og_batch_retire(self._h, sid) != 0:
             raise OgError(f"could not retire search {sid}")
@@ -73,6 +87,18 @@ class Batch:
             raise OgError(e.value.decode())
         return [BatchHit(self._out[i].tag, self._out[i].search, self._out[i].offset,
                          bytes(self._out[i].packed)) for i in range(n.value)]
+
+    def step_tags(self):
+        """One step; the tag of every hit as a numpy array (no per-hit Python
+        objects: for throughput runs whose searches hit often)."""
+        import numpy as np
+        n = ctypes.c_uint32()
+        e = err_buf()
+        rc = lib().og_batch_step(self._h, self._out, len(self._out), ctypes.byref(n), e, len(e))
+        if rc != 0:
+            raise OgError(e.value.decode())
+        raw = np.frombuffer(self._out, dtype=np.uint64, count=n.value * (ctypes.sizeof(_Hit) // 8))
+        return raw.reshape(n.value, ctypes.sizeof(_Hit) // 8)[:, 0].copy()
 
     def search_state(self, sid):
         tag, examined, hits = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()

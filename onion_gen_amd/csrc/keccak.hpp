@@ -19,11 +19,24 @@ OG_HD uint64_t rotl64(uint64_t x, int n) { return n == 0 ? x : (x << n) | (x >> 
    0x000000000000800aull, 0x800000008000000aull, 0x8000000080008081ull,                 \
    0x8000000000008080ull, 0x0000000080000001ull, 0x8000000080008008ull}
 
+// The round constants: in constant memory for the device (the round index is
+// wave-uniform, so each is one scalar load), a static table for the host.  A
+// local array indexed by the loop counter would live in scratch memory.
+static __constant__ uint64_t kKeccakRcDev[24] = OG_KECCAK_RC;
+static const uint64_t kKeccakRcHost[24] = OG_KECCAK_RC;
+
+OG_HD uint64_t keccak_rc(int round) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return kKeccakRcDev[round];
+#else
+  return kKeccakRcHost[round];
+#endif
+}
+
 // The 24 rounds over a 5x5 lane state a[x + 5y].  Rho offsets and the pi
 // permutation are written out so that every index is a compile-time constant
 // and the state stays in registers.
 OG_HD void keccak_f1600(uint64_t a[25]) {
-  const uint64_t rc[24] = OG_KECCAK_RC;
 #pragma unroll 1
   for (int round = 0; round < 24; ++round) {
     uint64_t c[5], d[5], b[25];
@@ -65,7 +78,7 @@ OG_HD void keccak_f1600(uint64_t a[25]) {
 #pragma unroll
       for (int x = 0; x < 5; ++x) a[y + x] = b[y + x] ^ (~b[y + (x + 1) % 5] & b[y + (x + 2) % 5]);
     }
-    a[0] ^= rc[round];
+    a[0] ^= keccak_rc(round);
   }
 }
 

@@ -3,12 +3,16 @@
 // A batch keeps any number of *resident* searches on the device - each with
 // its own chain state (chains x 160 B in HBM), filter tables and next
 // candidate index - and runs `slots` of them per step.  One step is a captured
-// HIP graph: clear the survivor counter, run every chain of every scheduled
-// search for `rounds` windows, copy the counter back.  Before each launch the
+// HIP graph: clear the survivor counters, run every chain of every scheduled
+// search for `rounds` windows (two kernels, one per filter family, as parallel
+// branches), copy the counters back.  Before each launch the
 // host writes the per-slot descriptors (which search, its state, its filter,
-// its k base), so scheduling never re-captures the graph.  Resident searches
-// take the slots round-robin: with more searches than slots the device is
-// time-sliced between them, and no search waits for another to finish.
+// its k base), so scheduling never re-captures the graph.  With more resident
+// searches than slots the device is time-sliced between them and no search
+// waits for another to finish.  The default policy gives the slots to the
+// searches that have examined the fewest candidates (least attained service):
+// a new search runs at the next step, which keeps time-to-first-hit short when
+// job sizes span many orders of magnitude; round-robin is the alternative.
 // Searches of different filter kinds share the launch; each block reads its
 // slot's descriptor once.
 //
@@ -37,15 +41,16 @@ struct SlotDesc {
   DevFilterArgs f;      // f.mode == 0: no search in this slot this step
   uint64_t k_base;      // first candidate of this step for the slot's search
   uint32_t *state;      // the search's chains: [4 coords][10 limbs][chains]
+  uint32_t budget;      // hit records the search may still write this step
 };
 
 struct BatchArgs {
   const uint32_t *table;
-  uint32_t *scratch;    // [kPlanesPerPair*half][10][slots*chains]
+  uint32_t *scratch;    // [scratch_planes(half)][10][slots*chains]
   const SlotDesc *slots;
   uint32_t nslots, chains, half, rounds;
   uint32_t stride[30];
-  uint32_t *counter;
+  uint32_t *counter;    // [0]: ring records; [1 + slot]: every hit of the slot
   HitRec *hits;
   uint32_t cap;
 };
@@ -61,38 +66,38 @@ __device__ __forceinline__ void batch_body(const BatchArgs &a, const SlotDesc &d
     st.ymx.v[l] = a.stride[10 + l];
     st.t2d.v[l] = a.stride[20 + l];
   }
-  HitSink sink{a.counter, a.hits, a.cap, slot, nullptr};
+  HitSink sink{a.counter, a.hits, a.cap, slot, a.counter + 1, d.budget};
   search_chain<MODE>(P, st, (cptr)a.table, a.scratch, a.nslots * a.chains,
                      slot * a.chains + local, a.half, a.rounds, a.chains, local, d.k_base, d.f,
                      banks, sink);
   st_point(d.state, a.chains, local, P);
 }
 
+// One kernel per filter family, so that each gets its own register
+// allocation: the key forms (prefix, bitmap) never carry the registers of the
+// checksum path.  Both run over every slot; a block whose slot belongs to the
+// other family (or is empty) returns at once.
+template <bool kPatternFamily>
 __global__ __launch_bounds__(256) OG_SEARCH_ATTR void og_batch_kernel(BatchArgs a) {
   const uint32_t blocks_per_slot = a.chains / 256;
   const uint32_t slot = blockIdx.x / blocks_per_slot;
   const uint32_t local = (blockIdx.x % blocks_per_slot) * 256 + threadIdx.x;
   const SlotDesc d = a.slots[slot];  // block-uniform
-  __shared__ ScanBank banks[kMaxBanks];
-  if (d.f.mode & kModePattern) load_banks(banks, d.f);
-  switch (d.f.mode) {
-    case kModePrefix: batch_body<kModePrefix>(a, d, slot, local, banks); break;
-    case kModeBitmap: batch_body<kModeBitmap>(a, d, slot, local, banks); break;
-    case kModePattern: batch_body<kModePattern>(a, d, slot, local, banks); break;
-    case kModePrefix | kModePattern:
-      batch_body<kModePrefix | kModePattern>(a, d, slot, local, banks);
-      break;
-    case kModeBitmap | kModePattern:
-      batch_body<kModeBitmap | kModePattern>(a, d, slot, local, banks);
-      break;
-    default: break;  // an empty slot
+  if (((d.f.mode & kModePattern) != 0) != kPatternFamily) return;
+  constexpr uint32_t kAnyKey = kModePrefix | kModeBitmap;  // the key form read at run time
+  if (kPatternFamily) {
+    __shared__ ScanBank banks[kMaxBanks];
+    load_banks(banks, d.f);
+    batch_body<kModePattern | kAnyKey>(a, d, slot, local, banks);
+  } else if (d.f.mode) {
+    batch_body<kAnyKey>(a, d, slot, local, nullptr);
   }
 }
 
-// Clears the survivor counter (a kernel rather than a memset node: the graph
-// is replayed many times).
-__global__ void og_batch_clear(uint32_t *counter) {
-  if (threadIdx.x == 0) counter[0] = 0;
+// Clears the ring counter and the per-slot counts (a kernel rather than a
+// memset node: the graph is replayed many times).
+__global__ void og_batch_clear(uint32_t *counter, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) counter[i] = 0;
 }
 
 __global__ __launch_bounds__(256) void og_batch_setup(SetupArgs a) {
@@ -123,7 +128,10 @@ struct Search {
   uint64_t tag = 0;
   uint64_t k_next = 0;    // next candidate index of this search
   uint64_t examined = 0;
-  uint64_t hits = 0;
+  uint64_t hits = 0;      // hits recorded (written to the caller)
+  uint64_t found = 0;     // hits found, recorded or not
+  uint64_t want = ~0ull;  // record at most this many hits in all
+  uint64_t seq = 0;       // admission order
   uint32_t *d_state = nullptr;
   DevFilterHost filt;
   size_t bytes = 0;       // device bytes held (state + tables)
@@ -134,6 +142,8 @@ struct Search {
 struct og_batch {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;         // the pattern family's branch of the step
+  hipEvent_t fork = nullptr, join = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -147,7 +157,9 @@ struct og_batch {
   HitRec *h_hits = nullptr;
   std::vector<Search> searches;       // indexed by search id
   std::vector<uint32_t> free_ids;
-  std::deque<uint32_t> runnable;      // round-robin order
+  std::deque<uint32_t> runnable;      // resident searches, in round-robin order
+  std::vector<uint32_t> pick;         // this step's choice
+  uint64_t admitted = 0;              // admission counter
   std::vector<int64_t> assigned;      // search id per slot in the last step, -1 empty
   BatchArgs args;
   HostCurve curve;
@@ -183,6 +195,9 @@ void free_batch(og_batch *b) {
   (void)hipHostFree(b->h_hits);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->join) (void)hipEventDestroy(b->join);
+  if (b->side) (void)hipStreamDestroy(b->side);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -219,6 +234,10 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
             "chains must be a multiple of 256, slots*chains at most 2^22, half at most 4096");
     return -1;
   }
+  if (o.policy > OG_POLICY_ROUND_ROBIN) {
+    put_err(err, errcap, "unknown scheduling policy " + std::to_string(o.policy));
+    return -1;
+  }
   std::unique_ptr<og_batch, void (*)(og_batch *)> b(new og_batch(), free_batch);
   b->device = ordinal;
   b->o = o;
@@ -230,14 +249,17 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
   OG_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   OG_TRY(hipEventCreate(&b->ev0));
   OG_TRY(hipEventCreate(&b->ev1));
+  OG_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+  OG_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  OG_TRY(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
   const std::vector<uint32_t> table = b->curve.offset_table10(o.half);
   OG_TRY(hipMalloc(&b->d_table, std::max<size_t>(table.size(), 30) * 4));
-  OG_TRY(hipMalloc(&b->d_scratch, (size_t)kPlanesPerPair * o.half * 10 * T * 4));
+  OG_TRY(hipMalloc(&b->d_scratch, scratch_planes(o.half) * 10 * T * 4));
   OG_TRY(hipMalloc(&b->d_slots, o.slots * sizeof(SlotDesc)));
   OG_TRY(hipHostMalloc(&b->h_slots, o.slots * sizeof(SlotDesc), hipHostMallocDefault));
-  OG_TRY(hipMalloc(&b->d_counter, 4));
+  OG_TRY(hipMalloc(&b->d_counter, (1 + (size_t)o.slots) * 4));
   OG_TRY(hipMalloc(&b->d_hits, (size_t)o.ring_cap * sizeof(HitRec)));
-  OG_TRY(hipHostMalloc(&b->h_counter, 4, hipHostMallocDefault));
+  OG_TRY(hipHostMalloc(&b->h_counter, (1 + (size_t)o.slots) * 4, hipHostMallocDefault));
   OG_TRY(hipHostMalloc(&b->h_hits, (size_t)o.ring_cap * sizeof(HitRec), hipHostMallocDefault));
   if (!table.empty())
     OG_TRY(hipMemcpyAsync(b->d_table, table.data(), table.size() * 4, hipMemcpyHostToDevice,
@@ -264,11 +286,20 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
   a.hits = b->d_hits;
   a.cap = o.ring_cap;
 
-  // Capture one step.
+  // Capture one step: clear, then the two family kernels as parallel branches
+  // (fork/join through events), then the counters back to the host.
   OG_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  hipLaunchKernelGGL(og_batch_clear, dim3(1), dim3(64), 0, b->stream, b->d_counter);
-  hipLaunchKernelGGL(og_batch_kernel, dim3((unsigned)(T / 256)), dim3(256), 0, b->stream, a);
-  (void)hipMemcpyAsync(b->h_counter, b->d_counter, 4, hipMemcpyDeviceToHost, b->stream);
+  hipLaunchKernelGGL(og_batch_clear, dim3(1), dim3(256), 0, b->stream, b->d_counter,
+                     1u + o.slots);
+  (void)hipEventRecord(b->fork, b->stream);
+  (void)hipStreamWaitEvent(b->side, b->fork, 0);
+  hipLaunchKernelGGL(og_batch_kernel<false>, dim3((unsigned)(T / 256)), dim3(256), 0, b->stream,
+                     a);
+  hipLaunchKernelGGL(og_batch_kernel<true>, dim3((unsigned)(T / 256)), dim3(256), 0, b->side, a);
+  (void)hipEventRecord(b->join, b->side);
+  (void)hipStreamWaitEvent(b->stream, b->join, 0);
+  (void)hipMemcpyAsync(b->h_counter, b->d_counter, (1 + (size_t)o.slots) * 4,
+                       hipMemcpyDeviceToHost, b->stream);
   OG_TRY(hipStreamEndCapture(b->stream, &b->graph));
   OG_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   *out = b.release();
@@ -328,6 +359,7 @@ int og_batch_admit(og_batch *b, const uint8_t start_xyzt[128], const og_filterse
   }
   s.live = true;
   s.tag = tag;
+  s.seq = b->admitted++;
   s.bytes = (size_t)b->o.chains * 160 + s.filt.device_bytes();
   b->resident_bytes += s.bytes;
   b->runnable.push_back(id);
@@ -359,19 +391,39 @@ int og_batch_step(og_batch *b, og_batch_hit *out, uint32_t cap, uint32_t *n, cha
   }
   *n = 0;
   OG_TRY(hipSetDevice(b->device));
-  // Round-robin: the next `slots` resident searches take the slots.
-  const size_t take = std::min<size_t>(b->o.slots, b->runnable.size());
+  // The policy picks which resident searches take the slots this step.
+  std::vector<uint32_t> &pick = b->pick;
+  if (b->o.policy == OG_POLICY_ROUND_ROBIN) {
+    pick.clear();
+    const size_t n_rr = std::min<size_t>(b->o.slots, b->runnable.size());
+    for (size_t s = 0; s < n_rr; ++s) {
+      pick.push_back(b->runnable.front());
+      b->runnable.pop_front();
+      b->runnable.push_back(pick.back());
+    }
+  } else {
+    pick.assign(b->runnable.begin(), b->runnable.end());
+    auto served_less = [b](uint32_t x, uint32_t y) {
+      const Search &p = b->searches[x], &q = b->searches[y];
+      return p.examined != q.examined ? p.examined < q.examined : p.seq < q.seq;
+    };
+    if (pick.size() > b->o.slots) {
+      std::nth_element(pick.begin(), pick.begin() + b->o.slots, pick.end(), served_less);
+      pick.resize(b->o.slots);
+    }
+  }
+  const size_t take = pick.size();
   std::memset(b->h_slots, 0, b->o.slots * sizeof(SlotDesc));
   for (size_t s = 0; s < b->o.slots; ++s) b->assigned[s] = -1;
   for (size_t s = 0; s < take; ++s) {
-    const uint32_t id = b->runnable.front();
-    b->runnable.pop_front();
-    b->runnable.push_back(id);
+    const uint32_t id = pick[s];
     Search &sr = b->searches[id];
     b->assigned[s] = id;
     b->h_slots[s].f = sr.filt.args();
     b->h_slots[s].k_base = sr.k_next;
     b->h_slots[s].state = sr.d_state;
+    const uint64_t left = sr.want > sr.hits ? sr.want - sr.hits : 0;
+    b->h_slots[s].budget = (uint32_t)std::min<uint64_t>(left, 0xffffffffu);
   }
   OG_TRY(hipMemcpyAsync(b->d_slots, b->h_slots, b->o.slots * sizeof(SlotDesc),
                         hipMemcpyHostToDevice, b->stream));
@@ -385,6 +437,7 @@ int og_batch_step(og_batch *b, og_batch_hit *out, uint32_t cap, uint32_t *n, cha
     Search &sr = b->searches[b->assigned[s]];
     sr.k_next += b->per_step;
     sr.examined += b->per_step;
+    sr.found += b->h_counter[1 + s];
   }
   const uint32_t found = *b->h_counter;
   const uint32_t kept = std::min(found, b->o.ring_cap);
@@ -429,6 +482,17 @@ int og_batch_search_state(const og_batch *b, int id, uint64_t *tag, uint64_t *ex
   if (hits) *hits = s.hits;
   if (live) *live = s.live ? 1 : 0;
   return 0;
+}
+
+int og_batch_set_want(og_batch *b, int id, uint64_t want) {
+  if (!b || id < 0 || (size_t)id >= b->searches.size() || !b->searches[id].live) return -1;
+  b->searches[id].want = want;
+  return 0;
+}
+
+uint64_t og_batch_search_found(const og_batch *b, int id) {
+  if (!b || id < 0 || (size_t)id >= b->searches.size()) return 0;
+  return b->searches[id].found;
 }
 
 uint64_t og_batch_per_step(const og_batch *b) { return b ? b->per_step : 0; }

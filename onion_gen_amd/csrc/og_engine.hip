@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) OG_SEARCH_ATTR void og_search_kernel(SearchArg
   }
   __shared__ ScanBank banks[(MODE & kModePattern) ? kMaxBanks : 1];
   if (MODE & kModePattern) load_banks(banks, a.f);
-  HitSink sink{a.counter, a.hits, a.cap, 0, nullptr};
+  HitSink sink{a.counter, a.hits, a.cap, 0, nullptr, 0};
   search_chain<MODE>(P, st, (cptr)a.table, a.scratch, T, g, a.half, a.rounds, T, g, a.k_base, a.f,
                      banks, sink);
   st_point(a.state, T, g, P);
@@ -302,7 +302,7 @@ int create_engine(int ordinal, const uint8_t start_xyzt[128], const og_engine_op
   put_fe10(a.stride + 20, stride.t2d);
 
   // ---- buffers
-  const size_t scratch_words = (size_t)kPlanesPerPair * o.half * 10 * o.threads;
+  const size_t scratch_words = scratch_planes(o.half) * 10 * o.threads;
   e->cap = o.hit_cap ? o.hit_cap
                      : og_hit_room((uint64_t)o.threads * S * o.rounds, e->filt.bits ? e->filt.bits : 24);
   OG_TRY(hipMalloc(&e->d_state, (size_t)40 * o.threads * 4));

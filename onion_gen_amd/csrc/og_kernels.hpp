@@ -50,8 +50,10 @@ enum : uint32_t {
   kModePattern = 4,  // general forms: the bank automaton of og_scan.hpp
 };
 
-// Scratch planes (field elements of 10 limbs) per pair: D0, D1, C.
+// Scratch planes (field elements of 10 limbs) per pair: D0, D1, C; a window
+// takes one more, where the first deferred address-stage candidate parks.
 constexpr uint32_t kPlanesPerPair = 3;
+inline size_t scratch_planes(uint32_t half) { return (size_t)kPlanesPerPair * half + 1; }
 
 constexpr int kMaxBuckets = 16;
 constexpr int kMaxPatterns = 64;  // symbol-wise records; the banks bound the set
@@ -74,7 +76,7 @@ struct DevFilterArgs {
 struct SearchArgs {
   uint32_t *state;        // [4 coords][10 limbs][threads]
   const uint32_t *table;  // [half][xq, yq, dq][10]
-  uint32_t *scratch;      // [kPlanesPerPair*half][10][threads]
+  uint32_t *scratch;      // [scratch_planes(half)][10][threads]
   uint32_t threads, half, rounds;
   uint64_t k_base;        // first candidate of this launch
   uint32_t stride[30];    // (y+x, y-x, 2dxy) of threads*(2H+1)*8G, Z = 1

@@ -62,25 +62,40 @@ __device__ __forceinline__ bool bitmap_bit(const DevFilterArgs &f, uint32_t head
 // The cheap test every candidate gets: on the low 32 bits of the key (its
 // first four bytes), for both the value and the value minus p.  Never false
 // for a candidate the exact test accepts.
+// A MODE with both key bits set leaves the key form to the filter at run time
+// (f.mode, wave-uniform): one instantiation serves every key form, which keeps
+// the slot engine's kernels to one search loop each.
+template <uint32_t MODE>
+__device__ __forceinline__ bool key_is_prefix(const DevFilterArgs &f) {
+  constexpr bool kBoth = (MODE & kModePrefix) && (MODE & kModeBitmap);
+  return kBoth ? (f.mode & kModePrefix) != 0 : (MODE & kModePrefix) != 0;
+}
+
+template <uint32_t MODE>
+__device__ __forceinline__ bool key_is_bitmap(const DevFilterArgs &f) {
+  constexpr bool kBoth = (MODE & kModePrefix) && (MODE & kModeBitmap);
+  return kBoth ? (f.mode & kModeBitmap) != 0 : (MODE & kModeBitmap) != 0;
+}
+
 template <uint32_t MODE>
 __device__ __forceinline__ bool maybe_key(const Fe10 &y, const DevFilterArgs &f) {
   const uint32_t lo = fe10_low32(y), lo19 = lo + 19u;
-  if (MODE & kModePrefix)
+  if (key_is_prefix<MODE>(f))
     return (((lo ^ f.want[0]) & f.mask[0]) == 0) | (((lo19 ^ f.want[0]) & f.mask[0]) == 0);
-  if (MODE & kModeBitmap) return bitmap_bit(f, bswap32(lo)) | bitmap_bit(f, bswap32(lo19));
+  if (key_is_bitmap<MODE>(f)) return bitmap_bit(f, bswap32(lo)) | bitmap_bit(f, bswap32(lo19));
   return false;
 }
 
 // The exact key-side test on the canonical key (sign clear).
 template <uint32_t MODE>
 __device__ __forceinline__ bool exact_key(const Fe &c, const DevFilterArgs &f) {
-  if (MODE & kModePrefix) {
+  if (key_is_prefix<MODE>(f)) {
     uint32_t diff = 0;
 #pragma unroll
     for (int l = 0; l < 8; ++l) diff |= (c.v[l] ^ f.want[l]) & f.mask[l];
     return diff == 0;
   }
-  if (MODE & kModeBitmap) {
+  if (key_is_bitmap<MODE>(f)) {
     const uint32_t head = bswap32(c.v[0]);
     if (!bitmap_bit(f, head)) return false;
     if (f.nbuckets == 0) return true;
@@ -104,10 +119,15 @@ __device__ __forceinline__ bool exact_key(const Fe &c, const DevFilterArgs &f) {
 }
 
 // The general forms symbol by symbol on the full address: the exact check for
-// patterns longer than one automaton bank.
-__device__ __forceinline__ bool pattern_test(const uint32_t w[9], const DevFilterArgs &f) {
-  for (uint32_t p = 0; p < f.npat; ++p) {
-    const uint32_t *rec = f.pats + p * kPatWords;
+// patterns longer than one automaton bank.  Out of line, with its own copy of
+// the address, so that its run-time symbol index does not push the caller's
+// words into scratch memory.
+__device__ __noinline__ bool pattern_slow(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3,
+                                          uint32_t w4, uint32_t w5, uint32_t w6, uint32_t w7,
+                                          uint32_t w8, const uint32_t *pats, uint32_t npat) {
+  const uint32_t w[9] = {w0, w1, w2, w3, w4, w5, w6, w7, w8};
+  for (uint32_t p = 0; p < npat; ++p) {
+    const uint32_t *rec = pats + p * kPatWords;
     const uint32_t len = rec[0], first = rec[1], last = rec[2];
     for (uint32_t s = first; s <= last; ++s) {
       bool ok = true;
@@ -145,16 +165,27 @@ struct HitSink {
   HitRec *hits;
   uint32_t cap;
   uint32_t slot;
-  uint32_t *slot_hits;  // per-slot hit counters, or null
+  uint32_t *slot_found;  // per-slot counts of every hit this launch, or null
+  uint32_t budget;       // with slot_found: records the slot may still write
 };
 
+// A hit is always counted; with a per-slot budget only the first `budget` of
+// the slot's hits this launch (in the order the waves count them) are written
+// to the ring, so a search that wants n hits never floods it.
 __device__ __forceinline__ void report(bool hit, uint64_t k, const Fe &c, const HitSink &sink) {
-  if (sink.slot_hits) {
-    const uint64_t m = __ballot(hit);
-    if (m && __lane_id() == (uint32_t)__ffsll((unsigned long long)m) - 1)
-      atomicAdd(sink.slot_hits + sink.slot, (uint32_t)__popcll(m));
+  if (!sink.slot_found) {
+    append(hit, sink.slot, k, c, sink.counter, sink.hits, sink.cap);
+    return;
   }
-  append(hit, sink.slot, k, c, sink.counter, sink.hits, sink.cap);
+  const uint64_t mask = __ballot(hit);
+  if (mask == 0) return;
+  const uint32_t lane = __lane_id();
+  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(sink.slot_found + sink.slot, (uint32_t)__popcll(mask));
+  base = __shfl(base, (int)leader);
+  const uint32_t rank = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+  append(hit && rank < sink.budget, sink.slot, k, c, sink.counter, sink.hits, sink.cap);
 }
 
 // Key forms only: the low-32-bit probe, and the exact test on the lanes the
@@ -183,19 +214,54 @@ __device__ __forceinline__ void key_stage(const Fe &c, const uint32_t w[8], cons
   need &= !hit;
 }
 
-// The address stage for the lanes that need it: the sign of x (bit 0 of its
-// canonical form), SHA3-256 for the checksum, the exact automaton pass.
-__device__ __forceinline__ bool address_stage(const uint32_t kw[8], const Fe10 &x,
-                                              const DevFilterArgs &f, const ScanBank *banks) {
+// The address stage is deferred.  A lane whose key pass leaves a checksum
+// placement possible computes the sign of x (bit 0 of its canonical form) and
+// parks the candidate - packed key with the sign in bit 255, and its index -
+// in a scratch plane the backward pass has already consumed, counting down
+// from the plane past the window's 3H.  After the backward pass the window
+// drains the parked candidates (SHA3-256, the exact automaton pass) with
+// little else live, so the Keccak state never competes with the chain's.
+// Entry j lands in plane 3H - j; after the centre and pairs H-1..m at most
+// 2(H-m)+1 entries exist, and 3H - 2(H-m) >= 3m, so an entry never reaches a
+// plane the backward pass has yet to read.
+__device__ __forceinline__ void park(uint32_t *scratch, uint32_t top, uint32_t T, uint32_t g,
+                                     uint32_t &n, const Fe &c, const Fe10 &x, uint64_t k) {
   const uint32_t sign = fe10_canon(x).v[0] & 1u;
-  uint32_t pk[8];
+  Fe10 e;
 #pragma unroll
-  for (int l = 0; l < 7; ++l) pk[l] = __builtin_bswap32(kw[l]);
-  pk[7] = __builtin_bswap32(kw[7]) | (sign << 31);
-  uint32_t w[9];
-  address_words(kw, sign, onion_checksum(pk), w);
-  const uint32_t r = scan_address(w, banks, f.nbanks);
-  return r == 1 || (r == 2 && pattern_test(w, f));
+  for (int l = 0; l < 8; ++l) e.v[l] = c.v[l];
+  e.v[7] |= sign << 31;
+  e.v[8] = (uint32_t)k;
+  e.v[9] = (uint32_t)(k >> 32);
+  st10(scratch, top - n, T, g, e);
+  ++n;
+}
+
+template <uint32_t MODE>
+__device__ __forceinline__ void drain(const uint32_t *scratch, uint32_t top, uint32_t T,
+                                      uint32_t g, uint32_t n, const DevFilterArgs &f,
+                                      const ScanBank *banks, const HitSink &sink) {
+  for (uint32_t j = 0; j < n; ++j) {
+    const Fe10 e = ld10(scratch, top - j, T, g);  // j is uniform over the active lanes
+    uint32_t pk[8];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) pk[l] = e.v[l];
+    uint32_t w[9];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) w[l] = __builtin_bswap32(pk[l]);  // big-endian, sign included
+    const uint32_t chk = onion_checksum(pk);
+    w[8] = ((chk & 0xffu) << 24) | (((chk >> 8) & 0xffu) << 16) | (0x03u << 8);
+    const uint32_t r = scan_address(w, banks, f.nbanks);
+    const bool hit =
+        r == 1 ||
+        (r == 2 && pattern_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8], f.pats,
+                                f.npat));
+    Fe c;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) c.v[l] = pk[l];
+    c.v[7] &= 0x7fffffffu;  // survivors carry the key with the sign clear
+    report(hit, (uint64_t)e.v[8] | ((uint64_t)e.v[9] << 32), c, sink);
+  }
 }
 
 // `rounds` windows of one chain (the method is described in og_kernels.hpp).
@@ -229,6 +295,8 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
     // inv = 1/(D Z): the centre takes 1/Z = inv D, and inv Z = 1/D starts
     // the backward pass
     Fe10 inv = fe10_invert(fe10_mul(D, P.Z));
+    const uint32_t top = kPlanesPerPair * H;  // the parking plane past the window's
+    uint32_t parked = 0;
     {
       const Fe10 iz = fe10_mul(inv, D);
       const Fe10 y = fe10_mul(P.Y, iz);
@@ -240,7 +308,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         key_stage<MODE>(c, kw, f, banks, hit, need);
         if (__ballot(need)) {
           const Fe10 x = fe10_mul(P.X, iz);
-          if (need) hit = address_stage(kw, x, f, banks);
+          if (need) park(scratch, top, T, g, parked, c, x, kc);
         }
         report(hit, kc, c, sink);
       } else {
@@ -287,8 +355,8 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
           // x(P - Q) has the denominator dp, x(P + Q) the denominator dm
           const Fe10 XY = fe10_mul(P.X, yq);
           const Fe10 YX = fe10_mul(P.Y, xq);
-          if (nm) hm = address_stage(wm, fe10_mul(fe10_sub(XY, YX), idp), f, banks);
-          if (np) hp = address_stage(wp, fe10_mul(fe10_add(XY, YX), idm), f, banks);
+          if (nm) park(scratch, top, T, g, parked, cm, fe10_mul(fe10_sub(XY, YX), idp), km);
+          if (np) park(scratch, top, T, g, parked, cp, fe10_mul(fe10_add(XY, YX), idm), kp);
         }
         report(hm, km, cm, sink);
         report(hp, kp, cp, sink);
@@ -297,6 +365,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         test_key_only<MODE>(yp, kp, f, sink);
       }
     }
+    if (kPat) drain<MODE>(scratch, top, T, g, parked, f, banks, sink);
     P = pt10_add1(P, st);
   }
 }

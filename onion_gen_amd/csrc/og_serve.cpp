@@ -4,9 +4,11 @@
 // once (its chain state and filter tables are set up) when its footprint fits
 // the device budget, else it waits in a bounded FIFO queue, and an arrival
 // that finds the queue full is rejected (the back-pressure policy).  Every
-// iteration runs one batch step, in which the resident requests take the
-// slots round-robin, credits each survivor to its request, and retires a
-// request the step its n-th hit arrives, which frees its memory for the queue.
+// iteration runs one batch step, in which the slot engine's policy picks the
+// resident requests that run (least served first by default), credits each
+// survivor to its request, and retires a request the step its n-th hit
+// arrives, which frees its memory for the queue.  A request records only the
+// hits it asked for (og_batch_set_want), so easy filters cannot flood the ring.
 // A request's chain state and filter tables stay on the device for its whole
 // life; no request waits for another to finish.
 //
@@ -113,6 +115,8 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
         continue;
       }
       queue.pop_front();
+      // the request records only the hits it asked for
+      if (reqs[r].want_hits) (void)og_batch_set_want(b, id, reqs[r].want_hits);
       if ((size_t)id >= live.size()) live.resize(id + 1, Live{0, nullptr, 0});
       live[id] = Live{r, fs, bytes};
       used += bytes;
@@ -175,6 +179,7 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
       uint64_t ex = 0;
       og_batch_search_state(b, (int)id, nullptr, &ex, nullptr, nullptr);
       res.examined = ex;
+      res.seen = og_batch_search_found(b, (int)id);
       if (res.hits >= reqs[l.req].want_hits) {
         res.done_s = t_step;
         res.status = OG_REQ_DONE;

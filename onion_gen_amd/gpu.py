@@ -66,9 +66,10 @@ class Engine:
     that takes a compiled :class:`onion_gen_amd.filters.FilterSet`.
     """
 
-    def __init__(self, handle):
+    def __init__(self, handle, room=1 << 18):
         self._h = handle
         self._out = None
+        self._room = room
 
     @classmethod
     def new(cls, start_xyzt, threads, bitmap, bitmap_bits, device=0):
@@ -90,7 +91,7 @@ class Engine:
                                            filters.handle, ctypes.byref(h), e, len(e))
         if rc != 0:
             raise OgError(e.value.decode())
-        return cls(h)
+        return cls(h, max(1 << 18, hit_cap))
 
     def threads(self):
         return lib().og_hip_engine_threads(self._h)
@@ -101,8 +102,10 @@ class Engine:
     def last_ms(self):
         return lib().og_hip_engine_last_ms(self._h)
 
-    def advance(self, cap=1 << 18):
-        """Examines the next ``batch()`` candidates; returns the survivors."""
+    def advance(self, cap=0):
+        """Examines the next ``batch()`` candidates; returns the survivors
+        (room for ``cap`` of them; by default the engine's survivor room)."""
+        cap = cap or self._room
         if self._out is None or len(self._out) < cap:
             self._out = (_Survivor * cap)()
         n = ctypes.c_uint32()

@@ -2,7 +2,7 @@
 import ctypes
 
 from ._lib import OgError, err_buf, lib
-from .batch import _Opts
+from .batch import LEAST_SERVED, _Opts
 
 
 class Request(ctypes.Structure):
@@ -18,7 +18,7 @@ class Result(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("hits", ctypes.c_uint32),
                 ("examined", ctypes.c_uint64), ("admitted_s", ctypes.c_double),
                 ("first_hit_s", ctypes.c_double), ("done_s", ctypes.c_double),
-                ("found", _Found * 16)]
+                ("found", _Found * 16), ("seen", ctypes.c_uint64)]
 
 
 class Stats(ctypes.Structure):
@@ -36,7 +36,7 @@ STATUS = {0: "not_arrived", 1: "queued", 2: "running", 3: "done", 4: "rejected"}
 
 
 def run(requests, duration, slots=64, chains=2048, half=256, rounds=1, max_queue=256,
-        device_budget=0, device=0):
+        device_budget=0, policy=LEAST_SERVED, ring_cap=1 << 20, device=0):
     """``requests``: list of (arrival_s, seed32, filter_text, want_hits), sorted
     by arrival.  Returns (stats dict, list of result dicts)."""
     n = len(requests)
@@ -49,16 +49,17 @@ def run(requests, duration, slots=64, chains=2048, half=256, rounds=1, max_queue
     results = (Result * max(n, 1))()
     stats = Stats()
     e = err_buf()
-    rc = lib().og_serve_run(device, ctypes.byref(_Opts(slots, chains, half, rounds, 0)), reqs, n,
-                            duration, max_queue, device_budget, results, ctypes.byref(stats), e,
-                            len(e))
+    opts = _Opts(slots, chains, half, rounds, ring_cap, policy)
+    rc = lib().og_serve_run(device, ctypes.byref(opts), reqs, n, duration, max_queue,
+                            device_budget, results, ctypes.byref(stats), e, len(e))
     if rc != 0:
         raise OgError(e.value.decode())
     st = {f: getattr(stats, f) for f, _ in Stats._fields_}
     out = []
     for i in range(n):
         r = results[i]
-        out.append({"status": STATUS[r.status], "hits": r.hits, "examined": r.examined,
+        out.append({"status": STATUS[r.status], "hits": r.hits, "seen": r.seen,
+                    "examined": r.examined,
                     "admitted_s": r.admitted_s, "first_hit_s": r.first_hit_s,
                     "done_s": r.done_s,
                     "found": [(r.found[j].offset, bytes(r.found[j].packed))

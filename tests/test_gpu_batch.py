@@ -115,3 +115,28 @@ def test_more_searches_than_slots_share_the_device_round_robin():
         want = oracle(seeds[i], ["2x", "suffix:aqd"], st["examined"])
         assert sorted(got[i]) == want and want, i
     b.close()
+
+
+def test_hit_quota_records_only_what_the_search_wants():
+    """A search that wants 5 hits records 5 (real ones, each among the
+    oracle's); every hit is still counted; the search beside it, with no
+    quota, records all of its own."""
+    say("case: quota 5 on an easy filter, 2 slots x 256 chains, half 32")
+    b = batch.Batch(slots=2, chains=256, half=32, rounds=1)
+    s1, s2 = hashlib.sha256(b"quota").digest(), hashlib.sha256(b"free").digest()
+    f = filters.FilterSet.parse("suffix:aqd")
+    a = b.admit(key.start_point(s1), f, 1)
+    c = b.admit(key.start_point(s2), f, 2)
+    b.set_want(a, 5)
+    got = {1: [], 2: []}
+    for _ in range(4):
+        for h in b.step():
+            got[h.tag].append((h.offset // 8, h.packed))
+    n = b.search_state(a)["examined"]
+    want1 = oracle(s1, ["suffix:aqd"], n)
+    want2 = oracle(s2, ["suffix:aqd"], n)
+    assert len(want1) > 5
+    assert len(got[1]) == 5 and set(got[1]) <= set(want1)
+    assert b.found(a) == len(want1) and b.search_state(a)["hits"] == 5
+    assert sorted(got[2]) == want2 and b.found(c) == len(want2)
+    b.close()
