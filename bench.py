@@ -58,19 +58,16 @@ def run_mix(searches, seconds, slots, chains, half, policy, ring_cap=1 << 20):
         sid = b.admit(key.start_point(seed), filters.FilterSet.parse(text), i)
         b.set_want(sid, want)
         tag_of[sid] = i
-    per = b.per_step()
     hits = [0] * len(searches)
     first, done = [None] * len(searches), [None] * len(searches)
     live = set(tag_of)
-    examined = steps = 0
+    steps = 0
     device_ms = 0.0
     t0 = time.perf_counter()
     while live and time.perf_counter() - t0 < seconds:
-        ran = min(len(live), slots)
         tags = b.step_tags()
         now = time.perf_counter() - t0
         steps += 1
-        examined += ran * per
         device_ms += b.last_ms()
         if len(tags):
             for t, c in zip(*np.unique(tags, return_counts=True)):
@@ -82,6 +79,7 @@ def run_mix(searches, seconds, slots, chains, half, policy, ring_cap=1 << 20):
             b.retire(sid)
             live.discard(sid)
     wall = time.perf_counter() - t0
+    examined = b.examined()
     b.close()
     got = [t for t in first if t is not None]
     return {"keys_per_s": examined / wall, "device_keys_per_s": examined / (device_ms * 1e-3),

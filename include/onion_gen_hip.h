@@ -201,8 +201,14 @@ int og_batch_step(og_batch *b, og_batch_hit *out, uint32_t cap, uint32_t *n, cha
                   size_t errcap);
 int og_batch_search_state(const og_batch *b, int id, uint64_t *tag, uint64_t *examined,
                           uint64_t *hits, int *live);
-/* Candidates one step examines per scheduled search. */
+/* Candidates one step examines per unit slot a search takes. */
 uint64_t og_batch_per_step(const og_batch *b);
+/* Unit slots a resident search takes per step (its chains are this many times
+ * the unit's); 0 when not resident.  With fewer resident searches than slots
+ * each is laid out wider from its next candidate, so the device stays full. */
+uint32_t og_batch_search_width(const og_batch *b, int id);
+/* Candidates examined by every step so far, over all searches. */
+uint64_t og_batch_examined(const og_batch *b);
 /* Device bytes held by resident searches. */
 uint64_t og_batch_resident_bytes(const og_batch *b);
 /* Device time of the last step (HIP events around the graph launch). */

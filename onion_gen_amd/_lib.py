@@ -75,6 +75,8 @@ SIGNATURES.update({
                                   _c.POINTER(_c.c_int), _c.c_char_p, _c.c_size_t]),
     "og_batch_set_want": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_uint64]),
     "og_batch_search_found": (_c.c_uint64, [_c.c_void_p, _c.c_int]),
+    "og_batch_search_width": (_c.c_uint32, [_c.c_void_p, _c.c_int]),
+    "og_batch_examined": (_c.c_uint64, [_c.c_void_p]),
     "og_batch_retire": (_c.c_int, [_c.c_void_p, _c.c_int]),
     "og_batch_step": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_uint32, _c.POINTER(_c.c_uint32),
                                  _c.c_char_p, _c.c_size_t]),

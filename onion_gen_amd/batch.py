@@ -75,6 +75,14 @@ class Batch:
     def found(self, sid):
         return lib().og_batch_search_found(self._h, sid)
 
+    def width(self, sid):
+        """Unit slots the search takes per step (0 when not resident)."""
+        return lib().og_batch_search_width(self._h, sid)
+
+    def examined(self):
+        """Candidates examined by every step so far, over all searches."""
+        return lib().og_batch_examined(self._h)
+
     def retire(self, sid):
         if lib().og_batch_retire(self._h, sid) != 0:
             raise OgError(f"could not retire search {sid}")

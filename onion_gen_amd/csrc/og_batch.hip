@@ -37,20 +37,25 @@ using namespace og;
 
 namespace {
 
+// One unit slot of a step.  A search of width w takes w consecutive unit
+// slots; unit i runs its chains [i*chains, (i+1)*chains).
 struct SlotDesc {
-  DevFilterArgs f;      // f.mode == 0: no search in this slot this step
-  uint64_t k_base;      // first candidate of this step for the slot's search
-  uint32_t *state;      // the search's chains: [4 coords][10 limbs][chains]
-  uint32_t budget;      // hit records the search may still write this step
+  DevFilterArgs f;        // f.mode == 0: no search in this slot this step
+  uint64_t k_base;        // first candidate of this step for the slot's search
+  uint32_t *state;        // the search's chains: [4 coords][10 limbs][chains_total]
+  const uint32_t *stride; // (y+x, y-x, 2dxy) of chains_total*(2H+1)*8G, Z = 1
+  uint32_t budget;        // hit records the search may still write this step
+  uint32_t chain_lo;      // the search's first chain in this unit
+  uint32_t chains_total;  // the search's chains
+  uint32_t lead;          // the search's first unit slot: its hit counter
 };
 
 struct BatchArgs {
   const uint32_t *table;
   uint32_t *scratch;    // [scratch_planes(half)][10][slots*chains]
   const SlotDesc *slots;
-  uint32_t nslots, chains, half, rounds;
-  uint32_t stride[30];
-  uint32_t *counter;    // [0]: ring records; [1 + slot]: every hit of the slot
+  uint32_t nslots, chains, half, rounds;  // chains: per unit slot
+  uint32_t *counter;    // [0]: ring records; [1 + slot]: every hit of the search led there
   HitRec *hits;
   uint32_t cap;
 };
@@ -58,19 +63,18 @@ struct BatchArgs {
 template <uint32_t MODE>
 __device__ __forceinline__ void batch_body(const BatchArgs &a, const SlotDesc &d, uint32_t slot,
                                            uint32_t local, const ScanBank *banks) {
-  Point10 P = ld_point(d.state, a.chains, local);
+  const uint32_t chain = d.chain_lo + local;  // within the search
+  Point10 P = ld_point(d.state, d.chains_total, chain);
+  const cptr stride = (cptr)d.stride;
   Cached10 st;
-#pragma unroll
-  for (int l = 0; l < 10; ++l) {
-    st.ypx.v[l] = a.stride[l];
-    st.ymx.v[l] = a.stride[10 + l];
-    st.t2d.v[l] = a.stride[20 + l];
-  }
-  HitSink sink{a.counter, a.hits, a.cap, slot, a.counter + 1, d.budget};
+  st.ypx = ld10_const(stride);
+  st.ymx = ld10_const(stride + 10);
+  st.t2d = ld10_const(stride + 20);
+  HitSink sink{a.counter, a.hits, a.cap, slot, a.counter + 1 + d.lead, d.budget};
   search_chain<MODE>(P, st, (cptr)a.table, a.scratch, a.nslots * a.chains,
-                     slot * a.chains + local, a.half, a.rounds, a.chains, local, d.k_base, d.f,
-                     banks, sink);
-  st_point(d.state, a.chains, local, P);
+                     slot * a.chains + local, a.half, a.rounds, d.chains_total, chain, d.k_base,
+                     d.f, banks, sink);
+  st_point(d.state, d.chains_total, chain, P);
 }
 
 // One kernel per filter family, so that each gets its own register
@@ -132,6 +136,8 @@ struct Search {
   uint64_t found = 0;     // hits found, recorded or not
   uint64_t want = ~0ull;  // record at most this many hits in all
   uint64_t seq = 0;       // admission order
+  uint32_t width = 0;     // unit slots it takes; chains = width * unit chains
+  uint8_t start[128];     // its s0*B, for laying the chains out again
   uint32_t *d_state = nullptr;
   DevFilterHost filt;
   size_t bytes = 0;       // device bytes held (state + tables)
@@ -159,8 +165,11 @@ struct og_batch {
   std::vector<uint32_t> free_ids;
   std::deque<uint32_t> runnable;      // resident searches, in round-robin order
   std::vector<uint32_t> pick;         // this step's choice
+  std::vector<uint32_t> lead;         // and the first unit slot of each
   uint64_t admitted = 0;              // admission counter
   std::vector<int64_t> assigned;      // search id per slot in the last step, -1 empty
+  std::vector<uint32_t *> d_stride;   // per width: the stride point of its chains
+  uint64_t examined = 0;              // candidates examined by every step
   BatchArgs args;
   HostCurve curve;
   float last_ms = 0;
@@ -185,6 +194,7 @@ void free_batch(og_batch *b) {
     if (s.live) drop_search(s);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (uint32_t *p : b->d_stride) (void)hipFree(p);
   (void)hipFree(b->d_table);
   (void)hipFree(b->d_scratch);
   (void)hipFree(b->d_slots);
@@ -200,6 +210,65 @@ void free_batch(og_batch *b) {
   if (b->side) (void)hipStreamDestroy(b->side);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
+}
+
+// The stride point of a search of `width` unit slots: every chain advances by
+// chains_total * (2H+1) * 8G per window round.  Built on first use.
+hipError_t stride_for(og_batch *b, uint32_t width, const uint32_t **out) {
+  if (!b->d_stride[width]) {
+    const uint64_t S = 2ull * b->o.half + 1;
+    const Cached1 c = b->curve.multiple_of_eight_cached1((uint64_t)width * b->o.chains * S);
+    uint32_t words[30];
+    put_fe10(words, c.ypx);
+    put_fe10(words + 10, c.ymx);
+    put_fe10(words + 20, c.t2d);
+    hipError_t e = hipMalloc(&b->d_stride[width], sizeof words);
+    if (e == hipSuccess)
+      e = hipMemcpy(b->d_stride[width], words, sizeof words, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(b->d_stride[width]);
+      b->d_stride[width] = nullptr;
+      return e;
+    }
+  }
+  *out = b->d_stride[width];
+  return hipSuccess;
+}
+
+// Lays a search's chains out again at `width` unit slots, from its next
+// candidate: chain c's first window starts at k_next + c(2H+1).  Its examined
+// candidates stay the prefix [0, k_next), so nothing is skipped or repeated.
+hipError_t layout(og_batch *b, Search &s, uint32_t width) {
+  const uint32_t chains = width * b->o.chains;
+  const uint32_t *stride = nullptr;
+  hipError_t e = stride_for(b, width, &stride);
+  if (e != hipSuccess) return e;
+  e = hipStreamSynchronize(b->stream);  // no step still reads the old state
+  if (e != hipSuccess) return e;
+  uint32_t *state = nullptr;
+  e = hipMalloc(&state, (size_t)chains * 160);
+  if (e != hipSuccess) return e;
+  SetupArgs sa = b->curve.setup_args(s.start, state, chains, 0, chains, b->o.half, s.k_next);
+  hipLaunchKernelGGL(og_batch_setup, dim3(chains / 256), dim3(256), 0, b->stream, sa);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(state);
+    return e;
+  }
+  (void)hipFree(s.d_state);
+  s.d_state = state;
+  b->resident_bytes -= s.bytes;
+  s.bytes = (size_t)chains * 160 + s.filt.device_bytes();
+  b->resident_bytes += s.bytes;
+  s.width = width;
+  return hipSuccess;
+}
+
+// The width every resident search should have: the unit slots shared out.
+uint32_t target_width(const og_batch *b, size_t live) {
+  if (live == 0) return 1;
+  return (uint32_t)std::max<size_t>(1, std::min<size_t>(b->o.slots, b->o.slots / live));
 }
 
 }  // namespace
@@ -278,10 +347,7 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
   a.chains = o.chains;
   a.half = o.half;
   a.rounds = o.rounds;
-  const Cached1 stride = b->curve.multiple_of_eight_cached1((uint64_t)o.chains * S);
-  put_fe10(a.stride, stride.ypx);
-  put_fe10(a.stride + 10, stride.ymx);
-  put_fe10(a.stride + 20, stride.t2d);
+  b->d_stride.assign(o.slots + 1, nullptr);
   a.counter = b->d_counter;
   a.hits = b->d_hits;
   a.cap = o.ring_cap;
@@ -338,30 +404,20 @@ int og_batch_admit(og_batch *b, const uint8_t start_xyzt[128], const og_filterse
     return -1;
   }
   OG_TRY(hipSetDevice(b->device));
-  hipError_t e = hipMalloc(&s.d_state, (size_t)b->o.chains * 160);
-  if (e == hipSuccess) e = s.filt.to_device(b->stream);
+  std::memcpy(s.start, start_xyzt, 128);
+  hipError_t e = s.filt.to_device(b->stream);
+  if (e == hipSuccess) e = layout(b, s, target_width(b, b->runnable.size() + 1));
   if (e != hipSuccess) {
+    b->resident_bytes -= s.bytes;
+    s.bytes = 0;
     drop_search(s);
     b->free_ids.push_back(id);
     put_err(err, errcap, std::string("admission: ") + hipGetErrorString(e));
     return -1;
   }
-  SetupArgs sa = b->curve.setup_args(start_xyzt, s.d_state, b->o.chains, 0, b->o.chains,
-                                     b->o.half);
-  hipLaunchKernelGGL(og_batch_setup, dim3(b->o.chains / 256), dim3(256), 0, b->stream, sa);
-  e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-  if (e != hipSuccess) {
-    drop_search(s);
-    b->free_ids.push_back(id);
-    put_err(err, errcap, std::string("chain set-up: ") + hipGetErrorString(e));
-    return -1;
-  }
   s.live = true;
   s.tag = tag;
   s.seq = b->admitted++;
-  s.bytes = (size_t)b->o.chains * 160 + s.filt.device_bytes();
-  b->resident_bytes += s.bytes;
   b->runnable.push_back(id);
   *id_out = (int)id;
   return 0;
@@ -391,39 +447,75 @@ int og_batch_step(og_batch *b, og_batch_hit *out, uint32_t cap, uint32_t *n, cha
   }
   *n = 0;
   OG_TRY(hipSetDevice(b->device));
-  // The policy picks which resident searches take the slots this step.
+  // Widths follow the number of resident searches: with fewer searches than
+  // unit slots each is laid out wider (more chains from its next candidate),
+  // so a lone search still fills the device; under contention wide searches
+  // narrow again.  A search widens only when that gains a quarter or more.
+  const uint32_t wt = target_width(b, b->runnable.size());
+  uint64_t demand = 0;
+  for (uint32_t id : b->runnable) demand += b->searches[id].width;
+  for (uint32_t id : b->runnable) {
+    Search &sr = b->searches[id];
+    const bool widen = sr.width < wt && 4 * sr.width <= 3 * wt;
+    const bool narrow = sr.width > wt && demand > b->o.slots;
+    if (!widen && !narrow) continue;
+    demand -= sr.width;
+    OG_TRY(layout(b, sr, wt));
+    demand += sr.width;
+  }
+  // The policy orders the resident searches; they take unit slots in that
+  // order while their widths fit.
   std::vector<uint32_t> &pick = b->pick;
+  pick.clear();
   if (b->o.policy == OG_POLICY_ROUND_ROBIN) {
-    pick.clear();
-    const size_t n_rr = std::min<size_t>(b->o.slots, b->runnable.size());
-    for (size_t s = 0; s < n_rr; ++s) {
-      pick.push_back(b->runnable.front());
+    uint32_t used = 0;
+    const size_t n_live = b->runnable.size();
+    for (size_t i = 0; i < n_live && used < b->o.slots; ++i) {
+      const uint32_t id = b->runnable.front();
+      if (used + b->searches[id].width > b->o.slots) break;
+      used += b->searches[id].width;
+      pick.push_back(id);
       b->runnable.pop_front();
-      b->runnable.push_back(pick.back());
+      b->runnable.push_back(id);
     }
   } else {
-    pick.assign(b->runnable.begin(), b->runnable.end());
+    std::vector<uint32_t> order(b->runnable.begin(), b->runnable.end());
     auto served_less = [b](uint32_t x, uint32_t y) {
       const Search &p = b->searches[x], &q = b->searches[y];
       return p.examined != q.examined ? p.examined < q.examined : p.seq < q.seq;
     };
-    if (pick.size() > b->o.slots) {
-      std::nth_element(pick.begin(), pick.begin() + b->o.slots, pick.end(), served_less);
-      pick.resize(b->o.slots);
+    const size_t head = std::min<size_t>(order.size(), b->o.slots);
+    std::partial_sort(order.begin(), order.begin() + head, order.end(), served_less);
+    uint32_t used = 0;
+    for (size_t i = 0; i < head; ++i) {
+      const uint32_t w = b->searches[order[i]].width;
+      if (used + w > b->o.slots) continue;
+      used += w;
+      pick.push_back(order[i]);
     }
   }
-  const size_t take = pick.size();
   std::memset(b->h_slots, 0, b->o.slots * sizeof(SlotDesc));
   for (size_t s = 0; s < b->o.slots; ++s) b->assigned[s] = -1;
-  for (size_t s = 0; s < take; ++s) {
-    const uint32_t id = pick[s];
+  b->lead.clear();
+  uint32_t unit = 0;
+  for (uint32_t id : pick) {
     Search &sr = b->searches[id];
-    b->assigned[s] = id;
-    b->h_slots[s].f = sr.filt.args();
-    b->h_slots[s].k_base = sr.k_next;
-    b->h_slots[s].state = sr.d_state;
+    const uint32_t *stride = nullptr;
+    OG_TRY(stride_for(b, sr.width, &stride));
     const uint64_t left = sr.want > sr.hits ? sr.want - sr.hits : 0;
-    b->h_slots[s].budget = (uint32_t)std::min<uint64_t>(left, 0xffffffffu);
+    b->lead.push_back(unit);
+    for (uint32_t i = 0; i < sr.width; ++i, ++unit) {
+      SlotDesc &d = b->h_slots[unit];
+      b->assigned[unit] = id;
+      d.f = sr.filt.args();
+      d.k_base = sr.k_next;
+      d.state = sr.d_state;
+      d.stride = stride;
+      d.budget = (uint32_t)std::min<uint64_t>(left, 0xffffffffu);
+      d.chain_lo = i * b->o.chains;
+      d.chains_total = sr.width * b->o.chains;
+      d.lead = b->lead.back();
+    }
   }
   OG_TRY(hipMemcpyAsync(b->d_slots, b->h_slots, b->o.slots * sizeof(SlotDesc),
                         hipMemcpyHostToDevice, b->stream));
@@ -433,11 +525,13 @@ int og_batch_step(og_batch *b, og_batch_hit *out, uint32_t cap, uint32_t *n, cha
   OG_TRY(hipStreamSynchronize(b->stream));
   OG_TRY(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
   b->steps++;
-  for (size_t s = 0; s < take; ++s) {
-    Search &sr = b->searches[b->assigned[s]];
-    sr.k_next += b->per_step;
-    sr.examined += b->per_step;
-    sr.found += b->h_counter[1 + s];
+  for (size_t i = 0; i < pick.size(); ++i) {
+    Search &sr = b->searches[pick[i]];
+    const uint64_t ran = (uint64_t)sr.width * b->per_step;
+    sr.k_next += ran;
+    sr.examined += ran;
+    b->examined += ran;
+    sr.found += b->h_counter[1 + b->lead[i]];
   }
   const uint32_t found = *b->h_counter;
   const uint32_t kept = std::min(found, b->o.ring_cap);
@@ -494,6 +588,13 @@ uint64_t og_batch_search_found(const og_batch *b, int id) {
   if (!b || id < 0 || (size_t)id >= b->searches.size()) return 0;
   return b->searches[id].found;
 }
+
+uint32_t og_batch_search_width(const og_batch *b, int id) {
+  if (!b || id < 0 || (size_t)id >= b->searches.size() || !b->searches[id].live) return 0;
+  return b->searches[id].width;
+}
+
+uint64_t og_batch_examined(const og_batch *b) { return b ? b->examined : 0; }
 
 uint64_t og_batch_per_step(const og_batch *b) { return b ? b->per_step : 0; }
 

@@ -144,7 +144,7 @@ std::vector<uint32_t> HostCurve::offset_table10(uint32_t half) const {
 }
 
 SetupArgs HostCurve::setup_args(const uint8_t start_xyzt[128], uint32_t *state, uint32_t threads,
-                                uint32_t first, uint32_t count, uint32_t half) const {
+                                uint32_t first, uint32_t count, uint32_t half, uint64_t k0) const {
   SetupArgs sa;
   std::memset(&sa, 0, sizeof sa);
   sa.state = state;
@@ -152,6 +152,7 @@ SetupArgs HostCurve::setup_args(const uint8_t start_xyzt[128], uint32_t *state, 
   sa.first = first;
   sa.count = count;
   sa.half = half;
+  sa.k0 = k0;
   for (int c = 0; c < 4; ++c) {
     Fe v = fe_from_bytes(start_xyzt + 32 * c);
     for (int l = 0; l < 8; ++l) sa.start[8 * c + l] = v.v[l];

@@ -22,9 +22,10 @@ struct HostCurve {
   // The same table in the 10-limb form the kernels read: 30 words per entry.
   std::vector<uint32_t> offset_table10(uint32_t half) const;
   // Set-up arguments for chains [first, first + count) of a start point given
-  // as 128 bytes xyzt, in a state buffer of SoA stride `threads`.
+  // as 128 bytes xyzt, in a state buffer of SoA stride `threads`; chain c's
+  // first window starts at candidate k0 + c(2*half+1).
   SetupArgs setup_args(const uint8_t start_xyzt[128], uint32_t *state, uint32_t threads,
-                       uint32_t first, uint32_t count, uint32_t half) const;
+                       uint32_t first, uint32_t count, uint32_t half, uint64_t k0 = 0) const;
   // m * 8G with Z scaled to one.
   Cached1 multiple_of_eight_cached1(uint64_t m) const;
   // scalar * B, 256-bit scalar, double-and-add from the top bit

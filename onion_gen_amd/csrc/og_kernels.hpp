@@ -92,6 +92,7 @@ struct SetupArgs {
   uint32_t first;         // first chain to set up
   uint32_t count;         // chains to set up
   uint32_t half;
+  uint64_t k0;            // candidate index of chain 0's first window
   uint32_t start[32];     // P0 as X, Y, Z, T (8 x 32)
   uint32_t eight[32];     // 8G cached: y+x, y-x, Z, 2dT
   uint32_t two_d[8];

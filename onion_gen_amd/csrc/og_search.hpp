@@ -165,15 +165,15 @@ struct HitSink {
   HitRec *hits;
   uint32_t cap;
   uint32_t slot;
-  uint32_t *slot_found;  // per-slot counts of every hit this launch, or null
-  uint32_t budget;       // with slot_found: records the slot may still write
+  uint32_t *found;   // the search's count of every hit this launch, or null
+  uint32_t budget;   // with `found`: records the search may still write
 };
 
-// A hit is always counted; with a per-slot budget only the first `budget` of
-// the slot's hits this launch (in the order the waves count them) are written
-// to the ring, so a search that wants n hits never floods it.
+// A hit is always counted; with a budget only the first `budget` of the
+// search's hits this launch (in the order the waves count them) are written to
+// the ring, so a search that wants n hits never floods it.
 __device__ __forceinline__ void report(bool hit, uint64_t k, const Fe &c, const HitSink &sink) {
-  if (!sink.slot_found) {
+  if (!sink.found) {
     append(hit, sink.slot, k, c, sink.counter, sink.hits, sink.cap);
     return;
   }
@@ -182,7 +182,7 @@ __device__ __forceinline__ void report(bool hit, uint64_t k, const Fe &c, const 
   const uint32_t lane = __lane_id();
   const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
   uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(sink.slot_found + sink.slot, (uint32_t)__popcll(mask));
+  if (lane == leader) base = atomicAdd(sink.found, (uint32_t)__popcll(mask));
   base = __shfl(base, (int)leader);
   const uint32_t rank = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
   append(hit && rank < sink.budget, sink.slot, k, c, sink.counter, sink.hits, sink.cap);
@@ -393,8 +393,8 @@ __device__ __forceinline__ void st_point(uint32_t *state, uint32_t T, uint32_t g
   st10(state, 3, T, g, P.T);
 }
 
-// Chain start points: P_c = P0 + (c(2H+1) + H) * 8G, computed in the 8 x 32
-// representation and stored in the search's 10-limb one.
+// Chain start points: P_c = P0 + (k0 + c(2H+1) + H) * 8G, computed in the
+// 8 x 32 representation and stored in the search's 10-limb one.
 __device__ __forceinline__ void setup_chain(const SetupArgs &a, uint32_t local) {
   Cached eight, start;
   Point p0;
@@ -412,7 +412,7 @@ __device__ __forceinline__ void setup_chain(const SetupArgs &a, uint32_t local) 
   for (int l = 0; l < 8; ++l) two_d.v[l] = a.two_d[l];
   start = pt_to_cached(p0, two_d);
   const uint64_t S = 2ull * a.half + 1;
-  const uint64_t m = (uint64_t)local * S + a.half;
+  const uint64_t m = a.k0 + (uint64_t)local * S + a.half;
   Point p = pt_add(pt_mul_u64(m, eight), start);
   Point10 q;
   q.X = fe10_from(p.X);

@@ -158,8 +158,7 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
     steps++;
     busy_ms += og_batch_last_ms(b);
     const double t_step = now();
-    const uint64_t scheduled = std::min<uint64_t>(resident, o.slots ? o.slots : 64);
-    examined += scheduled * og_batch_per_step(b);
+    examined = og_batch_examined(b);
     for (uint32_t i = 0; i < nh; ++i) {
       const og_batch_hit &h = hits[i];
       if (h.search >= live.size() || !live[h.search].fs) continue;

@@ -140,3 +140,41 @@ def test_hit_quota_records_only_what_the_search_wants():
     assert b.found(a) == len(want1) and b.search_state(a)["hits"] == 5
     assert sorted(got[2]) == want2 and b.found(c) == len(want2)
     b.close()
+
+
+def test_searches_widen_into_idle_slots_and_narrow_under_load():
+    """Widths follow the resident count: a lone search takes every unit slot,
+    new arrivals narrow it, retirements widen the rest again.  Each relayout
+    starts from the search's next candidate, so every search's hits over what
+    it examined are exactly the oracle's."""
+    say("case: 8 unit slots x 256 chains, half 32; 2 -> 8 -> 2 resident searches")
+    b = batch.Batch(slots=8, chains=256, half=32, rounds=1)
+    f = filters.FilterSet.parse(["2x"])
+    seeds = [hashlib.sha256(b"wide%d" % i).digest() for i in range(8)]
+    got = {i: [] for i in range(8)}
+
+    def steps(n):
+        for _ in range(n):
+            for h in b.step():
+                got[h.tag].append((h.offset // 8, h.packed))
+
+    ids = [b.admit(key.start_point(seeds[0]), f, 0)]
+    assert b.width(ids[0]) == 8
+    steps(1)
+    ids.append(b.admit(key.start_point(seeds[1]), f, 1))
+    steps(2)
+    assert [b.width(i) for i in ids] == [4, 4]
+    ids += [b.admit(key.start_point(seeds[i]), f, i) for i in range(2, 8)]
+    steps(3)
+    assert [b.width(i) for i in ids] == [1] * 8
+    for i in ids[2:]:
+        b.retire(i)
+    steps(2)
+    assert [b.width(i) for i in ids[:2]] == [4, 4]
+    total = 0
+    for i, sid in enumerate(ids):
+        n = b.search_state(sid)["examined"]
+        total += n
+        assert sorted(got[i]) == oracle(seeds[i], ["2x"], n), i
+    assert total == b.examined()
+    b.close()
