@@ -38,6 +38,22 @@ def field_muls_per_candidate(half, with_x=False):
     return window / (2 * half + 1) + (2 if with_x else 0)
 
 
+def measured_traffic(threads, half, rounds):
+    """HBM bytes per launch of the search kernel at this shape, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy
+    calibration it records), or None when that summary is for another shape."""
+    path = os.path.join(ROOT, "profiles", "search_prefix6_traffic.md")
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None
+    want = f"traffic: threads={threads} half={half} rounds={rounds} bytes_per_launch="
+    for line in lines:
+        if line.startswith(want):
+            return int(line[len(want):])
+    return None
+
+
 def _pct(xs, q):
     if not xs:
         return None
@@ -168,6 +184,9 @@ def main():
     ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--filter", default="",
+                    help="search this filter instead of the 6-symbol prefix (for profiling the "
+                         "other forms; the bench line then names it)")
     ap.add_argument("--mix-seconds", type=float, default=5.0,
                     help="wall-time window of each batched-mix run (0 skips the leg)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
@@ -194,10 +213,11 @@ def main():
 
     c2 = workload.config2()
     prefix, seed = c2["prefix"], c2["seed"]
+    text = args.filter or prefix
     start = key.start_point(seed)
-    fs = filters.FilterSet.parse(prefix)
+    fs = filters.FilterSet.parse(text)
     eng = gpu.Engine.with_filters(start, fs, threads=args.threads, half=args.half,
-                                  rounds=args.rounds)
+                                  rounds=args.rounds, hit_cap=0 if not args.filter else 1 << 22)
     batch = eng.batch()
     steps = args.steps or -(-c2["candidates"] // batch)
     survivors = []
@@ -213,13 +233,13 @@ def main():
     rate = examined / wall
 
     verified = 0
-    for s in survivors:
+    for s in survivors[:4096]:
         got = key.finish_hit(seed, s.offset)
-        if got is not None and got[1].startswith(prefix) and \
+        if got is not None and fs.match_address(key.address_bytes(got[0])) >= 0 and \
                 got[0][:31] == s.packed[:31] and (got[0][31] & 0x7F) == s.packed[31]:
             verified += 1
 
-    F = field_muls_per_candidate(args.half)
+    F = field_muls_per_candidate(args.half, with_x=text.startswith("suffix:"))
     kernel_rate = examined / (kernel_ms * 1e-3)
     achieved = kernel_rate * F * 64  # 32x32->64 partial products per second
     out = {
@@ -235,19 +255,23 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": f"one 6-symbol prefix '{prefix}', {examined} sequential "
-                               f"candidates (2^34 at the default step count)",
+        "config": {"workload": (f"one 6-symbol prefix '{prefix}', {examined} sequential "
+                                f"candidates (2^34 at the default step count)") if not args.filter
+                   else f"one filter '{text}', {examined} sequential candidates",
                    "threads": args.threads, "half": args.half, "rounds": args.rounds,
                    "candidates_per_step": batch},
         "hits": len(survivors),
-        "hits_rederived": verified,
+        "hits_rederived": verified,  # of the first 4096
         "roofline": {
             "bound": "valu",
             "achieved": achieved / 1e12,
             "peak": VALU_PEAK_LANE_OPS / 1e12,
             "unit": "T mad32/s",
             "frac": achieved / VALU_PEAK_LANE_OPS,
-            "traffic": None,
+            "traffic": None if args.filter else measured_traffic(args.threads, args.half,
+                                                                  args.rounds),
+            "traffic_source": "profiles/search_prefix6_traffic.md (rocprofv3 FETCH_SIZE x2.0 "
+                              "+ WRITE_SIZE x1.0, copy-calibrated)",
             "kernel_avg_ms": kernel_ms / steps,
             "field_muls_per_candidate": F,
         },
