@@ -42,7 +42,7 @@ def measured_traffic(threads, half, rounds):
     """HBM bytes per launch of the search kernel at this shape, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy
     calibration it records), or None when that summary is for another shape."""
-    path = os.path.join(ROOT, "profiles", "search_prefix6_traffic.md")
+    path = os.path.join(ROOT, "profiles", "search_prefix6_traffic.md")  # lists each shape measured
     try:
         lines = open(path).read().splitlines()
     except OSError:
@@ -179,9 +179,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="0: cover 2^34 candidates")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--threads", type=int, default=1 << 18)
-    ap.add_argument("--half", type=int, default=256)
-    ap.add_argument("--rounds", type=int, default=2)
+    # The shape the autotune walk (og_hip_engine_tune) picks for this workload on
+    # MI355X: 2^19 chains, 1024 pairs per window, one window per launch.
+    ap.add_argument("--threads", type=int, default=1 << 19)
+    ap.add_argument("--half", type=int, default=1024)
+    ap.add_argument("--rounds", type=int, default=1)
+    ap.add_argument("--tuned", action="store_true",
+                    help="run the autotune walk first and search with the shape it picks")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--filter", default="",
@@ -216,6 +220,10 @@ def main():
     text = args.filter or prefix
     start = key.start_point(seed)
     fs = filters.FilterSet.parse(text)
+    tuned = None
+    if args.tuned:
+        tuned = gpu.Engine.tune(start, fs)
+        args.threads, args.half, args.rounds = tuned["threads"], tuned["half"], tuned["rounds"]
     eng = gpu.Engine.with_filters(start, fs, threads=args.threads, half=args.half,
                                   rounds=args.rounds, hit_cap=0 if not args.filter else 1 << 22)
     batch = eng.batch()
@@ -259,7 +267,7 @@ def main():
                                 f"candidates (2^34 at the default step count)") if not args.filter
                    else f"one filter '{text}', {examined} sequential candidates",
                    "threads": args.threads, "half": args.half, "rounds": args.rounds,
-                   "candidates_per_step": batch},
+                   "candidates_per_step": batch, "tuning_walk": tuned},
         "hits": len(survivors),
         "hits_rederived": verified,  # of the first 4096
         "roofline": {

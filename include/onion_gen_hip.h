@@ -42,6 +42,17 @@ typedef struct {
   uint8_t packed[32]; /* canonical y, little-endian, sign bit clear */
 } og_survivor;
 
+/* Replaces tuning::Shape (src/tuning.rs:27-39), extended with `rounds` and
+ * `radix`; first = pairs per window (half), second = chains (threads). */
+typedef struct {
+  uint32_t first;
+  uint32_t second;
+  double rate;
+  int64_t at;
+  uint32_t rounds;
+  uint32_t radix;
+} og_tuning_shape;
+
 typedef struct og_engine og_engine;
 typedef struct og_filterset og_filterset;
 
@@ -49,16 +60,18 @@ typedef struct og_filterset og_filterset;
 uint32_t og_hit_room(uint64_t batch, uint32_t bitmap_bits);
 
 /* Replaces gpu::Engine::new (src/gpu/mod.rs:346-353) with the arguments the
- * reference binds: the start point, the chain count (0 = take the shape the
- * tuning cache recorded, else the default), and the prefix bitmap of
+ * reference binds: the start point, the chain count (0 = the shape the tuning
+ * cache recorded, measured first when there is none), and the prefix bitmap of
  * FilterSet::index_words (src/filter.rs:1857-1859) as 64-bit words. */
 int og_hip_engine_create(int ordinal, const uint8_t start_xyzt[128], uint32_t threads,
                          const uint64_t *bitmap, size_t bitmap_words, uint32_t bitmap_bits,
                          og_engine **out, char *err, size_t errcap);
 
-/* Shape of an engine.  Zero fields take defaults. */
+/* Shape of an engine.  threads = 0 takes the whole shape (threads, half,
+ * rounds) from the tuning cache, or measures it first (og_hip_engine_tune);
+ * otherwise zero half/rounds take their defaults. */
 typedef struct {
-  uint32_t threads; /* chains (multiple of 256); 0 = default */
+  uint32_t threads; /* chains (multiple of 256); 0 = the tuned shape */
   uint32_t half;    /* pairs per window: one window is 2*half+1 candidates */
   uint32_t rounds;  /* windows per chain per advance */
   uint32_t hit_cap; /* survivor room per advance; 0 = og_hit_room */
@@ -71,6 +84,17 @@ typedef struct {
 int og_hip_engine_create_ex(int ordinal, const uint8_t start_xyzt[128], const og_engine_opts *opts,
                             const og_filterset *filters, og_engine **out, char *err,
                             size_t errcap);
+
+/* Replaces the automatic choice of Engine::new with retune = true
+ * (src/gpu/mod.rs:354-412): times one launch of every shape of a walk over
+ * chains {2^19 .. 2^16} x half {1024 .. 128} (rounds for about 2^28
+ * candidates a launch) with the given filters, records the fastest in the
+ * tuning cache under device/<name>/hip/0/<version>, and writes it. */
+int og_hip_engine_tune(int ordinal, const uint8_t start_xyzt[128], const og_filterset *filters,
+                       og_tuning_shape *best, char *err, size_t errcap);
+
+/* The shape an engine runs: chains, pairs per window, windows per launch. */
+void og_hip_engine_shape(const og_engine *e, uint32_t *threads, uint32_t *half, uint32_t *rounds);
 
 /* Replaces Engine::threads / Engine::batch (src/gpu/mod.rs:446-467). */
 uint32_t og_hip_engine_threads(const og_engine *e);
@@ -270,16 +294,7 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
 
 /* ----------------------------------------------------------------- tuning */
 
-/* Replaces tuning::Shape (src/tuning.rs:27-39), extended with `rounds` and
- * `radix`; first = pairs per window (half), second = chains (threads). */
-typedef struct {
-  uint32_t first;
-  uint32_t second;
-  double rate;
-  int64_t at;
-  uint32_t rounds;
-  uint32_t radix;
-} og_tuning_shape;
+
 
 /* Replaces tuning::path (src/tuning.rs:80-91); OG_TUNING_PATH overrides. */
 int og_tuning_path(char *out, size_t cap);

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <ctime>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -366,13 +368,129 @@ uint32_t og_hit_room(uint64_t batch, uint32_t bitmap_bits) {
   return (uint32_t)room;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The cache key of this device's search shape: device/<name>/hip/0/<version>
+// (src/tuning.rs:65-74, the threads field 0 as the reference's automatic
+// choice records it).
+std::string tuning_key(int ordinal) {
+  char name[256] = {0};
+  uint64_t mem = 0;
+  int cus = 0;
+  og_hip_device_info(ordinal, name, sizeof name, &mem, &cus);
+  std::string safe(name);
+  for (char &c : safe)
+    if (c == '/' || c == '\n') c = '_';
+  return "device/" + safe + "/hip/0/" + kTuningVersion;
+}
+
+// The shapes the walk times, largest first (src/gpu/mod.rs:315-340 walks its
+// batch sizes down the same way): chains in flight times pairs per window,
+// with enough rounds for about 2^28 candidates per launch.
+std::vector<og_engine_opts> tuning_walk() {
+  std::vector<og_engine_opts> out;
+  for (uint32_t chains : {1u << 19, 1u << 18, 1u << 17, 1u << 16})
+    for (uint32_t half : {1024u, 512u, 256u, 128u}) {
+      og_engine_opts o{};
+      o.threads = chains;
+      o.half = half;
+      o.rounds = std::max<uint32_t>(1, (uint32_t)((1ull << 28) / ((uint64_t)chains * (2 * half + 1))));
+      o.hit_cap = 1u << 20;
+      out.push_back(o);
+    }
+  return out;
+}
+
+// Times one launch of every shape after a warm-up launch (which pays for the
+// tables reaching the device), as src/gpu/mod.rs:415-428 does, keeps the
+// fastest in candidates per second of wall time and records it.
+int tune(int ordinal, const uint8_t start_xyzt[128], const FilterSet *fs, const uint64_t *bitmap,
+         size_t bitmap_words, uint32_t bitmap_bits, og_tuning_shape *best, char *err,
+         size_t errcap) {
+  og_tuning_shape b{};
+  std::string last = "no shape was tried";
+  for (const og_engine_opts &o : tuning_walk()) {
+    og_engine *e = nullptr;
+    char e2[512] = {0};
+    if (create_engine(ordinal, start_xyzt, &o, fs, bitmap, bitmap_words, bitmap_bits, &e, e2,
+                      sizeof e2) != 0) {
+      last = e2;
+      continue;
+    }
+    std::vector<og_survivor> room(e->cap);
+    uint32_t n = 0;
+    double rate = 0;
+    if (og_hip_engine_advance(e, room.data(), e->cap, &n, e2, sizeof e2) == 0) {
+      const auto t0 = std::chrono::steady_clock::now();
+      if (og_hip_engine_advance(e, room.data(), e->cap, &n, e2, sizeof e2) == 0) {
+        const double sec =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (sec > 0) rate = (double)og_hip_engine_batch(e) / sec;
+      }
+    }
+    if (rate == 0 && e2[0]) last = e2;
+    og_hip_engine_destroy(e);
+    if (rate > b.rate) {
+      b.first = o.half;
+      b.second = o.threads;
+      b.rounds = o.rounds;
+      b.rate = rate;
+    }
+  }
+  if (b.rate == 0) {
+    put_err(err, errcap, "no search shape this device would take: " + last);
+    return -1;
+  }
+  b.at = (int64_t)std::time(nullptr);
+  b.radix = 10;
+  og_tuning_put(tuning_key(ordinal).c_str(), &b);
+  if (best) *best = b;
+  return 0;
+}
+
+// threads == 0: the whole shape from the tuning cache, else measured now
+// (src/gpu/mod.rs:354-412).  A cached shape that no longer builds is measured
+// again.
+int create_tuned(int ordinal, const uint8_t start_xyzt[128], const og_engine_opts *opts,
+                 const FilterSet *fs, const uint64_t *bitmap, size_t bitmap_words,
+                 uint32_t bitmap_bits, og_engine **out, char *err, size_t errcap) {
+  if (opts && opts->threads)
+    return create_engine(ordinal, start_xyzt, opts, fs, bitmap, bitmap_words, bitmap_bits, out,
+                         err, errcap);
+  if (check_device(ordinal, err, errcap)) return -1;
+  og_engine_opts o{};
+  if (opts) o.hit_cap = opts->hit_cap;
+  og_tuning_shape s{};
+  if (og_tuning_get(tuning_key(ordinal).c_str(), &s) == 1 && s.first && s.second) {
+    o.threads = s.second;
+    o.half = s.first;
+    o.rounds = s.rounds ? s.rounds : 1;
+    if (create_engine(ordinal, start_xyzt, &o, fs, bitmap, bitmap_words, bitmap_bits, out, err,
+                      errcap) == 0)
+      return 0;
+  }
+  if (tune(ordinal, start_xyzt, fs, bitmap, bitmap_words, bitmap_bits, &s, err, errcap) != 0)
+    return -1;
+  o.threads = s.second;
+  o.half = s.first;
+  o.rounds = s.rounds;
+  return create_engine(ordinal, start_xyzt, &o, fs, bitmap, bitmap_words, bitmap_bits, out, err,
+                       errcap);
+}
+
+}  // namespace
+
+extern "C" {
+
 int og_hip_engine_create(int ordinal, const uint8_t start_xyzt[128], uint32_t threads,
                          const uint64_t *bitmap, size_t bitmap_words, uint32_t bitmap_bits,
                          og_engine **out, char *err, size_t errcap) {
   og_engine_opts o{};
   o.threads = threads;
-  return create_engine(ordinal, start_xyzt, &o, nullptr, bitmap, bitmap_words, bitmap_bits, out,
-                       err, errcap);
+  return create_tuned(ordinal, start_xyzt, &o, nullptr, bitmap, bitmap_words, bitmap_bits, out,
+                      err, errcap);
 }
 
 int og_hip_engine_create_ex(int ordinal, const uint8_t start_xyzt[128], const og_engine_opts *opts,
@@ -382,8 +500,25 @@ int og_hip_engine_create_ex(int ordinal, const uint8_t start_xyzt[128], const og
     put_err(err, errcap, "null filter set");
     return -1;
   }
-  return create_engine(ordinal, start_xyzt, opts, filterset_of(filters), nullptr, 0, 0, out, err,
-                       errcap);
+  return create_tuned(ordinal, start_xyzt, opts, filterset_of(filters), nullptr, 0, 0, out, err,
+                      errcap);
+}
+
+int og_hip_engine_tune(int ordinal, const uint8_t start_xyzt[128], const og_filterset *filters,
+                       og_tuning_shape *best, char *err, size_t errcap) {
+  if (!start_xyzt || !filters) {
+    put_err(err, errcap, "null argument");
+    return -1;
+  }
+  if (check_device(ordinal, err, errcap)) return -1;
+  return tune(ordinal, start_xyzt, filterset_of(filters), nullptr, 0, 0, best, err, errcap);
+}
+
+void og_hip_engine_shape(const og_engine *e, uint32_t *threads, uint32_t *half,
+                         uint32_t *rounds) {
+  if (threads) *threads = e ? e->threads : 0;
+  if (half) *half = e ? e->half : 0;
+  if (rounds) *rounds = e ? e->rounds : 0;
 }
 
 uint32_t og_hip_engine_threads(const og_engine *e) { return e ? e->threads : 0; }

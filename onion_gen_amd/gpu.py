@@ -93,6 +93,25 @@ class Engine:
             raise OgError(e.value.decode())
         return cls(h, max(1 << 18, hit_cap))
 
+    @staticmethod
+    def tune(start_xyzt, filters, device=0):
+        """Measures the walk of shapes (og_hip_engine_tune), records the
+        fastest in the tuning cache and returns it as a dict."""
+        from ._lib import TuningShape
+        best = TuningShape()
+        e = err_buf()
+        if lib().og_hip_engine_tune(device, bytes(start_xyzt), filters.handle, ctypes.byref(best),
+                                    e, len(e)) != 0:
+            raise OgError(e.value.decode())
+        return {"threads": best.second, "half": best.first, "rounds": best.rounds,
+                "rate": best.rate, "at": best.at}
+
+    def shape(self):
+        """(threads, half, rounds) this engine runs."""
+        t, h, r = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        lib().og_hip_engine_shape(self._h, ctypes.byref(t), ctypes.byref(h), ctypes.byref(r))
+        return t.value, h.value, r.value
+
     def threads(self):
         return lib().og_hip_engine_threads(self._h)
 

@@ -228,3 +228,22 @@ def test_device_matcher_agrees_with_oracle_on_every_placement():
             hits += hit
             needed += need
     assert hits >= 400 and needed > 0 and checked == 400 * 7
+
+
+def test_tuning_cache_round_trip(tmp_path, monkeypatch):
+    """The reference's file format: written atomically, read back, foreign
+    entries kept, an unreadable file is no entry (src/tuning.rs:1-126)."""
+    from onion_gen_amd import tuning
+    p = tmp_path / "sub" / "tuning.yaml"
+    monkeypatch.setenv("OG_TUNING_PATH", str(p))
+    assert tuning.path() == str(p)
+    assert tuning.get("device/x/hip/0/0.1.0") is None
+    assert tuning.put("device/x/hip/0/0.1.0", 512, 262144, 2.9e10, 1700000000, 1, 10)
+    assert tuning.put("device/y/opencl/0/0.1.0", 256, 65536, 1.5e9, 1700000001)
+    got = tuning.get("device/x/hip/0/0.1.0")
+    assert got["first"] == 512 and got["second"] == 262144 and got["rounds"] == 1
+    assert abs(got["rate"] - 2.9e10) < 1e3 and got["at"] == 1700000000 and got["radix"] == 10
+    text = p.read_text()
+    assert text.startswith("format: 1") and "entries:" in text and "device/y/opencl/0/0.1.0" in text
+    p.write_text("format: 2\nentries: {}\n")
+    assert tuning.get("device/x/hip/0/0.1.0") is None
