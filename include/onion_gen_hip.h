@@ -117,7 +117,9 @@ void og_hip_engine_destroy(og_engine *e);
 /* Elementwise GF(2^255-19) arithmetic for parity tests of the field itself
  * (the device forms of src/field.rs:82-255): ops 0..5 in the 8 x 32 form and
  * 10..15 in the 10 x 25.5 form the search kernels run in; each block is mul,
- * sq, add, sub, invert, canonical; op 16 is (2x - y)(x - y) in the 10-limb
+ * sq, add, sub, invert, canonical; op 20 is the address checksum of a as a
+ * packed key (SHA3-256(".onion checksum" | a | 03)[0..2] in the low 16 bits of
+ * the output); op 16 is (2x - y)(x - y) in the 10-limb
  * form, the loosest operands the search multiplies.  Inputs are n raw 32-byte little-endian
  * words (any value below 2^256); outputs are canonical encodings.  The _host
  * variant runs the same code on the CPU. */

@@ -82,9 +82,120 @@ OG_HD void keccak_f1600(uint64_t a[25]) {
   }
 }
 
+#ifdef __HIP_DEVICE_COMPILE__
+// The checksum permutation for gfx950: each 64-bit lane as two 32-bit halves,
+// rotations as two v_alignbit_b32 (funnel shifts), chi and the five-way column
+// XOR as v_bitop3_b32 (any three-input bitwise function), and a last round cut
+// to the one lane the checksum reads.  About 190 VALU instructions a round
+// where the 64-bit form compiles to 288.
+struct KLane {
+  uint32_t lo, hi;
+};
+
+// rotl by a compile-time amount (1..63, not 32: no rho offset is 0 mod 32
+// besides lane 0's, which is not rotated).
+template <int N>
+__device__ __forceinline__ KLane krotl(KLane x) {
+  static_assert(N > 0 && N < 64 && N != 32, "rotation");
+  if (N < 32)
+    return {__builtin_amdgcn_alignbit(x.lo, x.hi, 32 - N), __builtin_amdgcn_alignbit(x.hi, x.lo, 32 - N)};
+  return {__builtin_amdgcn_alignbit(x.hi, x.lo, 64 - N), __builtin_amdgcn_alignbit(x.lo, x.hi, 64 - N)};
+}
+
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)0x96);  // a ^ b ^ c
+}
+
+__device__ __forceinline__ uint32_t chi3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)0xd2);  // a ^ (~b & c)
+}
+
+__device__ __forceinline__ KLane kx(KLane a, KLane b) { return {a.lo ^ b.lo, a.hi ^ b.hi}; }
+
+__device__ __forceinline__ void ktheta(const KLane a[25], KLane d[5]) {
+  KLane c[5];
+#pragma unroll
+  for (int x = 0; x < 5; ++x) {
+    c[x].lo = xor3(xor3(a[x].lo, a[x + 5].lo, a[x + 10].lo), a[x + 15].lo, a[x + 20].lo);
+    c[x].hi = xor3(xor3(a[x].hi, a[x + 5].hi, a[x + 10].hi), a[x + 15].hi, a[x + 20].hi);
+  }
+#pragma unroll
+  for (int x = 0; x < 5; ++x) d[x] = kx(c[(x + 4) % 5], krotl<1>(c[(x + 1) % 5]));
+}
+
+__device__ __forceinline__ uint32_t onion_checksum_gfx950(const uint32_t pk[8]) {
+  KLane a[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) a[i] = {0u, 0u};
+  // message bytes: ".onion checksum" (15) | pk (32) | 0x03, then SHA3 padding
+  // 0x06 at byte 48 and 0x80 at byte 135; lane i holds bytes 8i..8i+7.
+  a[0] = {0x696e6f2eu, 0x63206e6fu};                     // ".onion c"
+  a[1] = {0x6b636568u, 0x006d7573u | (pk[0] << 24)};     // "hecksum" + pk[0]
+#pragma unroll
+  for (int i = 2; i < 5; ++i)
+    a[i] = {__builtin_amdgcn_alignbit(pk[2 * i - 3], pk[2 * i - 4], 8),
+            __builtin_amdgcn_alignbit(pk[2 * i - 2], pk[2 * i - 3], 8)};
+  a[5] = {__builtin_amdgcn_alignbit(pk[7], pk[6], 8), (pk[7] >> 8) | (0x03u << 24)};
+  a[6] = {0x06u, 0u};
+  a[16] = {0u, 0x80000000u};
+#pragma unroll 1
+  for (int round = 0; round < 23; ++round) {
+    KLane d[5], b[25];
+    ktheta(a, d);
+    b[0] = kx(a[0], d[0]);
+    b[10] = krotl<1>(kx(a[1], d[1]));
+    b[20] = krotl<62>(kx(a[2], d[2]));
+    b[5] = krotl<28>(kx(a[3], d[3]));
+    b[15] = krotl<27>(kx(a[4], d[4]));
+    b[16] = krotl<36>(kx(a[5], d[0]));
+    b[1] = krotl<44>(kx(a[6], d[1]));
+    b[11] = krotl<6>(kx(a[7], d[2]));
+    b[21] = krotl<55>(kx(a[8], d[3]));
+    b[6] = krotl<20>(kx(a[9], d[4]));
+    b[7] = krotl<3>(kx(a[10], d[0]));
+    b[17] = krotl<10>(kx(a[11], d[1]));
+    b[2] = krotl<43>(kx(a[12], d[2]));
+    b[12] = krotl<25>(kx(a[13], d[3]));
+    b[22] = krotl<39>(kx(a[14], d[4]));
+    b[23] = krotl<41>(kx(a[15], d[0]));
+    b[8] = krotl<45>(kx(a[16], d[1]));
+    b[18] = krotl<15>(kx(a[17], d[2]));
+    b[3] = krotl<21>(kx(a[18], d[3]));
+    b[13] = krotl<8>(kx(a[19], d[4]));
+    b[14] = krotl<18>(kx(a[20], d[0]));
+    b[24] = krotl<2>(kx(a[21], d[1]));
+    b[9] = krotl<61>(kx(a[22], d[2]));
+    b[19] = krotl<56>(kx(a[23], d[3]));
+    b[4] = krotl<14>(kx(a[24], d[4]));
+#pragma unroll
+    for (int y = 0; y < 25; y += 5) {
+#pragma unroll
+      for (int x = 0; x < 5; ++x) {
+        a[y + x].lo = chi3(b[y + x].lo, b[y + (x + 1) % 5].lo, b[y + (x + 2) % 5].lo);
+        a[y + x].hi = chi3(b[y + x].hi, b[y + (x + 1) % 5].hi, b[y + (x + 2) % 5].hi);
+      }
+    }
+    const uint64_t rc = keccak_rc(round);
+    a[0].lo ^= (uint32_t)rc;
+    a[0].hi ^= (uint32_t)(rc >> 32);
+  }
+  // round 23, cut to the low half of lane 0: chi of b[0], b[1], b[2] and iota
+  KLane d[5];
+  ktheta(a, d);
+  const uint32_t b0 = a[0].lo ^ d[0].lo;
+  const uint32_t b1 = krotl<44>(kx(a[6], d[1])).lo;
+  const uint32_t b2 = krotl<43>(kx(a[12], d[2])).lo;
+  const uint32_t lane0 = chi3(b0, b1, b2) ^ (uint32_t)keccak_rc(23);
+  return lane0 & 0xffffu;
+}
+#endif
+
 // The two checksum bytes of a packed key given as eight little-endian words.
 // Returns checksum[0] | checksum[1] << 8.
 OG_HD uint32_t onion_checksum(const uint32_t pk[8]) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return onion_checksum_gfx950(pk);
+#endif
   // message bytes: ".onion checksum" (15) | pk (32) | 0x03, then SHA3 padding
   // 0x06 at byte 48 and 0x80 at byte 135 (the last byte of the rate).
   uint64_t a[25];

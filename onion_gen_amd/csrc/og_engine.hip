@@ -65,6 +65,12 @@ __host__ __device__ inline void field_op_one(int op, const uint32_t *a, const ui
     y.v[l] = b[l];
   }
   Fe r;
+  if (op == 20) {
+    // the address checksum of a as a packed key (sign bit as given)
+    for (int l = 0; l < 8; ++l) out[l] = 0;
+    out[0] = onion_checksum(a);
+    return;
+  }
   if (op < 10) {
     switch (op) {
       case 0: r = fe_mul(x, y); break;
@@ -614,8 +620,11 @@ int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
   return 0;
 }
 
+// ops 0..5, 10..16 and 20, as include/onion_gen_hip.h lists them
+static bool field_op_known(int op) { return (op >= 0 && op <= 5) || (op >= 10 && op <= 16) || op == 20; }
+
 int og_field_op_host(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, size_t n) {
-  if (op < 0 || (op > 5 && op < 10) || op > 16 || !a || !b || !out) return -1;
+  if (!field_op_known(op) || !a || !b || !out) return -1;
   for (size_t i = 0; i < n; ++i) {
     uint32_t x[8], y[8], r[8];
     std::memcpy(x, a + 32 * i, 32);
@@ -628,7 +637,7 @@ int og_field_op_host(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, s
 
 int og_hip_field_op(int ordinal, int op, const uint8_t *a, const uint8_t *b, uint8_t *out,
                     size_t n, char *err, size_t errcap) {
-  if (op < 0 || (op > 5 && op < 10) || op > 16 || !a || !b || !out) {
+  if (!field_op_known(op) || !a || !b || !out) {
     put_err(err, errcap, "bad field operation or null buffer");
     return -1;
   }

@@ -76,10 +76,14 @@ OG_HD void scan_key(const uint32_t w[8], const ScanBank *banks, uint32_t nbanks,
     const ScanBank &s = banks[k];
     const uint32_t js = s.jstart, iany = s.iany;
     uint32_t D = 0, acc = 0;
+    // A bank whose patterns all start past the key span skips the key pass.
+    // Otherwise every symbol runs, with the injection masked before jstart
+    // (no branch per symbol): D stays 0 until a pattern may start.
+    if (js < 49) {
 #pragma unroll
-    for (int j = 0; j < 49; ++j) {
-      if ((uint32_t)j >= js) {
-        D = ((D << 1) | (j == 0 ? (s.istart | iany) : iany)) & s.b[sym_at(w, j)];
+      for (int j = 0; j < 49; ++j) {
+        const uint32_t inj = (j == 0 ? s.istart : 0u) | ((uint32_t)j >= js ? iany : 0u);
+        D = ((D << 1) | inj) & s.b[sym_at(w, j)];
         acc |= D;
       }
     }
@@ -114,10 +118,19 @@ OG_HD uint32_t scan_address(const uint32_t w[9], const ScanBank *banks, uint32_t
     const ScanBank &s = banks[k];
     const uint32_t js = s.jstart, iany = s.iany;
     uint32_t D = 0, acc = 0;
+    if (js >= 48) {
+      // every pattern of the bank starts in the last 8 symbols (a suffix of
+      // at most 8): nothing before symbol 48 can change D
 #pragma unroll
-    for (int j = 0; j < 56; ++j) {
-      if ((uint32_t)j >= js) {
-        D = ((D << 1) | (j == 0 ? (s.istart | iany) : iany)) & s.b[sym_at(w, j)];
+      for (int j = 48; j < 56; ++j) {
+        D = ((D << 1) | ((uint32_t)j >= js ? iany : 0u)) & s.b[sym_at(w, j)];
+        acc |= D;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 56; ++j) {
+        const uint32_t inj = (j == 0 ? s.istart : 0u) | ((uint32_t)j >= js ? iany : 0u);
+        D = ((D << 1) | inj) & s.b[sym_at(w, j)];
         acc |= D;
       }
     }

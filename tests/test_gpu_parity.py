@@ -90,6 +90,20 @@ def test_field_kernels_match_big_integers():
     assert field_op(16, a, b) == [(x - y) * (2 * x - y) % P for x, y in zip(a, b)]
 
 
+def test_device_checksum_matches_sha3():
+    """The gfx950 Keccak (32-bit halves, alignbit rotations, bitop3 chi, cut
+    last round) against hashlib's SHA3-256 on random keys, sign bits set and
+    clear, and the all-zero and all-one keys."""
+    say("case: device address checksum, 20000 random keys")
+    rng = random.Random(11)
+    keys = [0, 2 ** 256 - 1, 2 ** 255, 2 ** 255 - 1] + [rng.getrandbits(256) for _ in range(20000)]
+    got = field_op(20, keys, keys)
+    for k, g in zip(keys, got):
+        pk = k.to_bytes(32, "little")
+        d = hashlib.sha3_256(b".onion checksum" + pk + b"\x03").digest()
+        assert g == d[0] | (d[1] << 8), pk.hex()
+
+
 def test_config1_golden_hits():
     """BASELINE configuration 1 on the device: 2^24 candidates, the 4-symbol
     prefix, the committed hit list."""

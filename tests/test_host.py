@@ -247,3 +247,13 @@ def test_tuning_cache_round_trip(tmp_path, monkeypatch):
     assert text.startswith("format: 1") and "entries:" in text and "device/y/opencl/0/0.1.0" in text
     p.write_text("format: 2\nentries: {}\n")
     assert tuning.get("device/x/hip/0/0.1.0") is None
+
+
+def test_host_checksum_matches_sha3():
+    """The host Keccak of the address checksum (the device one is checked on
+    the GPU) against hashlib's SHA3-256."""
+    rng = random.Random(12)
+    keys = [0, 2 ** 256 - 1] + [rng.getrandbits(256) for _ in range(500)]
+    for k, g in zip(keys, host_field_op(20, keys, keys)):
+        d = hashlib.sha3_256(b".onion checksum" + k.to_bytes(32, "little") + b"\x03").digest()
+        assert g == d[0] | (d[1] << 8)
