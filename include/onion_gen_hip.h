@@ -137,6 +137,40 @@ int og_field_op_host(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, s
 int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
                        double *muls_per_s, uint8_t *results, char *err, size_t errcap);
 
+/* --------------------------------------------------------------- hit sink */
+
+enum {
+  OG_HIT_OK = 0,        /* verified: the secret produces this key, which matched */
+  OG_HIT_CLAMPING = 1,  /* the offset broke clamping: dropped (src/key.rs:196-206) */
+  OG_HIT_MISMATCH = 2   /* the re-derived key disagrees with the survivor */
+};
+
+typedef struct {
+  uint64_t offset;             /* 8k, as the survivor's */
+  int32_t status;              /* OG_HIT_* */
+  uint8_t public_key[32];      /* with the sign bit, when status is OG_HIT_OK */
+  uint8_t expanded_secret[64]; /* tor's layout: scalar at the offset | right half */
+  char hostname[63];           /* "<56 symbols>.onion", NUL-terminated */
+} og_key_record;
+
+/* Replaces the per-hit host work of run::device_worker / record_hit
+ * (src/run.rs:1030-1116, 740-896) with key::expanded_secret_at_offset
+ * (src/key.rs:167-206) and the independent re-derivation of verify::one
+ * (src/verify.rs:89-189), for a whole batch of survivors of the search seeded
+ * by `seed`: the secret at each offset (clamping checked on the host), its
+ * public key recomputed on the device from the scalar alone, 255 bits compared
+ * with the survivor, the hostname built.  Writes n records. */
+int og_hip_finish_hits(int ordinal, const uint8_t seed[32], const og_survivor *hits, size_t n,
+                       og_key_record *out, char *err, size_t errcap);
+
+/* Replaces output::write_key (src/output.rs:42-71): under `dir`, a directory
+ * named after the hostname (mode 0700) holding hs_ed25519_secret_key (96 bytes,
+ * mode 0600), hs_ed25519_public_key (64 bytes) and hostname.  An existing
+ * directory is never touched.  Returns 0 when created, 1 when it existed (the
+ * path is written either way), -1 on an error (message in path_out). */
+int og_write_key(const char *dir, const uint8_t public_key[32], const uint8_t expanded_secret[64],
+                 char *path_out, size_t cap);
+
 /* ---------------------------------------------------------------- filters */
 
 /* Replaces FilterSet::parse_all + FilterSet::new / with_index_bits

@@ -52,6 +52,10 @@ SIGNATURES = {
     "og_key_address_bytes": (None, [_c.c_char_p, _c.c_char_p]),
     "og_key_address": (None, [_c.c_char_p, _c.c_char_p]),
     "og_run_block_seed": (None, [_c.c_char_p, _c.c_uint64, _c.c_char_p]),
+    "og_hip_finish_hits": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_void_p, _c.c_size_t,
+                                      _c.c_void_p, _c.c_char_p, _c.c_size_t]),
+    "og_write_key": (_c.c_int, [_c.c_char_p, _c.c_char_p, _c.c_char_p, _c.c_char_p,
+                                _c.c_size_t]),
 }
 
 

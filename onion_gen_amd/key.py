@@ -1,7 +1,26 @@
-"""Key derivation and hit reconstruction (src/key.rs, src/run.rs:211)."""
+"""Key derivation and hit reconstruction (src/key.rs, src/run.rs:211), the
+batched hit sink and the key-file writer (src/output.rs)."""
 import ctypes
+from dataclasses import dataclass
 
-from ._lib import lib
+from ._lib import OgError, err_buf, lib
+
+HIT_OK, HIT_CLAMPING, HIT_MISMATCH = 0, 1, 2
+
+
+class _KeyRecord(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_uint64), ("status", ctypes.c_int32),
+                ("public_key", ctypes.c_uint8 * 32), ("expanded_secret", ctypes.c_uint8 * 64),
+                ("hostname", ctypes.c_char * 63)]
+
+
+@dataclass(frozen=True)
+class KeyRecord:
+    offset: int
+    status: int
+    public_key: bytes
+    expanded_secret: bytes
+    hostname: str
 
 
 def secret_scalar(seed):
@@ -64,3 +83,35 @@ def finish_hit(seed, offset):
         return None
     pk = public_from_scalar(secret[:32])
     return pk, address(pk)
+
+
+def finish_hits(seed, survivors, device=0):
+    """The batched hit sink (og_hip_finish_hits): for each survivor (objects
+    with ``offset`` and ``packed``) its secret at the offset, its public key
+    re-derived on the device from the scalar alone and compared with the
+    survivor, and its hostname.  Returns KeyRecords in the same order."""
+    from .gpu import _Survivor
+    n = len(survivors)
+    if n == 0:
+        return []
+    src = (_Survivor * n)()
+    for i, s in enumerate(survivors):
+        src[i].offset = s.offset
+        src[i].packed[:] = list(s.packed)
+    out = (_KeyRecord * n)()
+    e = err_buf()
+    if lib().og_hip_finish_hits(device, bytes(seed), src, n, out, e, len(e)) != 0:
+        raise OgError(e.value.decode())
+    return [KeyRecord(r.offset, r.status, bytes(r.public_key), bytes(r.expanded_secret),
+                      r.hostname.decode()) for r in out]
+
+
+def write_key(directory, public_key, expanded_secret):
+    """Writes tor's key directory for one key (og_write_key).  Returns
+    (created, path); an existing directory is left as it is."""
+    path = ctypes.create_string_buffer(4096)
+    rc = lib().og_write_key(str(directory).encode(), bytes(public_key), bytes(expanded_secret),
+                            path, len(path))
+    if rc < 0:
+        raise OSError(path.value.decode())
+    return rc == 0, path.value.decode()

@@ -257,3 +257,29 @@ def test_host_checksum_matches_sha3():
     for k, g in zip(keys, host_field_op(20, keys, keys)):
         d = hashlib.sha3_256(b".onion checksum" + k.to_bytes(32, "little") + b"\x03").digest()
         assert g == d[0] | (d[1] << 8)
+
+
+def test_key_files_in_tor_layout(tmp_path):
+    """og_write_key (src/output.rs:42-88): three files with tor's headers, the
+    secret readable by its owner only, an existing directory left alone."""
+    import stat
+    seed = bytes(range(32))
+    secret = key.expanded_secret_at_offset(seed, 8 * 12345)
+    pk = key.public_from_scalar(secret[:32])
+    created, path = key.write_key(tmp_path / "keys", pk, secret)
+    host = key.address(pk) + ".onion"
+    assert created and path.endswith("/" + host)
+    raw = open(os.path.join(path, "hs_ed25519_secret_key"), "rb").read()
+    assert raw == b"== ed25519v1-secret: type0 ==\x00\x00\x00" + secret
+    assert open(os.path.join(path, "hs_ed25519_public_key"), "rb").read() == \
+        b"== ed25519v1-public: type0 ==\x00\x00\x00" + pk
+    assert open(os.path.join(path, "hostname")).read() == host + "\n"
+    assert stat.S_IMODE(os.stat(path).st_mode) == 0o700
+    assert stat.S_IMODE(os.stat(os.path.join(path, "hs_ed25519_secret_key")).st_mode) == 0o600
+    # the secret produces the public key (the check of src/verify.rs:120-125),
+    # by textbook big-integer arithmetic
+    from tests import ed25519_ref
+    assert ed25519_ref.encode(ed25519_ref.mul(int.from_bytes(secret[:32], "little"))) == pk
+    created2, path2 = key.write_key(tmp_path / "keys", pk, bytes(64))
+    assert not created2 and path2 == path
+    assert open(os.path.join(path, "hs_ed25519_secret_key"), "rb").read() == raw
