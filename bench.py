@@ -155,6 +155,58 @@ def served_leg(seconds, rate=200.0):
     }
 
 
+def dictionary_leg(threads, half, rounds):
+    """The dictionary workload: 10,000 distinct prefixes of 5-8 symbols
+    (ChaCha20 seeded 2) behind a 24-bit prefix bitmap and the exact table, 2^34
+    candidates, then every survivor through the batched hit sink (secret at the
+    offset, public key re-derived on the device, hostname).  Survivors stay in
+    one ctypes buffer from the engine to the sink."""
+    import ctypes
+    import numpy as np
+    from onion_gen_amd import filters, gpu, key, workload
+    from onion_gen_amd._lib import OgError, err_buf, lib
+    from onion_gen_amd.gpu import _Survivor
+    from onion_gen_amd.key import _KeyRecord
+    seed = workload.chacha_stream(0, 32)
+    fs = filters.FilterSet.parse(workload.dictionary(10000, 2), index_bits=24)
+    eng = gpu.Engine.with_filters(key.start_point(seed), fs, threads=threads, half=half,
+                                  rounds=rounds)
+    batch = eng.batch()
+    steps = -(-(1 << 34) // batch)
+    room = eng.room()
+    buf = (_Survivor * (steps * room))()
+    base = ctypes.addressof(buf)
+    n = 0
+    e = err_buf()
+    cnt = ctypes.c_uint32()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if lib().og_hip_engine_advance(eng._h, ctypes.c_void_p(base + n * ctypes.sizeof(_Survivor)),
+                                       room, ctypes.byref(cnt), e, len(e)) != 0:
+            raise OgError(e.value.decode())
+        n += cnt.value
+    search_s = time.perf_counter() - t0
+    eng.close()
+    recs = (_KeyRecord * max(n, 1))()
+    t1 = time.perf_counter()
+    if lib().og_hip_finish_hits(0, seed, buf, n, recs, e, len(e)) != 0:
+        raise OgError(e.value.decode())
+    sink_s = time.perf_counter() - t1
+    raw = np.frombuffer(recs, dtype=np.uint8).reshape(-1, ctypes.sizeof(_KeyRecord))[:n]
+    status = raw[:, _KeyRecord.status.offset:_KeyRecord.status.offset + 4].copy().view(np.int32)
+    return {
+        "workload": "10,000 prefixes of 5-8 symbols (ChaCha20 seeded 2), 24-bit bitmap + exact "
+                    f"table, {steps * batch} candidates ({steps} launches of {batch})",
+        "keys_per_s": steps * batch / search_s,
+        "survivors": n,
+        "sink_s": sink_s,
+        "sink_hits_per_s": n / sink_s if sink_s > 0 else None,
+        "verified": int((status == 0).sum()),
+        "clamping_dropped": int((status == 1).sum()),
+        "mismatched": int((status == 2).sum()),
+    }
+
+
 def cpu_baseline(prefix, seconds):
     from tests.oracle_lib import OracleFilters, load
     import ctypes
@@ -184,6 +236,9 @@ def main():
     ap.add_argument("--threads", type=int, default=1 << 19)
     ap.add_argument("--half", type=int, default=1024)
     ap.add_argument("--rounds", type=int, default=1)
+    ap.add_argument("--no-dict", action="store_true",
+                    help="skip the dictionary leg (10,000 prefixes over 2^34 candidates, every "
+                         "hit through the hit sink)")
     ap.add_argument("--tuned", action="store_true",
                     help="run the autotune walk first and search with the shape it picks")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -285,6 +340,8 @@ def main():
         },
     }
     eng.close()
+    if not args.no_dict and not args.filter:
+        out["dictionary"] = dictionary_leg(args.threads, args.half, args.rounds)
     if args.mix_seconds > 0:
         out["batched"] = batched_leg(args.mix_seconds)
     if args.serve_seconds > 0:

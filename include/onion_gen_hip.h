@@ -96,6 +96,12 @@ int og_hip_engine_tune(int ordinal, const uint8_t start_xyzt[128], const og_filt
 /* The shape an engine runs: chains, pairs per window, windows per launch. */
 void og_hip_engine_shape(const og_engine *e, uint32_t *threads, uint32_t *half, uint32_t *rounds);
 
+/* Survivors one advance may return: the reference's hit_room for its batch
+ * and bitmap (src/gpu/mod.rs:306-312), raised to four times the expected
+ * survivors plus 1024 when the filter set lets more through (a dictionary),
+ * at most 2^22.  Size the advance's output by it. */
+uint32_t og_hip_engine_room(const og_engine *e);
+
 /* Replaces Engine::threads / Engine::batch (src/gpu/mod.rs:446-467). */
 uint32_t og_hip_engine_threads(const og_engine *e);
 uint64_t og_hip_engine_batch(const og_engine *e);

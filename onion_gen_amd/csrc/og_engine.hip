@@ -248,6 +248,36 @@ int check_device(int ordinal, char *err, size_t errcap) {
   return 0;
 }
 
+// The chance that one candidate survives the device's filters: for a
+// bitmap alone its occupancy, for a filter set the sum over its filters of the
+// chance each matches (placements times the share of symbols each position
+// allows).
+double survivor_rate(const FilterSet *fs, const DevFilterHost &filt) {
+  if (!fs) {
+    if (!filt.bits || filt.bitmap.empty()) return 0;
+    uint64_t set = 0;
+    for (uint32_t w : filt.bitmap) set += (uint64_t)__builtin_popcount(w);
+    return (double)set / (double)(1ull << filt.bits);
+  }
+  double rate = 0;
+  for (const Filter &f : fs->filters) {
+    double p = 1;
+    for (uint32_t set : f.sets) p *= __builtin_popcount(set) / 32.0;
+    const int len = (int)f.sets.size();
+    rate += p * (f.anchor == kAnywhere ? (double)(kAddressLen - len + 1) : 1.0);
+  }
+  return rate;
+}
+
+// Survivor room per launch: the reference's hit_room (src/gpu/mod.rs:306-312),
+// raised to four times the expected survivors plus 1024 when the filters let
+// more through than one bitmap slot in 2^bits, up to 2^22 records.
+uint32_t survivor_room(uint64_t batch, const FilterSet *fs, const DevFilterHost &filt) {
+  const uint32_t floor_room = og_hit_room(batch, filt.bits ? filt.bits : 24);
+  const double want = 4.0 * (double)batch * survivor_rate(fs, filt) + 1024.0;
+  return (uint32_t)std::max<double>(floor_room, std::min<double>(want, (double)(1u << 22)));
+}
+
 int create_engine(int ordinal, const uint8_t start_xyzt[128], const og_engine_opts *opts,
                   const FilterSet *fs, const uint64_t *bitmap, size_t bitmap_words,
                   uint32_t bitmap_bits, og_engine **out, char *err, size_t errcap) {
@@ -312,7 +342,7 @@ int create_engine(int ordinal, const uint8_t start_xyzt[128], const og_engine_op
   // ---- buffers
   const size_t scratch_words = scratch_planes(o.half) * 10 * o.threads;
   e->cap = o.hit_cap ? o.hit_cap
-                     : og_hit_room((uint64_t)o.threads * S * o.rounds, e->filt.bits ? e->filt.bits : 24);
+                     : survivor_room((uint64_t)o.threads * S * o.rounds, fs, e->filt);
   OG_TRY(hipMalloc(&e->d_state, (size_t)40 * o.threads * 4));
   OG_TRY(hipMalloc(&e->d_table, std::max<size_t>(table.size(), 30) * 4));
   OG_TRY(hipMalloc(&e->d_scratch, std::max<size_t>(scratch_words, 1) * 4));
@@ -528,6 +558,8 @@ void og_hip_engine_shape(const og_engine *e, uint32_t *threads, uint32_t *half,
 }
 
 uint32_t og_hip_engine_threads(const og_engine *e) { return e ? e->threads : 0; }
+
+uint32_t og_hip_engine_room(const og_engine *e) { return e ? e->cap : 0; }
 
 uint64_t og_hip_engine_batch(const og_engine *e) {
   return e ? (uint64_t)e->threads * (2ull * e->half + 1) * e->rounds : 0;

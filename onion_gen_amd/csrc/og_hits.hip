@@ -9,7 +9,8 @@
 //      whole batch at once (the independent check of src/run.rs:1089-1104 and
 //      src/verify.rs:89-189, which the reference does per hit on the host);
 //   3. the 255 bits of y compared with the survivor's, the sign taken from the
-//      re-derived key, and the hostname built (src/key.rs:121-132).
+//      re-derived key, and the hostname built (src/key.rs:121-132); the
+//      checksum is hashed on the device beside the key, the host only encodes.
 //
 // The re-derivation shares nothing with the search but the curve: a fixed-base
 // comb over B in the 8 x 32 field of fe.hpp (64 four-bit windows, the 16
@@ -26,6 +27,7 @@
 #include <vector>
 
 #include "../../include/onion_gen_hip.h"
+#include "keccak.hpp"
 #include "og_host.hpp"
 
 using namespace og;
@@ -35,6 +37,8 @@ namespace {
 constexpr int kCombWindows = 64;  // four bits each
 constexpr int kCombWords = 24;    // y+x, y-x, 2dxy: 8 words each
 
+// out: per hit the packed public key (8 words) and its address checksum (one
+// word), so the host only has to base32-encode.
 __global__ __launch_bounds__(256) void og_base_mult_kernel(const uint32_t *scalars,
                                                            const uint32_t *comb, uint32_t *out,
                                                            uint32_t n) {
@@ -59,9 +63,28 @@ __global__ __launch_bounds__(256) void og_base_mult_kernel(const uint32_t *scala
   const Fe zinv = fe_invert(p.Z);
   const Fe y = fe_canon(fe_mul(p.Y, zinv));
   const Fe x = fe_canon(fe_mul(p.X, zinv));
+  uint32_t pk[8];
 #pragma unroll
-  for (int l = 0; l < 8; ++l) out[(size_t)i * 8 + l] = y.v[l];
-  out[(size_t)i * 8 + 7] |= (x.v[0] & 1u) << 31;
+  for (int l = 0; l < 8; ++l) pk[l] = y.v[l];
+  pk[7] |= (x.v[0] & 1u) << 31;
+#pragma unroll
+  for (int l = 0; l < 8; ++l) out[(size_t)i * 9 + l] = pk[l];
+  out[(size_t)i * 9 + 8] = onion_checksum(pk);
+}
+
+// base32 of the 35 address bytes (src/base32.rs:31-88): 56 symbols.
+void address_text(const uint8_t pk[32], uint32_t chk, char out[56]) {
+  static const char kAlphabet[] = "abcdefghijklmnopqrstuvwxyz234567";
+  uint8_t b[35];
+  std::memcpy(b, pk, 32);
+  b[32] = (uint8_t)chk;
+  b[33] = (uint8_t)(chk >> 8);
+  b[34] = 0x03;
+  for (int m = 0; m < 56; ++m) {
+    const int bit = 5 * m, byte = bit >> 3, sh = bit & 7;
+    const unsigned two = ((unsigned)b[byte] << 8) | (byte + 1 < 35 ? b[byte + 1] : 0u);
+    out[m] = kAlphabet[(two >> (11 - sh)) & 31u];
+  }
 }
 
 void put_err(char *err, size_t cap, const std::string &msg) {
@@ -129,17 +152,24 @@ int og_hip_finish_hits(int ordinal, const uint8_t seed[32], const og_survivor *h
     og_key_record &r = out[i];
     std::memset(&r, 0, sizeof r);
     r.offset = hits[i].offset;
-    r.status = og_key_expanded_secret_at_offset(seed, hits[i].offset, r.expanded_secret)
-                   ? OG_HIT_OK
-                   : OG_HIT_CLAMPING;
+    // add_to_scalar on the clamped left half; the right half is carried over
+    std::memcpy(r.expanded_secret, base, 64);
+    unsigned __int128 carry = hits[i].offset;
+    for (int b = 0; b < 32 && carry; ++b) {
+      carry += r.expanded_secret[b];
+      r.expanded_secret[b] = (uint8_t)carry;
+      carry >>= 8;
+    }
+    const uint8_t lo = r.expanded_secret[0], hi = r.expanded_secret[31];
+    r.status = ((lo & 248) == lo && ((hi & 63) | 64) == hi) ? OG_HIT_OK : OG_HIT_CLAMPING;
     std::memcpy(&scalars[i * 8], r.expanded_secret, 32);
   }
   const std::vector<uint32_t> &comb = comb_table();
   uint32_t *d_scalars = nullptr, *d_comb = nullptr, *d_out = nullptr;
-  std::vector<uint32_t> pks(n * 8);
+  std::vector<uint32_t> pks(n * 9);
   hipError_t e = hipSetDevice(ordinal);
   if (e == hipSuccess) e = hipMalloc(&d_scalars, n * 32);
-  if (e == hipSuccess) e = hipMalloc(&d_out, n * 32);
+  if (e == hipSuccess) e = hipMalloc(&d_out, n * 36);
   if (e == hipSuccess) e = hipMalloc(&d_comb, comb.size() * 4);
   if (e == hipSuccess) e = hipMemcpy(d_scalars, scalars.data(), n * 32, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(d_comb, comb.data(), comb.size() * 4, hipMemcpyHostToDevice);
@@ -148,7 +178,7 @@ int og_hip_finish_hits(int ordinal, const uint8_t seed[32], const og_survivor *h
                        d_scalars, d_comb, d_out, (uint32_t)n);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpy(pks.data(), d_out, n * 32, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(pks.data(), d_out, n * 36, hipMemcpyDeviceToHost);
   (void)hipFree(d_scalars);
   (void)hipFree(d_out);
   (void)hipFree(d_comb);
@@ -159,7 +189,7 @@ int og_hip_finish_hits(int ordinal, const uint8_t seed[32], const og_survivor *h
   for (size_t i = 0; i < n; ++i) {
     og_key_record &r = out[i];
     if (r.status != OG_HIT_OK) continue;
-    std::memcpy(r.public_key, &pks[i * 8], 32);
+    std::memcpy(r.public_key, &pks[i * 9], 32);
     // the survivor carries y with the sign bit clear: compare all 255 bits
     uint8_t y[32];
     std::memcpy(y, r.public_key, 32);
@@ -168,9 +198,7 @@ int og_hip_finish_hits(int ordinal, const uint8_t seed[32], const og_survivor *h
       r.status = OG_HIT_MISMATCH;
       continue;
     }
-    char addr[57];
-    og_key_address(r.public_key, addr);
-    std::memcpy(r.hostname, addr, 56);
+    address_text(r.public_key, pks[i * 9 + 8], r.hostname);
     std::memcpy(r.hostname + 56, ".onion", 7);
   }
   return 0;

@@ -66,10 +66,10 @@ class Engine:
     that takes a compiled :class:`onion_gen_amd.filters.FilterSet`.
     """
 
-    def __init__(self, handle, room=1 << 18):
+    def __init__(self, handle):
         self._h = handle
         self._out = None
-        self._room = room
+        self._room = lib().og_hip_engine_room(handle)
 
     @classmethod
     def new(cls, start_xyzt, threads, bitmap, bitmap_bits, device=0):
@@ -91,7 +91,7 @@ class Engine:
                                            filters.handle, ctypes.byref(h), e, len(e))
         if rc != 0:
             raise OgError(e.value.decode())
-        return cls(h, max(1 << 18, hit_cap))
+        return cls(h)
 
     @staticmethod
     def tune(start_xyzt, filters, device=0):
@@ -111,6 +111,10 @@ class Engine:
         t, h, r = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         lib().og_hip_engine_shape(self._h, ctypes.byref(t), ctypes.byref(h), ctypes.byref(r))
         return t.value, h.value, r.value
+
+    def room(self):
+        """Survivors one advance may return (og_hip_engine_room)."""
+        return self._room
 
     def threads(self):
         return lib().og_hip_engine_threads(self._h)
