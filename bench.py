@@ -10,8 +10,9 @@ scalar on the host and checked against the prefix.
 Prints one JSON line (rank 0).  `value` is candidates/s with the start state
 already on the device; `roofline` prices the search kernel's field
 multiplications against the VALU 32x32->64 multiply-add issue rate;
-`cpu_baseline` is the oracle's BatchEngine (the reference's CPU engine restated
-in C) on the host cores, on a bounded sample.
+`cpu_baseline` is the oracle's PairEngine (the reference's default CPU engine
+restated in C) on the host cores, on a bounded sample, with its BatchEngine
+beside it.
 """
 import argparse
 import json
@@ -208,6 +209,11 @@ def dictionary_leg(threads, half, rounds):
 
 
 def cpu_baseline(prefix, seconds):
+    """The reference's CPU engines restated in C (the oracle), on the host's
+    cores, one block per thread: PairEngine at its default batch of 8192
+    (src/pairs.rs, the reference's default and fastest CPU path, with its limb
+    probe) for `seconds`, and BatchEngine at width 256 (src/batch.rs) for half
+    that beside it."""
     from tests.oracle_lib import OracleFilters, load
     import ctypes
     from onion_gen_amd import workload
@@ -218,12 +224,17 @@ def cpu_baseline(prefix, seconds):
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
     fs = OracleFilters(prefix)
-    examined = ctypes.c_uint64()
-    rate = L.obench_batch(workload.chacha_stream(0, 32), 256, cores, seconds, fs.h,
-                          ctypes.byref(examined))
-    return {"value": rate, "unit": "keys/s", "cores": cores, "kind": "port",
-            "sample": f"oracle BatchEngine width 256 (src/batch.rs restated in C), one chain per "
-                      f"thread, prefix '{prefix}', {seconds:.0f} s wall, {examined.value} candidates"}
+    seed = workload.chacha_stream(0, 32)
+    ex_p, ex_b = ctypes.c_uint64(), ctypes.c_uint64()
+    pairs = L.obench_pairs(seed, 8192, cores, seconds, fs.h, ctypes.byref(ex_p))
+    batch = L.obench_batch(seed, 256, cores, seconds / 2, fs.h, ctypes.byref(ex_b))
+    return {"value": pairs, "unit": "keys/s", "cores": cores, "kind": "port",
+            "sample": f"oracle PairEngine batch 8192 (src/pairs.rs restated in C, limb probe), one "
+                      f"block per thread, prefix '{prefix}', {seconds:.0f} s wall, "
+                      f"{ex_p.value} candidates",
+            "batch_engine": {"value": batch, "sample": f"oracle BatchEngine width 256 "
+                                                       f"(src/batch.rs), {seconds / 2:.0f} s, "
+                                                       f"{ex_b.value} candidates"}}
 
 
 def main():

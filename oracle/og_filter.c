@@ -453,6 +453,14 @@ int ofilter_match_address(const ofilterset *s, const uint8_t addr[35]) {
   return -1;
 }
 
+/* BitmapIndex::may_match_head (src/filter.rs:2321-2604) for a key's leading 64
+ * bits, big-endian; 1 when the set has no index (nothing is ruled out). */
+int ofilter_may_match_head(const ofilterset *s, uint64_t head) {
+  if (!s->index_bits) return 1;
+  const uint64_t sl = head >> (64 - s->index_bits);
+  return (int)((s->words[sl >> 6] >> (sl & 63)) & 1);
+}
+
 int ofilter_match_key(const ofilterset *s, const uint8_t packed[32]) {
   /* FilterSet::match_literals (src/filter.rs:2194-2217) */
   if (s->index_bits) {

@@ -8,6 +8,7 @@
  *   field   : src/field.rs        (radix 2^64 x 4, folded with 2^256 = 38)
  *   curve   : src/curve.rs        (extended coordinates, cached addend)
  *   batch   : src/batch.rs        (BatchEngine: chain + one inversion per batch)
+ *   pairs   : src/pairs.rs        (PairEngine: two candidates per affine offset)
  *   key     : src/key.rs          (seed -> SHA-512 -> clamp, pack, checksum, address)
  *   base32  : src/base32.rs
  *   filter  : src/filter.rs       (parse, naive matcher, bitmap index, length buckets)
@@ -89,6 +90,23 @@ void obatch_run(obatch *e, opoint *acc);
 const uint8_t *obatch_packed(const obatch *e, size_t i); /* sign bit clear */
 void obatch_packed_with_sign(const obatch *e, size_t i, uint8_t out[32]);
 
+/* ---- pair engine: src/pairs.rs (the reference's default CPU engine) --- */
+struct ofilterset;
+typedef struct opairs opairs;
+/* batch: the request of PairEngine::new; a round yields 2*(batch/2)+1 candidates */
+opairs *opairs_new(size_t batch);
+void opairs_free(opairs *e);
+size_t opairs_batch_size(const opairs *e);
+/* PairEngine::begin: moves the block's base point to the first centre. */
+void opairs_begin(opairs *e, opoint *acc);
+/* PairEngine::run with the three-form limb probe of the set's bitmap (every
+ * candidate survives when probe is NULL); returns the survivor count. */
+size_t opairs_run(opairs *e, opoint *acc, const struct ofilterset *probe);
+/* Survivor i of the last round: its index in the round and canonical y. */
+void opairs_survivor(const opairs *e, size_t i, uint32_t *index, uint8_t packed[32]);
+/* PairEngine::delta: the scalar delta of a round index. */
+uint64_t opairs_delta(const opairs *e, uint32_t index);
+
 /* ---- filters: src/filter.rs ------------------------------------------- */
 typedef struct ofilterset ofilterset;
 /* Parses newline-separated filters ("abc", "prefix:abc", "suffix:xyd",
@@ -107,6 +125,8 @@ int ofilter_match_address(const ofilterset *s, const uint8_t addr[35]);
  * (FilterSet::match_key restricted to literal prefixes); -1 if none. */
 int ofilter_match_key(const ofilterset *s, const uint8_t packed[32]);
 int ofilter_needs_checksum(const ofilterset *s);
+/* The bitmap test of a key's leading 64 bits (big-endian); 1 without an index. */
+int ofilter_may_match_head(const ofilterset *s, uint64_t head);
 
 /* ---- the search oracle ------------------------------------------------ */
 typedef struct {
@@ -129,6 +149,9 @@ void odigest(const uint8_t start_xyzt[128], uint64_t k0, uint64_t n, size_t widt
  * for `seconds`, filter `s` (literal prefixes, bitmap path).  Returns
  * candidates/s; *examined receives the count. */
 double obench_batch(const uint8_t seed[32], size_t width, int threads, double seconds,
+                    const ofilterset *s, uint64_t *examined);
+/* The same harness over PairEngine(batch) with the limb probe. */
+double obench_pairs(const uint8_t seed[32], size_t batch, int threads, double seconds,
                     const ofilterset *s, uint64_t *examined);
 
 #ifdef __cplusplus

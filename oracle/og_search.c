@@ -153,3 +153,60 @@ double obench_batch(const uint8_t seed[32], size_t width, int threads, double se
   if (examined) *examined = total;
   return dt > 0 ? total / dt : 0;
 }
+
+typedef struct {
+  uint8_t start[128];
+  size_t batch;
+  double seconds;
+  const ofilterset *s;
+  uint64_t examined;
+  uint64_t hits;
+} pairs_arg;
+
+static void *pairs_thread(void *p) {
+  pairs_arg *a = p;
+  opoint acc;
+  opoint_from_bytes(&acc, a->start);
+  opairs *e = opairs_new(a->batch);
+  opairs_begin(e, &acc);
+  const size_t n = opairs_batch_size(e);
+  double t0 = now_s();
+  while (now_s() - t0 < a->seconds) {
+    const size_t k = opairs_run(e, &acc, a->s);
+    for (size_t i = 0; i < k; i++) {
+      uint32_t idx;
+      uint8_t packed[32];
+      opairs_survivor(e, i, &idx, packed);
+      if (ofilter_match_key(a->s, packed) >= 0) a->hits++;
+    }
+    a->examined += n;
+  }
+  opairs_free(e);
+  return NULL;
+}
+
+double obench_pairs(const uint8_t seed[32], size_t batch, int threads, double seconds,
+                    const ofilterset *s, uint64_t *examined) {
+  pthread_t *th = calloc((size_t)threads, sizeof *th);
+  pairs_arg *args = calloc((size_t)threads, sizeof *args);
+  double t0 = now_s();
+  for (int t = 0; t < threads; t++) {
+    uint8_t bs[32];
+    orun_block_seed(seed, (uint64_t)t, bs);
+    okey_start_point(bs, args[t].start);
+    args[t].batch = batch;
+    args[t].seconds = seconds;
+    args[t].s = s;
+    pthread_create(&th[t], NULL, pairs_thread, &args[t]);
+  }
+  uint64_t total = 0;
+  for (int t = 0; t < threads; t++) {
+    pthread_join(th[t], NULL);
+    total += args[t].examined;
+  }
+  double dt = now_s() - t0;
+  free(th);
+  free(args);
+  if (examined) *examined = total;
+  return dt > 0 ? total / dt : 0;
+}

@@ -55,6 +55,20 @@ def load():
     L.obatch_packed.restype = c.POINTER(c.c_uint8)
     L.obatch_packed.argtypes = [c.c_void_p, c.c_size_t]
     L.obatch_packed_with_sign.argtypes = [c.c_void_p, c.c_size_t, c.c_char_p]
+    L.obench_pairs.restype = c.c_double
+    L.obench_pairs.argtypes = [c.c_char_p, c.c_size_t, c.c_int, c.c_double, c.c_void_p,
+                               c.POINTER(c.c_uint64)]
+    L.opairs_new.restype = c.c_void_p
+    L.opairs_new.argtypes = [c.c_size_t]
+    L.opairs_free.argtypes = [c.c_void_p]
+    L.opairs_batch_size.restype = c.c_size_t
+    L.opairs_batch_size.argtypes = [c.c_void_p]
+    L.opairs_begin.argtypes = [c.c_void_p, c.c_void_p]
+    L.opairs_run.restype = c.c_size_t
+    L.opairs_run.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p]
+    L.opairs_survivor.argtypes = [c.c_void_p, c.c_size_t, c.POINTER(c.c_uint32), c.c_char_p]
+    L.opairs_delta.restype = c.c_uint64
+    L.opairs_delta.argtypes = [c.c_void_p, c.c_uint32]
     for name in ("oracle_fe_mul", "oracle_fe_add", "oracle_fe_sub"):
         getattr(L, name).argtypes = [c.c_char_p, c.c_char_p, c.c_char_p, c.c_size_t]
     for name in ("oracle_fe_invert", "oracle_fe_canon"):
@@ -151,3 +165,28 @@ def batch_chain(start, width, batches):
             out.append(b.raw)
     L.obatch_free(e)
     return out
+
+
+def pair_engine_hits(start, batch, rounds, filters):
+    """The PairEngine restatement over `rounds` rounds from `start` (the point
+    of delta 0): (sorted (k, packed y) of the survivors that match the set's
+    key filters, the set of every delta the rounds covered)."""
+    L = load()
+    acc = ctypes.create_string_buffer(4 * 4 * 8)  # opoint: 4 field elements of 4 u64
+    L.opoint_from_bytes(acc, bytes(start))
+    e = L.opairs_new(batch)
+    L.opairs_begin(e, acc)
+    n = L.opairs_batch_size(e)
+    hits, deltas = [], set()
+    idx = ctypes.c_uint32()
+    packed = ctypes.create_string_buffer(32)
+    for _ in range(rounds):
+        k = L.opairs_run(e, acc, filters.h)
+        for i in range(n):
+            deltas.add(L.opairs_delta(e, i))
+        for i in range(k):
+            L.opairs_survivor(e, i, ctypes.byref(idx), packed)
+            if L.ofilter_match_key(filters.h, packed.raw) >= 0:
+                hits.append((L.opairs_delta(e, idx.value) // 8, packed.raw))
+    L.opairs_free(e)
+    return sorted(hits), deltas

@@ -245,3 +245,19 @@ def test_config1_hits_match_golden_fixture(oracle):
     for k, pk, addr in want[:3]:
         assert ref.encode(ref.mul(s0 + 8 * k)) == pk
         assert addr.startswith(g["prefix"])
+
+
+def test_pair_engine_matches_batch_engine():
+    """The PairEngine restatement (src/pairs.rs) covers deltas 8, 16, ...
+    exactly once per round and finds the BatchEngine oracle's hits there."""
+    from tests.oracle_lib import OracleFilters, pair_engine_hits, search, start_point
+    seed = bytes(range(32))
+    start = start_point(seed)
+    for batch, rounds, prefix in ((256, 40, "q"), (33, 300, "2"), (2, 2000, "ab")):
+        fs = OracleFilters(prefix)
+        got, deltas = pair_engine_hits(start, batch, rounds, fs)
+        n = rounds * (2 * (batch // 2) + 1)
+        assert deltas == {8 * k for k in range(1, n + 1)}
+        want = sorted((k, pk[:31] + bytes([pk[31] & 0x7F]))
+                      for k, pk, _ in search(start, 1, n, 256, fs))
+        assert got == want and want, (batch, len(got), len(want))
