@@ -96,11 +96,20 @@ int og_hip_engine_tune(int ordinal, const uint8_t start_xyzt[128], const og_filt
 /* The shape an engine runs: chains, pairs per window, windows per launch. */
 void og_hip_engine_shape(const og_engine *e, uint32_t *threads, uint32_t *half, uint32_t *rounds);
 
+/* The device ordinal an engine runs on. */
+int og_hip_engine_device(const og_engine *e);
+
 /* Survivors one advance may return: the reference's hit_room for its batch
  * and bitmap (src/gpu/mod.rs:306-312), raised to four times the expected
  * survivors plus 1024 when the filter set lets more through (a dictionary),
  * at most 2^22.  Size the advance's output by it. */
 uint32_t og_hip_engine_room(const og_engine *e);
+
+/* Starts the engine over from another start point (its candidate 0), keeping
+ * its shape, filters and buffers: the next advance examines k = 0..batch-1 of
+ * the new start.  No counterpart in the reference, which builds an engine per
+ * block (src/run.rs:1048-1056); here a lease's blocks reuse one engine. */
+int og_hip_engine_restart(og_engine *e, const uint8_t start_xyzt[128], char *err, size_t errcap);
 
 /* Replaces Engine::threads / Engine::batch (src/gpu/mod.rs:446-467). */
 uint32_t og_hip_engine_threads(const og_engine *e);
@@ -176,6 +185,35 @@ int og_hip_finish_hits(int ordinal, const uint8_t seed[32], const og_survivor *h
  * path is written either way), -1 on an error (message in path_out). */
 int og_write_key(const char *dir, const uint8_t public_key[32], const uint8_t expanded_secret[64],
                  char *path_out, size_t cap);
+
+/* ------------------------------------------------------------------ lease */
+
+/* proto::Lease (src/proto.rs:47-57) as the adapter takes it. */
+typedef struct {
+  uint64_t lease;
+  uint8_t seed[32];     /* the root seed (the wire carries it as hex) */
+  uint64_t first_block;
+  uint64_t blocks;
+} og_lease;
+
+/* proto::Found (src/proto.rs:78-84): where a find is. */
+typedef struct {
+  uint64_t lease;
+  uint64_t block;
+  uint64_t offset;      /* within the block's chain: 8k */
+} og_lease_find;
+
+/* A lease worker's search (src/worker/run.rs:165-229 over run::run with
+ * first_block / block_limit, src/run.rs:256-261): for each block b of the lease,
+ * the chain of block_seed(seed, b) (src/run.rs:211-220) through `launches` advances
+ * of the engine (0 = BATCHES_PER_BLOCK, 64, src/run.rs:29), every survivor
+ * verified by og_hip_finish_hits, and confirmed against `filters` when given,
+ * before it is reported.  The engine is restarted
+ * per block.  Writes up to `cap` finds (an overflow is an error), the candidates
+ * examined and the blocks done. */
+int og_hip_run_lease(og_engine *e, const og_lease *lease, const og_filterset *filters,
+                     uint32_t launches, og_lease_find *out, size_t cap, size_t *n,
+                     uint64_t *candidates, uint64_t *blocks_done, char *err, size_t errcap);
 
 /* ---------------------------------------------------------------- filters */
 

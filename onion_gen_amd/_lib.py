@@ -67,6 +67,12 @@ class TuningShape(_c.Structure):
 
 SIGNATURES.update({
     "og_hip_engine_room": (_c.c_uint32, [_c.c_void_p]),
+    "og_hip_engine_restart": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_char_p, _c.c_size_t]),
+    "og_hip_engine_device": (_c.c_int, [_c.c_void_p]),
+    "og_hip_run_lease": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_uint32,
+                                    _c.c_void_p, _c.c_size_t, _c.POINTER(_c.c_size_t),
+                                    _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint64),
+                                    _c.c_char_p, _c.c_size_t]),
     "og_hip_engine_tune": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_void_p,
                                       _c.POINTER(TuningShape), _c.c_char_p, _c.c_size_t]),
     "og_hip_engine_shape": (None, [_c.c_void_p, _c.POINTER(_c.c_uint32),

@@ -561,6 +561,23 @@ uint32_t og_hip_engine_threads(const og_engine *e) { return e ? e->threads : 0; 
 
 uint32_t og_hip_engine_room(const og_engine *e) { return e ? e->cap : 0; }
 
+int og_hip_engine_device(const og_engine *e) { return e ? e->device : -1; }
+
+int og_hip_engine_restart(og_engine *e, const uint8_t start_xyzt[128], char *err, size_t errcap) {
+  if (!e || !start_xyzt) {
+    put_err(err, errcap, "null argument");
+    return -1;
+  }
+  OG_TRY(hipSetDevice(e->device));
+  HostCurve hc;
+  SetupArgs sa = hc.setup_args(start_xyzt, e->d_state, e->threads, 0, e->threads, e->half);
+  hipLaunchKernelGGL(og_setup_kernel, dim3(e->threads / 256), dim3(256), 0, e->stream, sa);
+  OG_TRY(hipGetLastError());
+  OG_TRY(hipStreamSynchronize(e->stream));
+  e->launches = 0;
+  return 0;
+}
+
 uint64_t og_hip_engine_batch(const og_engine *e) {
   return e ? (uint64_t)e->threads * (2ull * e->half + 1) * e->rounds : 0;
 }

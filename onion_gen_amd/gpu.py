@@ -112,6 +112,12 @@ class Engine:
         lib().og_hip_engine_shape(self._h, ctypes.byref(t), ctypes.byref(h), ctypes.byref(r))
         return t.value, h.value, r.value
 
+    def restart(self, start_xyzt):
+        """Starts over from another start point (og_hip_engine_restart)."""
+        e = err_buf()
+        if lib().og_hip_engine_restart(self._h, bytes(start_xyzt), e, len(e)) != 0:
+            raise OgError(e.value.decode())
+
     def room(self):
         """Survivors one advance may return (og_hip_engine_room)."""
         return self._room
