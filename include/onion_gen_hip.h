@@ -220,8 +220,10 @@ int og_hip_run_lease(og_engine *e, const og_lease *lease, const og_filterset *fi
 /* Replaces FilterSet::parse_all + FilterSet::new / with_index_bits
  * (src/filter.rs:1930-1950, 1326-1476): newline-separated filters in the
  * reference's syntax ("abc", "prefix:abc", "suffix:..d", "contains:abcde",
- * '?' and '[a-d]' classes).  index_bits = 0 chooses the width as
- * narrowest_acceptable does (src/filter.rs:2279-2299). */
+ * '?' and '[a-d]' classes, "regex:<expression>" as src/expression.rs parses
+ * it).  index_bits = 0 chooses the width as narrowest_acceptable does
+ * (src/filter.rs:2279-2299).  Expressions are kept after the other filters,
+ * in source order, never absorbed (src/filter.rs:869-876). */
 og_filterset *og_filters_parse(const char *text, uint32_t index_bits, char *err, size_t errcap);
 void og_filters_free(og_filterset *s);
 /* Filters kept after dedupe and absorption. */
@@ -234,15 +236,23 @@ uint32_t og_filters_index_words(const og_filterset *s, uint64_t *words, size_t c
  * or -1. */
 int og_filters_match_key(const og_filterset *s, const uint8_t packed[32]);
 /* Replaces FilterSet::match_whole_address (src/filter.rs:1673-1691): index of
- * a filter the 35 address bytes match, or -1. */
+ * a filter the 35 address bytes match, or -1.  Expressions are matched over
+ * the 56 address symbols as Expression::find does (src/expression.rs:209-215). */
 int og_filters_match_address(const og_filterset *s, const uint8_t address[35]);
+/* Replaces FilterSet::describe_structures (src/filter.rs:1700-1845): one line
+ * per structure the device reads, and per expression its proxies (the
+ * obligatory symbol sequences that guard its automaton) and reach.  Copies at
+ * most cap-1 bytes and a terminator; returns the full length. */
+size_t og_filters_describe(const og_filterset *s, char *out, size_t cap);
 /* The device matcher of the general forms (suffix, contains, class prefixes)
- * run on the host for one key: the two stages of FilterSet::examine_key
- * (src/filter.rs:1625-1691) with Filter::key_prefilter (:782-810).  packed is
- * the canonical y (sign bit clear), sign the sign of x.  key_hit: a placement
- * inside symbols 0..48 matched; need: the key pass left a placement reaching
+ * and the expressions, run on the host for one key: the two stages of
+ * FilterSet::examine_key (src/filter.rs:1625-1691) with Filter::key_prefilter
+ * (:782-810) and the expressions' literal guards (:914-1005).  packed is the
+ * canonical y (sign bit clear), sign the sign of x.  key_hit: the key's
+ * symbols 0..48 settled a match; need: the key stage left a match reaching
  * past symbol 48 possible; hit: the verdict on the whole address.  Returns 0,
- * or -1 when the set has no general filter or does not compile. */
+ * or -1 when the set has neither general filter nor expression, or does not
+ * compile. */
 int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int sign,
                          int *key_hit, int *need, int *hit);
 

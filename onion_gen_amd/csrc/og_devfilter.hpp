@@ -14,20 +14,23 @@
 
 namespace og {
 
-// The general filters of a set packed into automaton banks (og_scan.hpp):
-// first fit, each pattern on consecutive bits of one bank.  Empty string on
-// success.
-inline std::string build_banks(const FilterSet &fs, std::vector<ScanBank> &out) {
+// The general filters of a set, then its expressions' proxies, packed into
+// automaton banks (og_scan.hpp): first fit, each pattern on consecutive bits
+// of one bank.  An expression whose proxies do not fit even shortened gets
+// none, and `every` says so: its automaton then runs on every candidate.
+// Empty string on success.
+inline std::string build_banks(const FilterSet &fs, std::vector<ScanBank> &out,
+                               bool *every = nullptr) {
   out.clear();
   std::vector<int> used;
-  for (uint32_t idx : fs.general) {
-    const Filter &f = fs.filters[idx];
-    const int len = (int)f.sets.size();
+  // 0: placed, 1: no room
+  auto place = [&](const std::vector<uint32_t> &sets, Anchor anchor, bool proxy) {
+    const int len = (int)sets.size();
     const int kept = std::min(len, kBankBits);
     size_t k = 0;
     while (k < out.size() && used[k] + kept > kBankBits) ++k;
     if (k == out.size()) {
-      if (out.size() == (size_t)kMaxBanks) return "the general filters need more than 8 automaton banks";
+      if (out.size() == (size_t)kMaxBanks) return 1;
       ScanBank b;
       std::memset(&b, 0, sizeof b);
       b.jstart = kAddressLen;
@@ -39,15 +42,44 @@ inline std::string build_banks(const FilterSet &fs, std::vector<ScanBank> &out) 
     used[k] += kept;
     for (int i = 0; i < kept; ++i)
       for (int c = 0; c < 32; ++c)
-        if ((f.sets[i] >> c) & 1u) b.b[c] |= 1u << (off + i);
+        if ((sets[i] >> c) & 1u) b.b[c] |= 1u << (off + i);
     const uint32_t first = 1u << off, last = 1u << (off + kept - 1);
-    if (f.anchor == kStart) b.istart |= first;
+    if (anchor == kStart) b.istart |= first;
     else b.iany |= first;
-    if (len > kBankBits) b.flong |= last;
-    else if (f.anchor == kEnd) b.fend |= last;
+    if (proxy) b.fmaybe |= last;
+    else if (len > kBankBits) b.flong |= last;
+    else if (anchor == kEnd) b.fend |= last;
     else b.fany |= last;
-    const uint32_t js = f.anchor == kEnd ? (uint32_t)(kAddressLen - len) : 0u;
+    const uint32_t js = anchor == kEnd ? (uint32_t)(kAddressLen - len) : 0u;
     b.jstart = std::min(b.jstart, js);
+    return 0;
+  };
+  for (uint32_t idx : fs.general) {
+    const Filter &f = fs.filters[idx];
+    if (place(f.sets, f.anchor, false))
+      return "the general filters need more than 8 automaton banks";
+  }
+  if (every) *every = false;
+  for (uint32_t idx : fs.regexes) {
+    const Regex &re = *fs.filters[idx].regex;
+    int free_bits = (kMaxBanks - (int)out.size()) * kBankBits;
+    for (int u : used) free_bits += kBankBits - u;
+    bool placed = false;
+    // fragmentation can defeat a budget that fits in total: halve and retry
+    for (int budget = free_bits; budget > 0 && !placed; budget /= 2) {
+      const std::vector<RegexProxy> ps = re.proxies(budget);
+      if (ps.empty()) break;
+      const std::vector<ScanBank> out0 = out;
+      const std::vector<int> used0 = used;
+      placed = true;
+      for (const RegexProxy &p : ps)
+        if (place(p.sets, p.anchored ? kStart : kAnywhere, true)) placed = false;
+      if (!placed) {
+        out = out0;
+        used = used0;
+      }
+    }
+    if (!placed && every) *every = true;
   }
   for (ScanBank &b : out) {
     for (int c = 0; c < 32; ++c) {
@@ -68,13 +100,18 @@ struct DevFilterHost {
   std::vector<uint64_t> keys;
   std::vector<uint32_t> pats;     // kPatWords per pattern
   std::vector<ScanBank> banks;    // the automaton over the same patterns
+  std::vector<uint32_t> regex;    // per expression: entry offset, first state
+  std::vector<uint16_t> dfa;      // the expressions' automata
+  bool regex_every = false;
   // device copies
   uint32_t *d_bitmap = nullptr, *d_buckets = nullptr, *d_pats = nullptr, *d_banks = nullptr;
+  uint32_t *d_regex = nullptr;
+  uint16_t *d_dfa = nullptr;
   uint64_t *d_keys = nullptr;
 
   size_t device_bytes() const {
     return bitmap.size() * 4 + buckets.size() * 4 + keys.size() * 8 + pats.size() * 4 +
-           banks.size() * sizeof(ScanBank);
+           banks.size() * sizeof(ScanBank) + regex.size() * 4 + dfa.size() * 2;
   }
 
   // Empty string on success.
@@ -104,9 +141,11 @@ struct DevFilterHost {
         keys.insert(keys.end(), b.keys.begin(), b.keys.end());
       }
     }
-    if (!fs.general.empty()) {
+    if (!fs.general.empty() || !fs.regexes.empty()) {
       if (fs.general.size() > (size_t)kMaxPatterns)
         return "at most 64 general (suffix, contains, class) filters per search";
+      if (fs.regexes.size() > (size_t)kMaxRegexes)
+        return "at most 16 regex: filters per search";
       mode |= kModePattern;
       for (uint32_t i : fs.general) {
         DevPattern p = dev_pattern(fs.filters[i]);
@@ -116,8 +155,14 @@ struct DevFilterHost {
         pats.push_back(p.key_last);
         for (int s = 0; s < kAddressLen; ++s) pats.push_back(p.sets[s]);
       }
-      const std::string e = build_banks(fs, banks);
+      const std::string e = build_banks(fs, banks, &regex_every);
       if (!e.empty()) return e;
+      for (uint32_t i : fs.regexes) {
+        const Regex &re = *fs.filters[i].regex;
+        regex.push_back((uint32_t)dfa.size());
+        regex.push_back(re.dfa_start());
+        dfa.insert(dfa.end(), re.dfa().begin(), re.dfa().end());
+      }
     }
     if (!mode) return "the filter set is empty";
     return "";
@@ -144,6 +189,8 @@ struct DevFilterHost {
     if (e == hipSuccess) e = upload(&d_buckets, buckets, s);
     if (e == hipSuccess) e = upload(&d_keys, keys, s);
     if (e == hipSuccess) e = upload(&d_pats, pats, s);
+    if (e == hipSuccess) e = upload(&d_regex, regex, s);
+    if (e == hipSuccess) e = upload(&d_dfa, dfa, s);
     if (e == hipSuccess && !banks.empty()) {
       std::vector<uint32_t> words(banks.size() * kBankWords);
       std::memcpy(words.data(), banks.data(), words.size() * 4);
@@ -160,7 +207,10 @@ struct DevFilterHost {
     (void)hipFree(d_keys);
     (void)hipFree(d_pats);
     (void)hipFree(d_banks);
-    d_bitmap = d_buckets = d_pats = d_banks = nullptr;
+    (void)hipFree(d_regex);
+    (void)hipFree(d_dfa);
+    d_bitmap = d_buckets = d_pats = d_banks = d_regex = nullptr;
+    d_dfa = nullptr;
     d_keys = nullptr;
   }
 
@@ -179,6 +229,10 @@ struct DevFilterHost {
     a.banks = d_banks;
     a.npat = (uint32_t)(pats.size() / kPatWords);
     a.pats = d_pats;
+    a.nregex = (uint32_t)(regex.size() / 2);
+    a.regex_every = regex_every ? 1u : 0u;
+    a.regex = d_regex;
+    a.dfa = d_dfa;
     return a;
   }
 };

@@ -261,6 +261,12 @@ double survivor_rate(const FilterSet *fs, const DevFilterHost &filt) {
   }
   double rate = 0;
   for (const Filter &f : fs->filters) {
+    if (f.regex) {
+      // its match rate is not computable from the text (src/filter.rs:1586-1590);
+      // the proxies' pass rate bounds it
+      rate += f.regex->proxy_rate();
+      continue;
+    }
     double p = 1;
     for (uint32_t set : f.sets) p *= __builtin_popcount(set) / 32.0;
     const int len = (int)f.sets.size();

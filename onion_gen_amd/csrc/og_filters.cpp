@@ -6,6 +6,9 @@
 //                                  kept one's is dropped, :873-882)
 //   prefix bitmap                  BitmapIndex :2321-2604, narrowest_acceptable :2279-2299
 //   exact lookup                   LengthBucket :2645-2701, match_literals :2194-2217
+//   regex: form                    Pattern::parse_regex :353-375 (the body after
+//                                  the first colon, trimmed), regexes held apart
+//                                  and never absorbed (:869-876, 1344-1358)
 #include "og_filters.hpp"
 
 #include <algorithm>
@@ -56,18 +59,25 @@ std::string parse_filter(const std::string &raw_in, Filter &out) {
   std::string raw = trim(raw_in);
   out = Filter();
   out.text = raw;
-  if (raw.rfind("regex:", 0) == 0 || raw.rfind("REGEX:", 0) == 0)
-    return "the regex: form is not served by the device path";
   std::string body;
   size_t colon = raw.find(':');
   if (colon != std::string::npos) {
     std::string name;
     for (char c : raw.substr(0, colon))
       if (!std::isspace((unsigned char)c)) name += (char)std::tolower((unsigned char)c);
+    if (name == "regex") {
+      out.anchor = kAnywhere;
+      out.literal = false;
+      out.canonical = trim(raw.substr(colon + 1));
+      const std::string e = Regex::compile(out.canonical, out.regex);
+      if (!e.empty()) return e;
+      if (out.regex->anchored()) out.anchor = kStart;
+      return "";
+    }
     if (name == "prefix") out.anchor = kStart;
     else if (name == "contains") out.anchor = kAnywhere;
     else if (name == "suffix") out.anchor = kEnd;
-    else return "unknown form \"" + name + "\": expected prefix, contains or suffix";
+    else return "unknown form \"" + name + "\": expected prefix, contains, suffix or regex";
     body = trim(raw.substr(colon + 1));
   } else {
     out.anchor = kStart;
@@ -149,11 +159,21 @@ std::string parse_filter(const std::string &raw_in, Filter &out) {
 }
 
 bool Filter::matches_key(const uint8_t packed[32]) const {
+  if (regex) return false;  // decided on the address
   const int last = nbytes - 1;
   return std::memcmp(packed, bytes, (size_t)last) == 0 && (packed[last] & mask) == bytes[last];
 }
 
+void address_text56(const uint8_t addr[35], char out[56]) {
+  for (int m = 0; m < kAddressLen; ++m) out[m] = kAlphabet[symbol_at(addr, 35, m)];
+}
+
 bool Filter::matches_address(const uint8_t addr[35]) const {
+  if (regex) {
+    char text[kAddressLen];
+    address_text56(addr, text);
+    return regex->find(text, kAddressLen);
+  }
   const int len = (int)sets.size();
   int first = 0, last = 0;
   if (anchor == kEnd) first = last = kAddressLen - len;
@@ -189,22 +209,28 @@ std::vector<uint64_t> bitmap_for(const FilterSet &s, uint32_t bits) {
 std::string build_filter_set(const std::vector<std::string> &lines, uint32_t index_bits,
                              FilterSet &out) {
   out = FilterSet();
-  std::vector<Filter> parsed;
+  std::vector<Filter> parsed, regexes;
   for (const std::string &line : lines) {
     if (trim(line).empty()) continue;
     Filter f;
     std::string e = parse_filter(line, f);
     if (!e.empty()) return trim(line) + ": " + e;
-    parsed.push_back(std::move(f));
+    (f.is_regex() ? regexes : parsed).push_back(std::move(f));
   }
-  std::stable_sort(parsed.begin(), parsed.end(),
-                   [](const Filter &a, const Filter &b) { return a.canonical < b.canonical; });
+  auto by_canonical = [](const Filter &a, const Filter &b) { return a.canonical < b.canonical; };
+  std::stable_sort(parsed.begin(), parsed.end(), by_canonical);
   for (Filter &f : parsed) {
     if (!out.filters.empty() && f.canonical.rfind(out.filters.back().canonical, 0) == 0) continue;
     out.filters.push_back(std::move(f));
   }
-  for (uint32_t i = 0; i < out.filters.size(); ++i)
-    (out.filters[i].is_fast() ? out.fast : out.general).push_back(i);
+  // expressions absorb nothing and are absorbed by nothing: their texts say
+  // nothing about the addresses they accept
+  std::stable_sort(regexes.begin(), regexes.end(), by_canonical);
+  for (Filter &f : regexes) out.filters.push_back(std::move(f));
+  for (uint32_t i = 0; i < out.filters.size(); ++i) {
+    const Filter &f = out.filters[i];
+    (f.is_regex() ? out.regexes : f.is_fast() ? out.fast : out.general).push_back(i);
+  }
   if (!out.fast.empty()) {
     if (index_bits) {
       if (index_bits >= 8 && index_bits <= 32) {

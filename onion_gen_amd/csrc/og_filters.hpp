@@ -3,8 +3,11 @@
 #pragma once
 #include <stdint.h>
 
+#include <memory>
 #include <string>
 #include <vector>
+
+#include "og_regex.hpp"
 
 namespace og {
 
@@ -22,8 +25,12 @@ struct Filter {
   uint8_t bytes[32] = {0};     // decoded head (<= 49 symbols), padding cleared
   int nbytes = 0;
   uint8_t mask = 0xff;         // significant bits of the last byte
+  std::shared_ptr<const Regex> regex;  // the regex: form (no symbol sets)
 
-  bool is_fast() const { return anchor == kStart && literal && (int)sets.size() <= kKeyOnlySymbols; }
+  bool is_regex() const { return regex != nullptr; }
+  bool is_fast() const {
+    return !regex && anchor == kStart && literal && (int)sets.size() <= kKeyOnlySymbols;
+  }
   bool matches_key(const uint8_t packed[32]) const;
   bool matches_address(const uint8_t addr[35]) const;
 };
@@ -49,7 +56,8 @@ struct DevPattern {
 struct FilterSet {
   std::vector<Filter> filters;  // sorted, deduplicated, absorbed
   std::vector<uint32_t> fast;   // literal prefixes (indices into filters)
-  std::vector<uint32_t> general;  // everything else
+  std::vector<uint32_t> general;  // suffix, contains, class and long prefixes
+  std::vector<uint32_t> regexes;  // regex: filters, after all the others
   uint32_t index_bits = 0;      // 0: no index (src/filter.rs:2321)
   std::vector<uint64_t> words;  // the bitmap
   std::vector<Bucket> buckets;
@@ -66,5 +74,7 @@ std::string build_filter_set(const std::vector<std::string> &lines, uint32_t ind
 std::vector<uint64_t> bitmap_for(const FilterSet &s, uint32_t bits);
 double bitmap_occupancy(const std::vector<uint64_t> &words, uint32_t bits);
 DevPattern dev_pattern(const Filter &f);
+// The 56 symbols of an address as text.
+void address_text56(const uint8_t addr[35], char out[56]);
 
 }  // namespace og

@@ -2,6 +2,7 @@
 // (src/key.rs) and the filter-set entry points of the C ABI.
 #include "og_host.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -261,11 +262,46 @@ int og_filters_match_address(const og_filterset *s, const uint8_t address[35]) {
   return s ? s->set.match_address(address) : -1;
 }
 
+size_t og_filters_describe(const og_filterset *s, char *out, size_t cap) {
+  if (!s) return 0;
+  const FilterSet &fs = s->set;
+  std::string t;
+  if (!fs.fast.empty())
+    t += "prefix table: " + std::to_string(fs.fast.size()) + " literal prefix(es)" +
+         (fs.index_bits ? ", bitmap of 2^" + std::to_string(fs.index_bits) + " bits" : "") + "\n";
+  DevFilterHost h;
+  const std::string e = h.compile(fs);
+  if (!fs.general.empty() || !fs.regexes.empty()) {
+    t += "automaton banks: " + std::to_string(h.banks.size()) + " for " +
+         std::to_string(fs.general.size()) + " general form(s)";
+    if (!fs.regexes.empty()) t += " and the proxies of " + std::to_string(fs.regexes.size()) + " expression(s)";
+    t += e.empty() ? "\n" : " (" + e + ")\n";
+  }
+  for (uint32_t i : fs.regexes) {
+    const Regex &re = *fs.filters[i].regex;
+    t += re.describe();
+    t += re.key_decidable() ? "; decided within the first 49 symbols"
+                            : "; an open candidate costs a SHA3-256";
+    t += "\n";
+  }
+  if (h.regex_every)
+    t += "an expression has no proxy in the banks: its automaton runs on every candidate\n";
+  if (out && cap) {
+    const size_t n = std::min(cap - 1, t.size());
+    std::memcpy(out, t.data(), n);
+    out[n] = 0;
+  }
+  return t.size();
+}
+
 int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int sign,
                          int *key_hit, int *need, int *hit) {
   if (!s || !packed || !key_hit || !need || !hit) return -1;
-  std::vector<ScanBank> banks;
-  if (s->set.general.empty() || !build_banks(s->set, banks).empty()) return -1;
+  // the tables the device reads, compiled the same way
+  DevFilterHost h;
+  if (!h.compile(s->set).empty() || !(h.mode & kModePattern)) return -1;
+  const ScanBank *banks = h.banks.data();
+  const uint32_t nb = (uint32_t)h.banks.size(), nre = (uint32_t)(h.regex.size() / 2);
   Fe c;
   for (int l = 0; l < 8; ++l) {
     c.v[l] = (uint32_t)packed[4 * l] | ((uint32_t)packed[4 * l + 1] << 8) |
@@ -273,14 +309,25 @@ int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int si
   }
   uint32_t kw[8];
   key_words(c, kw);
-  bool kh, nd;
-  scan_key(kw, banks.data(), (uint32_t)banks.size(), kh, nd);
+  // the key stage as the search kernels run it (without the key forms)
+  bool kh, nd, maybe;
+  scan_key(kw, banks, nb, kh, nd, maybe);
+  if (nre) {
+    maybe = (maybe || h.regex_every) && !kh;
+    if (maybe) {
+      const uint32_t r = regex_key(kw, h.regex.data(), h.dfa.data(), nre);
+      kh |= r == 2;
+      nd |= r == 1;
+    }
+  }
+  nd &= !kh;
+  // the address stage, on every key
   uint32_t pk[8];
   for (int l = 0; l < 8; ++l) pk[l] = c.v[l];
   pk[7] |= (uint32_t)(sign & 1) << 31;
   uint32_t w[9];
   address_words(kw, (uint32_t)(sign & 1), onion_checksum(pk), w);
-  uint32_t r = scan_address(w, banks.data(), (uint32_t)banks.size());
+  uint32_t r = scan_address(w, banks, nb);
   if (r == 2) {
     // a truncated pattern's kept part matched: the symbol-wise check
     uint8_t addr[35];
@@ -289,6 +336,7 @@ int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int si
     for (uint32_t g : s->set.general)
       if (s->set.filters[g].matches_address(addr)) r = 1;
   }
+  if (r != 1 && nre && regex_address(w, h.regex.data(), h.dfa.data(), nre)) r = 1;
   *key_hit = kh;
   *need = nd;
   *hit = r == 1;

@@ -47,7 +47,7 @@ struct HitRec {
 enum : uint32_t {
   kModePrefix = 1,   // masked compare of the key words (one literal prefix)
   kModeBitmap = 2,   // prefix bitmap, then the exact table when present
-  kModePattern = 4,  // general forms: the bank automaton of og_scan.hpp
+  kModePattern = 4,  // general forms and expressions: the bank automaton of og_scan.hpp
 };
 
 // Scratch planes (field elements of 10 limbs) per pair: D0, D1, C; a window
@@ -58,6 +58,7 @@ inline size_t scratch_planes(uint32_t half) { return (size_t)kPlanesPerPair * ha
 constexpr int kMaxBuckets = 16;
 constexpr int kMaxPatterns = 64;  // symbol-wise records; the banks bound the set
 constexpr int kPatWords = 4 + 56;  // DevPattern: len, first, last, key_last, sets[56]
+constexpr int kMaxRegexes = 16;
 
 struct DevFilterArgs {
   uint32_t mode;
@@ -71,6 +72,10 @@ struct DevFilterArgs {
   const uint32_t *banks;          // nbanks records of kBankWords
   uint32_t npat;                  // the same patterns, for the symbol-wise check
   const uint32_t *pats;           // npat records of kPatWords
+  uint32_t nregex;                // regular expressions (kModePattern): their
+  uint32_t regex_every;           //   automata; nonzero: one has no proxy in the
+  const uint32_t *regex;          //   banks, so every candidate runs them
+  const uint16_t *dfa;            //   (records and entries: og_scan.hpp)
 };
 
 struct SearchArgs {

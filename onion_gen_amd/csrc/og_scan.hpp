@@ -23,6 +23,12 @@
 // Patterns longer than one bank (32 symbols) keep their first 32 positions;
 // reaching the last of those marks the lane for the symbol-by-symbol check on
 // the full address (`flong`), which such a match makes rare.
+//
+// Regular expressions ride on the same banks: each contributes its obligatory
+// class sequences ("proxies", og_regex.hpp), whose last bits (`fmaybe`) never
+// make a hit.  A proxy occurrence only sends the lane to the expression's
+// deterministic automaton, which decides exactly: over symbols 0..48 first,
+// then, if the expression is still open, over the whole address.
 #pragma once
 #include "fe.hpp"
 
@@ -44,7 +50,8 @@ struct ScanBank {
   uint32_t uall;    // any symbol (52, 53)
   uint32_t u54;     // a, i, q or y
   uint32_t jstart;  // no pattern of the bank starts before this symbol
-  uint32_t pad[6];
+  uint32_t fmaybe;  // last bits of expression proxies
+  uint32_t pad[5];
 };
 static_assert(sizeof(ScanBank) == kBankWords * 4, "bank layout");
 
@@ -66,15 +73,17 @@ OG_HD void key_words(const Fe &c, uint32_t w[8]) {
 
 // The key pass.  `hit`: a placement inside symbols 0..48 matched.  `need`:
 // some placement reaching past 48 is still possible (or a truncated pattern
-// reached its last kept bit), so the full address decides.
+// reached its last kept bit), so the full address decides.  `maybe`: an
+// expression proxy occurred, or may reach past 48.
 OG_HD void scan_key(const uint32_t w[8], const ScanBank *banks, uint32_t nbanks, bool &hit,
-                    bool &need) {
+                    bool &need, bool &maybe) {
   hit = false;
   need = false;
+  maybe = false;
   const uint32_t y248 = w[7] & 1u;  // stream bit 255: the top bit of symbol 51
   for (uint32_t k = 0; k < nbanks; ++k) {
     const ScanBank &s = banks[k];
-    const uint32_t js = s.jstart, iany = s.iany;
+    const uint32_t js = s.jstart, iany = s.iany, fm = s.fmaybe;
     uint32_t D = 0, acc = 0;
     // A bank whose patterns all start past the key span skips the key pass.
     // Otherwise every symbol runs, with the injection masked before jstart
@@ -88,23 +97,31 @@ OG_HD void scan_key(const uint32_t w[8], const ScanBank *banks, uint32_t nbanks,
       }
     }
     hit |= (acc & s.fany) != 0;
-    uint32_t e = acc & s.flong;
+    uint32_t e = acc & s.flong, m = acc;
     const uint32_t t49 = sym_at(w, 49);  // sign bit clear: value 2 of the symbol
     D = ((D << 1) | iany) & (s.b[t49] | s.b[t49 | 2u]);
     e |= D & (s.fany | s.flong);
+    m |= D;
     D = ((D << 1) | iany) & s.b[sym_at(w, 50)];
     e |= D & (s.fany | s.flong);
+    m |= D;
     D = ((D << 1) | iany) & s.u51[y248];
     e |= D & (s.fany | s.flong);
+    m |= D;
     D = ((D << 1) | iany) & s.uall;
     e |= D & (s.fany | s.flong);
+    m |= D;
     D = ((D << 1) | iany) & s.uall;
     e |= D & (s.fany | s.flong);
+    m |= D;
     D = ((D << 1) | iany) & s.u54;
     e |= D & (s.fany | s.flong);
+    m |= D;
     D = ((D << 1) | iany) & s.b[3];  // 'd'
     e |= D & (s.fany | s.flong | s.fend);
+    m |= D;
     need |= e != 0;
+    maybe |= (m & fm) != 0;
   }
 }
 
@@ -147,6 +164,40 @@ OG_HD void address_words(const uint32_t kw[8], uint32_t sign, uint32_t chk, uint
   for (int l = 0; l < 8; ++l) w[l] = kw[l];
   w[7] |= sign << 7;  // the top bit of key byte 31
   w[8] = ((chk & 0xffu) << 24) | (((chk >> 8) & 0xffu) << 16) | (0x03u << 8);
+}
+
+// Regular expressions (og_regex.hpp): one deterministic automaton each, as
+// [states][32] 16-bit entries: the next state, and in bit 15 whether the
+// address may end there.  State 0 can no longer match, state 1 has matched.
+// `recs` holds two words per expression: its entries' offset and first state.
+
+// Symbols 0..48 (what the key settles) through every expression: 2 when one
+// has matched, 1 when one is still open, 0 when all are closed.
+OG_HD uint32_t regex_key(const uint32_t w[8], const uint32_t *recs, const uint16_t *dfa,
+                         uint32_t n) {
+  uint32_t out = 0;
+  for (uint32_t r = 0; r < n; ++r) {
+    const uint16_t *t = dfa + recs[2 * r];
+    uint32_t s = recs[2 * r + 1];
+#pragma unroll
+    for (int j = 0; j < 49; ++j) s = t[s * 32u + sym_at(w, j)] & 0x7fffu;
+    if (s == 1) return 2;
+    if (s != 0) out = 1;
+  }
+  return out;
+}
+
+// All 56 symbols of the address words: whether an expression matches.
+OG_HD bool regex_address(const uint32_t w[9], const uint32_t *recs, const uint16_t *dfa,
+                         uint32_t n) {
+  for (uint32_t r = 0; r < n; ++r) {
+    const uint16_t *t = dfa + recs[2 * r];
+    uint32_t e = recs[2 * r + 1];
+#pragma unroll
+    for (int j = 0; j < 56; ++j) e = t[(e & 0x7fffu) * 32u + sym_at(w, j)];
+    if (e >> 15) return true;
+  }
+  return false;
 }
 
 }  // namespace og

@@ -138,6 +138,25 @@ __device__ __noinline__ bool pattern_slow(uint32_t w0, uint32_t w1, uint32_t w2,
   return false;
 }
 
+// The expressions' automata, out of line like pattern_slow: over the key's
+// symbols 0..48 (2 matched, 1 open, 0 closed) and over the whole address.
+__device__ __noinline__ uint32_t regex_key_slow(uint32_t w0, uint32_t w1, uint32_t w2,
+                                                uint32_t w3, uint32_t w4, uint32_t w5,
+                                                uint32_t w6, uint32_t w7, const uint32_t *recs,
+                                                const uint16_t *dfa, uint32_t n) {
+  const uint32_t w[8] = {w0, w1, w2, w3, w4, w5, w6, w7};
+  return regex_key(w, recs, dfa, n);
+}
+
+__device__ __noinline__ bool regex_address_slow(uint32_t w0, uint32_t w1, uint32_t w2,
+                                                uint32_t w3, uint32_t w4, uint32_t w5,
+                                                uint32_t w6, uint32_t w7, uint32_t w8,
+                                                const uint32_t *recs, const uint16_t *dfa,
+                                                uint32_t n) {
+  const uint32_t w[9] = {w0, w1, w2, w3, w4, w5, w6, w7, w8};
+  return regex_address(w, recs, dfa, n);
+}
+
 // Appends a wave's hits with one atomic per wave.  Records past `cap` are
 // counted but not written: the host reports the overflow.
 __device__ __forceinline__ void append(bool hit, uint32_t slot, uint64_t k, const Fe &c,
@@ -206,11 +225,24 @@ __device__ __forceinline__ void test_key_only(const Fe10 &y, uint64_t k, const D
 
 // The key-derived part of a candidate under the general forms (and the key
 // forms when the set has both): `hit` is final, `need` asks for the address.
+// Lanes where an expression's proxy occurred (all lanes, for an expression
+// without one) run the automata over symbols 0..48: matched is a hit, still
+// open asks for the address.
 template <uint32_t MODE>
 __device__ __forceinline__ void key_stage(const Fe &c, const uint32_t w[8], const DevFilterArgs &f,
                                           const ScanBank *banks, bool &hit, bool &need) {
-  scan_key(w, banks, f.nbanks, hit, need);
+  bool maybe;
+  scan_key(w, banks, f.nbanks, hit, need, maybe);
   if (MODE & (kModePrefix | kModeBitmap)) hit |= exact_key<MODE>(c, f);
+  if (f.nregex) {
+    maybe = (maybe || f.regex_every != 0) && !hit;
+    if (__ballot(maybe) && maybe) {
+      const uint32_t r =
+          regex_key_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], f.regex, f.dfa, f.nregex);
+      hit |= r == 2;
+      need |= r == 1;
+    }
+  }
   need &= !hit;
 }
 
@@ -255,7 +287,9 @@ __device__ __forceinline__ void drain(const uint32_t *scratch, uint32_t top, uin
     const bool hit =
         r == 1 ||
         (r == 2 && pattern_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8], f.pats,
-                                f.npat));
+                                f.npat)) ||
+        (f.nregex && regex_address_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8],
+                                        f.regex, f.dfa, f.nregex));
     Fe c;
 #pragma unroll
     for (int l = 0; l < 8; ++l) c.v[l] = pk[l];

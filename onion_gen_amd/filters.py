@@ -45,13 +45,22 @@ class FilterSet:
     def match_address(self, address_bytes):
         return lib().og_filters_match_address(self.handle, bytes(address_bytes))
 
+    def describe(self):
+        """The structures the device reads, one line each (FilterSet::
+        describe_structures, src/filter.rs:1700)."""
+        n = lib().og_filters_describe(self.handle, None, 0)
+        buf = ctypes.create_string_buffer(n + 1)
+        lib().og_filters_describe(self.handle, buf, n + 1)
+        return buf.value.decode().splitlines()
+
     def scan(self, packed, sign):
-        """The device matcher of the general forms, run on the host for one
-        key: (key_hit, need, hit) as og_filters_scan_host defines them."""
+        """The device matcher of the general forms and the expressions, run on
+        the host for one key: (key_hit, need, hit) as og_filters_scan_host
+        defines them."""
         kh, nd, hit = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         if lib().og_filters_scan_host(self.handle, bytes(packed), sign, ctypes.byref(kh),
                                       ctypes.byref(nd), ctypes.byref(hit)) != 0:
-            raise OgError("the set has no general filter or does not compile")
+            raise OgError("the set has neither general filter nor expression, or does not compile")
         return bool(kh.value), bool(nd.value), bool(hit.value)
 
     def close(self):

@@ -1,0 +1,112 @@
+"""The regex: form on the device (src/expression.rs, src/filter.rs:884-1005)
+against the expression applied to every candidate's finished address.
+
+The oracle enumerates every candidate of the range (a set of all 32 one-symbol
+prefixes accepts every address) with its public key and address; Python's `re`
+decides each address.  The device must report exactly those candidates: its
+proxies may only narrow what the automata see, never what they accept.
+"""
+import hashlib
+import re
+
+import pytest
+
+from onion_gen_amd import batch, filters, gpu, key, workload
+from tests.oracle_lib import OracleFilters, search
+
+pytestmark = pytest.mark.gpu
+
+EVERY = list(workload.ALPHABET)  # one-symbol prefixes: every address matches one
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def clear_sign(pk):
+    return pk[:31] + bytes([pk[31] & 0x7F])
+
+
+def truth(start, expressions, n):
+    """(k, packed) of every k < n whose address one of the expressions matches."""
+    rx = [re.compile(e) for e in expressions]
+    every = search(start, 0, n, 256, OracleFilters(EVERY), cap=n)
+    assert len(every) == n
+    return sorted((k, clear_sign(pk)) for k, pk, addr in every
+                  if any(r.search(addr[:56]) for r in rx))
+
+
+def run_engine(start, fs, n, **shape):
+    eng = gpu.Engine.with_filters(start, fs, **shape)
+    got, done = [], 0
+    try:
+        while done < n:
+            got += [(s.offset // 8, s.packed) for s in eng.advance() if s.offset // 8 < n]
+            done += eng.batch()
+    finally:
+        eng.close()
+    return sorted(got)
+
+
+def filler_contains(seed):
+    """Eight 32-symbol substrings that fill all 256 automaton-bank bits, so an
+    expression's proxies find no room and its automaton runs on every
+    candidate.  Random, so none matches in the ranges tested."""
+    return ["contains:" + "".join(workload.ALPHABET[b & 31]
+                                  for b in hashlib.sha256(seed + bytes([i])).digest())
+            for i in range(8)]
+
+
+CASES = [
+    # (filter lines, the Python expressions that decide, n)
+    (["regex:^ab[2-7]"], ["^ab[2-7]"], 1 << 18),            # anchored proxy, key-decided
+    (["regex:zz[2-7]z"], ["zz[2-7]z"], 1 << 18),            # substring proxy into the tail
+    (["regex:^zz.*qd$"], ["^zz.*qd$"], 1 << 18),            # decided on the checksum symbols
+    (["regex:^[bcdfghjklmnp]{4}"], ["^[bcdfghjklmnp]{4}"], 1 << 16),  # class proxy, many hits
+    (["regex:(cat|dog)food|^my(shop|store)|q[2-7]{2}q"],
+     ["(cat|dog)food|^my(shop|store)|q[2-7]{2}q"], 1 << 17),
+    (["regex:^ab[2-7]", "regex:[2-7]{3}qd$"], ["^ab[2-7]", "[2-7]{3}qd$"], 1 << 17),
+    (["zz", "regex:^ab[2-7]", "suffix:[a-e]id"], ["^zz", "^ab[2-7]", "[a-e]id$"], 1 << 16),
+    (filler_contains(b"every") + ["regex:zz[2-7]z"], ["zz[2-7]z"], 1 << 17),  # no proxy
+    (["regex:\\Bq[2-7]\\B"], ["(?<=.)q[2-7](?=.)"], 1 << 16),  # looks in the middle
+]
+
+
+@pytest.mark.parametrize("lines,python,n", CASES)
+def test_engine_reports_exactly_the_matching_candidates(lines, python, n):
+    say(f"case: regex engine {lines[-1]} (+{len(lines) - 1}) n={n}")
+    start = key.start_point(hashlib.sha256(repr(lines).encode()).digest())
+    fs = filters.FilterSet.parse(lines)
+    got = run_engine(start, fs, n, threads=1024, half=128)
+    want = truth(start, python, n)
+    assert want, "the range holds no match: the case proves nothing"
+    assert got == want, (len(got), len(want))
+
+
+def test_every_mode_is_what_the_filler_case_runs():
+    fs = filters.FilterSet.parse(filler_contains(b"every") + ["regex:zz[2-7]z"])
+    assert any("every candidate" in line for line in fs.describe())
+
+
+def test_batch_runs_expressions_beside_other_searches():
+    say("case: batch 3 slots, regex searches beside a prefix search")
+    mix = [(b"r1", ["regex:zz[2-7]z"], ["zz[2-7]z"]),
+           (b"r2", ["regex:^zz.*qd$"], ["^zz.*qd$"]),
+           (b"p", ["2x"], ["^2x"])]
+    b = batch.Batch(slots=3, chains=512, half=64, rounds=1)
+    ids = {}
+    for i, (tag, lines, _) in enumerate(mix):
+        ids[tag] = b.admit(key.start_point(hashlib.sha256(tag).digest()),
+                           filters.FilterSet.parse(lines), i)
+    n = 1 << 17
+    got = {tag: [] for tag, _, _ in mix}
+    for _ in range(-(-n // b.per_step())):
+        for h in b.step():
+            tag = mix[h.tag][0]
+            assert h.search == ids[tag]
+            if h.offset // 8 < n:
+                got[tag].append((h.offset // 8, h.packed))
+    b.close()
+    for tag, _, python in mix:
+        want = truth(key.start_point(hashlib.sha256(tag).digest()), python, n)
+        assert sorted(got[tag]) == want and want, (tag, len(got[tag]), len(want))

@@ -157,7 +157,7 @@ def test_filter_matching_matches_oracle():
 
 def test_bad_filters_raise():
     for bad in ["ab0", "suffix:abc", "suffix:abd", "[", "a]", "[e-b]", "???", "wat:abc", "",
-                "regex:^ab"]:
+                "regex:^(ab", "regex:^sh0p"]:
         with pytest.raises(_lib.OgError):
             filters.FilterSet.parse(bad)
 
