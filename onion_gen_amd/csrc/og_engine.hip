@@ -267,10 +267,22 @@ double survivor_rate(const FilterSet *fs, const DevFilterHost &filt) {
       rate += f.regex->proxy_rate();
       continue;
     }
-    double p = 1;
-    for (uint32_t set : f.sets) p *= __builtin_popcount(set) / 32.0;
+    // per placement, symbol by symbol: uniform, except that symbol 55 is
+    // always 'd' and symbol 54 one of a, i, q, y (src/key.rs:309-327)
     const int len = (int)f.sets.size();
-    rate += p * (f.anchor == kAnywhere ? (double)(kAddressLen - len + 1) : 1.0);
+    const int first = f.anchor == kEnd ? kAddressLen - len : 0;
+    const int last = f.anchor == kStart ? 0 : kAddressLen - len;
+    for (int at = first; at <= last; ++at) {
+      double p = 1;
+      for (int i = 0; i < len; ++i) {
+        const uint32_t set = f.sets[i];
+        const int j = at + i;
+        if (j == 55) p *= (set >> 3) & 1u;
+        else if (j == 54) p *= __builtin_popcount(set & 0x01010101u) / 4.0;
+        else p *= __builtin_popcount(set) / 32.0;
+      }
+      rate += p;
+    }
   }
   return rate;
 }
