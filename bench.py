@@ -208,6 +208,43 @@ def dictionary_leg(threads, half, rounds):
     }
 
 
+FORMS = [
+    ("suffix:2xaqd", "a 5-symbol suffix: every candidate needs sign and checksum"),
+    ("contains:q2x7a", "a 5-symbol substring"),
+    ("regex:^ab[2-7]cd", "an anchored expression, settled by the key"),
+    ("regex:(cat|dog)food", "an unanchored expression, two obligatory sequences"),
+    ("regex:^zz.*qd$", "an expression reaching the checksum"),
+]
+
+
+def forms_leg(threads, half, rounds, launches):
+    """The other filter forms on the headline shape: `launches` timed launches
+    after one warm-up, each form in its own engine; every survivor is a
+    verified match of the form (the automata decide expressions exactly)."""
+    from onion_gen_amd import filters, gpu, key, workload
+    seed = workload.chacha_stream(0, 32)
+    start = key.start_point(seed)
+    out = {}
+    for text, what in FORMS:
+        fs = filters.FilterSet.parse(text)
+        eng = gpu.Engine.with_filters(start, fs, threads=threads, half=half, rounds=rounds)
+        batch = eng.batch()
+        eng.advance()
+        hits, kernel_ms = 0, 0.0
+        t0 = time.perf_counter()
+        for _ in range(launches):
+            hits += len(eng.advance())
+            kernel_ms += eng.last_ms()
+        wall = time.perf_counter() - t0
+        eng.close()
+        # wall includes handing every survivor to Python; the kernel rate does not
+        out[text] = {"what": what, "keys_per_s": launches * batch / wall,
+                     "kernel_keys_per_s": launches * batch / (kernel_ms * 1e-3),
+                     "kernel_avg_ms": kernel_ms / launches, "hits": hits,
+                     "candidates": launches * batch}
+    return out
+
+
 def cpu_baseline(prefix, seconds):
     """The reference's CPU engines restated in C (the oracle), on the host's
     cores, one block per thread: PairEngine at its default batch of 8192
@@ -261,6 +298,8 @@ def main():
                     help="wall-time window of each batched-mix run (0 skips the leg)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="arrival window of each served run (0 skips the leg)")
+    ap.add_argument("--forms-launches", type=int, default=4,
+                    help="timed launches per filter form in the forms leg (0 skips it)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="run only a 1 GiB device-to-device copy: the known-size kernel the "
                          "FETCH_SIZE / WRITE_SIZE counters are scaled by")
@@ -353,6 +392,8 @@ def main():
     eng.close()
     if not args.no_dict and not args.filter:
         out["dictionary"] = dictionary_leg(args.threads, args.half, args.rounds)
+    if args.forms_launches > 0 and not args.filter:
+        out["forms"] = forms_leg(args.threads, args.half, args.rounds, args.forms_launches)
     if args.mix_seconds > 0:
         out["batched"] = batched_leg(args.mix_seconds)
     if args.serve_seconds > 0:

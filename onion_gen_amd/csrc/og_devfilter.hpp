@@ -103,6 +103,10 @@ struct DevFilterHost {
   std::vector<uint32_t> regex;    // per expression: entry offset, first state
   std::vector<uint16_t> dfa;      // the expressions' automata
   bool regex_every = false;
+  // A set of key-decidable expressions only: their literal prefixes, which
+  // the key tables hold in place of the set's own (key_proxy).
+  bool key_proxy = false;
+  FilterSet proxy_set;
   // device copies
   uint32_t *d_bitmap = nullptr, *d_buckets = nullptr, *d_pats = nullptr, *d_banks = nullptr;
   uint32_t *d_regex = nullptr;
@@ -114,8 +118,47 @@ struct DevFilterHost {
            banks.size() * sizeof(ScanBank) + regex.size() * 4 + dfa.size() * 2;
   }
 
+  // The expressions' automata, in the order of fs.regexes.
+  void add_automata(const FilterSet &fs) {
+    for (uint32_t i : fs.regexes) {
+      const Regex &re = *fs.filters[i].regex;
+      regex.push_back((uint32_t)dfa.size());
+      regex.push_back(re.dfa_start());
+      dfa.insert(dfa.end(), re.dfa().begin(), re.dfa().end());
+    }
+  }
+
+  // Expressions alone, each settled by the key and guarded by few enough
+  // literal prefixes: the prefixes go to the key tables (the reference's
+  // Path::Bits, src/expression.rs:96-107).  Empty string when taken.
+  std::string compile_key_proxy(const FilterSet &fs) {
+    if (fs.regexes.empty() || !fs.fast.empty() || !fs.general.empty()) return "no";
+    if (fs.regexes.size() > (size_t)kMaxRegexes) return "no";
+    std::vector<std::string> lines;
+    for (uint32_t i : fs.regexes) {
+      const std::vector<std::string> w = fs.filters[i].regex->literal_prefixes(4096, 1.0 / 1024);
+      if (w.empty()) return "no";
+      lines.insert(lines.end(), w.begin(), w.end());
+    }
+    if (!build_filter_set(lines, 0, proxy_set).empty() || proxy_set.fast.empty() ||
+        !proxy_set.general.empty())
+      return "no";
+    const std::string e = compile(proxy_set);
+    if (!e.empty() || (mode & kModePattern)) return "no";
+    key_proxy = true;
+    add_automata(fs);
+    return "";
+  }
+
   // Empty string on success.
   std::string compile(const FilterSet &fs) {
+    if (!fs.regexes.empty() && fs.fast.empty() && fs.general.empty()) {
+      DevFilterHost k;
+      if (k.compile_key_proxy(fs).empty()) {
+        *this = std::move(k);
+        return "";
+      }
+    }
     if (fs.fast.size() == 1) {
       const Filter &f = fs.filters[fs.fast[0]];
       mode |= kModePrefix;
@@ -157,12 +200,7 @@ struct DevFilterHost {
       }
       const std::string e = build_banks(fs, banks, &regex_every);
       if (!e.empty()) return e;
-      for (uint32_t i : fs.regexes) {
-        const Regex &re = *fs.filters[i].regex;
-        regex.push_back((uint32_t)dfa.size());
-        regex.push_back(re.dfa_start());
-        dfa.insert(dfa.end(), re.dfa().begin(), re.dfa().end());
-      }
+      add_automata(fs);
     }
     if (!mode) return "the filter set is empty";
     return "";
@@ -231,6 +269,7 @@ struct DevFilterHost {
     a.pats = d_pats;
     a.nregex = (uint32_t)(regex.size() / 2);
     a.regex_every = regex_every ? 1u : 0u;
+    a.key_proxy = key_proxy ? 1u : 0u;
     a.regex = d_regex;
     a.dfa = d_dfa;
     return a;

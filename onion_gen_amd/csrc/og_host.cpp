@@ -271,12 +271,15 @@ size_t og_filters_describe(const og_filterset *s, char *out, size_t cap) {
          (fs.index_bits ? ", bitmap of 2^" + std::to_string(fs.index_bits) + " bits" : "") + "\n";
   DevFilterHost h;
   const std::string e = h.compile(fs);
-  if (!fs.general.empty() || !fs.regexes.empty()) {
+  if (!fs.general.empty() || (!fs.regexes.empty() && !h.key_proxy)) {
     t += "automaton banks: " + std::to_string(h.banks.size()) + " for " +
          std::to_string(fs.general.size()) + " general form(s)";
     if (!fs.regexes.empty()) t += " and the proxies of " + std::to_string(fs.regexes.size()) + " expression(s)";
     t += e.empty() ? "\n" : " (" + e + ")\n";
   }
+  if (h.key_proxy)
+    t += "key tables: " + std::to_string(h.proxy_set.fast.size()) +
+         " literal prefix(es) of the expressions, each hit confirmed by their automata\n";
   for (uint32_t i : fs.regexes) {
     const Regex &re = *fs.filters[i].regex;
     t += re.describe();
@@ -299,7 +302,7 @@ int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int si
   if (!s || !packed || !key_hit || !need || !hit) return -1;
   // the tables the device reads, compiled the same way
   DevFilterHost h;
-  if (!h.compile(s->set).empty() || !(h.mode & kModePattern)) return -1;
+  if (!h.compile(s->set).empty() || !(h.mode & kModePattern || h.key_proxy)) return -1;
   const ScanBank *banks = h.banks.data();
   const uint32_t nb = (uint32_t)h.banks.size(), nre = (uint32_t)(h.regex.size() / 2);
   Fe c;
@@ -310,9 +313,15 @@ int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int si
   uint32_t kw[8];
   key_words(c, kw);
   // the key stage as the search kernels run it (without the key forms)
-  bool kh, nd, maybe;
-  scan_key(kw, banks, nb, kh, nd, maybe);
-  if (nre) {
+  bool kh = false, nd = false, maybe = false;
+  if (h.key_proxy) {
+    // the expressions' literal prefixes in the key tables, then the automata
+    kh = h.proxy_set.match_key(packed) >= 0 &&
+         regex_key(kw, h.regex.data(), h.dfa.data(), nre) == 2;
+  } else {
+    scan_key(kw, banks, nb, kh, nd, maybe);
+  }
+  if (nre && !h.key_proxy) {
     maybe = (maybe || h.regex_every) && !kh;
     if (maybe) {
       const uint32_t r = regex_key(kw, h.regex.data(), h.dfa.data(), nre);

@@ -72,10 +72,14 @@ struct DevFilterArgs {
   const uint32_t *banks;          // nbanks records of kBankWords
   uint32_t npat;                  // the same patterns, for the symbol-wise check
   const uint32_t *pats;           // npat records of kPatWords
-  uint32_t nregex;                // regular expressions (kModePattern): their
-  uint32_t regex_every;           //   automata; nonzero: one has no proxy in the
-  const uint32_t *regex;          //   banks, so every candidate runs them
-  const uint16_t *dfa;            //   (records and entries: og_scan.hpp)
+  uint32_t nregex;                // regular expressions: their automata
+  uint32_t regex_every;           //   (kModePattern) nonzero: one has no proxy in
+                                  //   the banks, so every candidate runs them
+  uint32_t key_proxy;             //   (key forms) the key tables hold the
+                                  //   expressions' literal prefixes: a key hit
+                                  //   only sends the lane to the automata
+  const uint32_t *regex;          //   records and entries: og_scan.hpp
+  const uint16_t *dfa;
 };
 
 struct SearchArgs {

@@ -1301,6 +1301,44 @@ std::vector<RegexProxy> Regex::proxies(int budget) const {
   return {one};
 }
 
+std::vector<std::string> Regex::literal_prefixes(size_t limit, double max_rate) const {
+  if (!key_decidable_ || proxies_.empty()) return {};
+  size_t longest = 0;
+  for (const RegexProxy &p : proxies_) {
+    if (!p.anchored) return {};
+    longest = std::max(longest, p.sets.size());
+  }
+  for (size_t len = longest; len > 0; --len) {
+    double count = 0, rate = 0;
+    for (const RegexProxy &p : proxies_) {
+      double c = 1, q = 1;
+      for (size_t i = 0; i < std::min(len, p.sets.size()); ++i) {
+        c *= __builtin_popcount(p.sets[i]);
+        q *= __builtin_popcount(p.sets[i]) / 32.0;
+      }
+      count += c;
+      rate += q;
+    }
+    if (count > (double)limit) continue;
+    if (rate > max_rate) return {};
+    std::vector<std::string> out;
+    for (const RegexProxy &p : proxies_) {
+      std::vector<std::string> words{""};
+      for (size_t i = 0; i < std::min(len, p.sets.size()); ++i) {
+        std::vector<std::string> next;
+        for (const std::string &w : words)
+          for (uint32_t m = p.sets[i]; m; m &= m - 1) next.push_back(w + kAlphabet[__builtin_ctz(m)]);
+        words.swap(next);
+      }
+      out.insert(out.end(), words.begin(), words.end());
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
+  }
+  return {};
+}
+
 std::string Regex::describe() const {
   if (proxies_.empty()) return "regex:" + source_ + ": no proxy (" + why_none_ + "), automaton on every candidate";
   std::string t = "regex:" + source_ + ":";

@@ -77,6 +77,14 @@ class Regex {
   std::vector<RegexProxy> proxies(int budget) const;
   double proxy_rate() const { return proxy_rate_; }
 
+  // The anchored proxies of a key-decidable expression expanded into literal
+  // base32 prefixes, cut to the longest length at which there are at most
+  // `limit` of them.  Empty when the expression is not key-decidable, has no
+  // anchored proxy, or the cut prefixes would pass more than `max_rate` of
+  // the addresses.  Every match begins with one of them (the reference's
+  // Literals::Prefix, src/expression.rs:57-70).
+  std::vector<std::string> literal_prefixes(size_t limit, double max_rate) const;
+
   // One line: the proxies (classes as [..]) or why there are none.
   std::string describe() const;
 

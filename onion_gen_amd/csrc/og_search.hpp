@@ -219,6 +219,13 @@ __device__ __forceinline__ void test_key_only(const Fe10 &y, uint64_t k, const D
   if (maybe) {
     c = fe10_canon(y);
     hit = exact_key<MODE>(c, f);
+    if (f.key_proxy && hit) {
+      // an expression's prefix: its automata settle the key's 49 symbols
+      uint32_t w[8];
+      key_words(c, w);
+      hit = regex_key_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], f.regex, f.dfa,
+                           f.nregex) == 2;
+    }
   }
   report(hit, k, c, sink);
 }

@@ -59,7 +59,8 @@ def filler_contains(seed):
 
 CASES = [
     # (filter lines, the Python expressions that decide, n)
-    (["regex:^ab[2-7]"], ["^ab[2-7]"], 1 << 18),            # anchored proxy, key-decided
+    (["regex:^ab[2-7]"], ["^ab[2-7]"], 1 << 18),            # literal prefixes in the key tables
+    (["regex:^a[2-7]b", "regex:^(cat|dog)"], ["^a[2-7]b", "^(cat|dog)"], 1 << 18),
     (["regex:zz[2-7]z"], ["zz[2-7]z"], 1 << 18),            # substring proxy into the tail
     (["regex:^zz.*qd$"], ["^zz.*qd$"], 1 << 18),            # decided on the checksum symbols
     (["regex:^[bcdfghjklmnp]{4}"], ["^[bcdfghjklmnp]{4}"], 1 << 16),  # class proxy, many hits
@@ -83,17 +84,23 @@ def test_engine_reports_exactly_the_matching_candidates(lines, python, n):
     assert got == want, (len(got), len(want))
 
 
-def test_every_mode_is_what_the_filler_case_runs():
-    fs = filters.FilterSet.parse(filler_contains(b"every") + ["regex:zz[2-7]z"])
-    assert any("every candidate" in line for line in fs.describe())
+def test_each_case_takes_the_path_it_is_meant_to():
+    def lines_of(lines):
+        return filters.FilterSet.parse(lines).describe()
+    assert any("key tables" in l for l in lines_of(["regex:^ab[2-7]"]))
+    assert any("key tables" in l for l in lines_of(["regex:^a[2-7]b", "regex:^(cat|dog)"]))
+    assert any("automaton banks" in l for l in lines_of(["regex:^[bcdfghjklmnp]{4}"]))
+    assert any("every candidate" in l
+               for l in lines_of(filler_contains(b"every") + ["regex:zz[2-7]z"]))
 
 
 def test_batch_runs_expressions_beside_other_searches():
-    say("case: batch 3 slots, regex searches beside a prefix search")
+    say("case: batch 4 slots, regex searches beside a prefix search")
     mix = [(b"r1", ["regex:zz[2-7]z"], ["zz[2-7]z"]),
            (b"r2", ["regex:^zz.*qd$"], ["^zz.*qd$"]),
-           (b"p", ["2x"], ["^2x"])]
-    b = batch.Batch(slots=3, chains=512, half=64, rounds=1)
+           (b"p", ["2x"], ["^2x"]),
+           (b"r3", ["regex:^a[2-7]c"], ["^a[2-7]c"])]
+    b = batch.Batch(slots=4, chains=512, half=64, rounds=1)
     ids = {}
     for i, (tag, lines, _) in enumerate(mix):
         ids[tag] = b.admit(key.start_point(hashlib.sha256(tag).digest()),
