@@ -214,30 +214,56 @@ FORMS = [
     ("regex:^ab[2-7]cd", "an anchored expression, settled by the key"),
     ("regex:(cat|dog)food", "an unanchored expression, two obligatory sequences"),
     ("regex:^zz.*qd$", "an expression reaching the checksum"),
+    ("dict:contains8", "1,000 eight-symbol substrings (Python random, seed 5): one automaton"),
+    ("dict:contains5", "1,000 five-symbol substrings (Python random, seed 5): one automaton, "
+                       "most candidates hashed (a match can still end in the checksum)"),
 ]
+
+
+def form_lines(text):
+    """The filter lines of a forms-leg entry."""
+    if not text.startswith("dict:"):
+        return [text]
+    import random
+    from onion_gen_amd import workload
+    rng = random.Random(5)
+    length = int(text[len("dict:contains"):])
+    return ["contains:" + "".join(rng.choice(workload.ALPHABET) for _ in range(length))
+            for _ in range(1000)]
 
 
 def forms_leg(threads, half, rounds, launches):
     """The other filter forms on the headline shape: `launches` timed launches
     after one warm-up, each form in its own engine; every survivor is a
     verified match of the form (the automata decide expressions exactly)."""
+    import ctypes
     from onion_gen_amd import filters, gpu, key, workload
+    from onion_gen_amd._lib import OgError, err_buf, lib
+    from onion_gen_amd.gpu import _Survivor
     seed = workload.chacha_stream(0, 32)
     start = key.start_point(seed)
     out = {}
+    e = err_buf()
+    cnt = ctypes.c_uint32()
     for text, what in FORMS:
-        fs = filters.FilterSet.parse(text)
+        fs = filters.FilterSet.parse(form_lines(text))
         eng = gpu.Engine.with_filters(start, fs, threads=threads, half=half, rounds=rounds)
-        batch = eng.batch()
-        eng.advance()
+        batch, room = eng.batch(), eng.room()
+        buf = (_Survivor * room)()  # survivors stay in one ctypes buffer
+
+        def advance():
+            if lib().og_hip_engine_advance(eng._h, buf, room, ctypes.byref(cnt), e, len(e)) != 0:
+                raise OgError(e.value.decode())
+            return cnt.value
+        advance()
         hits, kernel_ms = 0, 0.0
         t0 = time.perf_counter()
         for _ in range(launches):
-            hits += len(eng.advance())
+            hits += advance()
             kernel_ms += eng.last_ms()
         wall = time.perf_counter() - t0
         eng.close()
-        # wall includes handing every survivor to Python; the kernel rate does not
+        # wall includes copying the survivors back; the kernel rate does not
         out[text] = {"what": what, "keys_per_s": launches * batch / wall,
                      "kernel_keys_per_s": launches * batch / (kernel_ms * 1e-3),
                      "kernel_avg_ms": kernel_ms / launches, "hits": hits,

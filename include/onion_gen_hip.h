@@ -255,6 +255,14 @@ size_t og_filters_describe(const og_filterset *s, char *out, size_t cap);
  * compile. */
 int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int sign,
                          int *key_hit, int *need, int *hit);
+/* The key stage of og_filters_scan_host for the two candidates of a pair at
+ * once, as the search kernels run it (one interleaved walk of the automata
+ * for both): key_hit[i] and need[i] as above.  No counterpart in the
+ * reference; it exists so that the pair form is checked against the single
+ * one.  Returns 0, or -1 as og_filters_scan_host (and for a set without
+ * general filters or expressions in the automaton banks). */
+int og_filters_scan_pair_host(const og_filterset *s, const uint8_t a[32], const uint8_t b[32],
+                              int key_hit[2], int need[2]);
 
 /* ------------------------------------------------------------------ batch */
 

@@ -20,7 +20,7 @@ namespace og {
 // none, and `every` says so: its automaton then runs on every candidate.
 // Empty string on success.
 inline std::string build_banks(const FilterSet &fs, std::vector<ScanBank> &out,
-                               bool *every = nullptr) {
+                               bool *every = nullptr, bool general = true) {
   out.clear();
   std::vector<int> used;
   // 0: placed, 1: no room
@@ -54,7 +54,7 @@ inline std::string build_banks(const FilterSet &fs, std::vector<ScanBank> &out,
     b.jstart = std::min(b.jstart, js);
     return 0;
   };
-  for (uint32_t idx : fs.general) {
+  for (uint32_t idx : general ? fs.general : std::vector<uint32_t>()) {
     const Filter &f = fs.filters[idx];
     if (place(f.sets, f.anchor, false))
       return "the general filters need more than 8 automaton banks";
@@ -100,32 +100,69 @@ struct DevFilterHost {
   std::vector<uint64_t> keys;
   std::vector<uint32_t> pats;     // kPatWords per pattern
   std::vector<ScanBank> banks;    // the automaton over the same patterns
-  std::vector<uint32_t> regex;    // per expression: entry offset, first state
+  std::vector<uint32_t> regex;    // per expression: entry offset, first state, live offset
   std::vector<uint16_t> dfa;      // the expressions' automata
+  std::vector<uint32_t> live;     // their tail-liveness bits
   bool regex_every = false;
+  bool proxyless = false;         // an expression found no room for its proxies
+  // General filters beyond the banks, compiled into one automaton (a
+  // dictionary of substrings and suffixes): run on every candidate.
+  std::shared_ptr<const Regex> dictionary;
+  size_t dictionary_forms = 0;
   // A set of key-decidable expressions only: their literal prefixes, which
   // the key tables hold in place of the set's own (key_proxy).
   bool key_proxy = false;
   FilterSet proxy_set;
   // device copies
   uint32_t *d_bitmap = nullptr, *d_buckets = nullptr, *d_pats = nullptr, *d_banks = nullptr;
-  uint32_t *d_regex = nullptr;
+  uint32_t *d_regex = nullptr, *d_live = nullptr;
   uint16_t *d_dfa = nullptr;
   uint64_t *d_keys = nullptr;
 
   size_t device_bytes() const {
     return bitmap.size() * 4 + buckets.size() * 4 + keys.size() * 8 + pats.size() * 4 +
-           banks.size() * sizeof(ScanBank) + regex.size() * 4 + dfa.size() * 2;
+           banks.size() * sizeof(ScanBank) + regex.size() * 4 + dfa.size() * 2 + live.size() * 4;
+  }
+
+  void add_automaton(const Regex &re) {
+    regex.push_back((uint32_t)dfa.size());
+    regex.push_back(re.dfa_start());
+    regex.push_back((uint32_t)live.size());
+    dfa.insert(dfa.end(), re.dfa().begin(), re.dfa().end());
+    live.insert(live.end(), re.tail_live().begin(), re.tail_live().end());
   }
 
   // The expressions' automata, in the order of fs.regexes.
   void add_automata(const FilterSet &fs) {
-    for (uint32_t i : fs.regexes) {
-      const Regex &re = *fs.filters[i].regex;
-      regex.push_back((uint32_t)dfa.size());
-      regex.push_back(re.dfa_start());
-      dfa.insert(dfa.end(), re.dfa().begin(), re.dfa().end());
+    for (uint32_t i : fs.regexes) add_automaton(*fs.filters[i].regex);
+  }
+
+  // The general filters as one expression: an alternation of their symbol
+  // sets, anchored as their forms are (the reference's flat substring
+  // automaton, src/filter.rs:1124-1207, here over the whole address with the
+  // suffixes and class prefixes in the same automaton).
+  static std::string dictionary_source(const FilterSet &fs) {
+    static const char kSym[] = "abcdefghijklmnopqrstuvwxyz234567";
+    std::string src;
+    for (uint32_t idx : fs.general) {
+      const Filter &f = fs.filters[idx];
+      if (!src.empty()) src += '|';
+      if (f.anchor == kStart) src += '^';
+      for (uint32_t set : f.sets) {
+        if (set == 0xffffffffu) {
+          src += '.';
+        } else if (__builtin_popcount(set) == 1) {
+          src += kSym[__builtin_ctz(set)];
+        } else {
+          src += '[';
+          for (int c = 0; c < 32; ++c)
+            if ((set >> c) & 1u) src += kSym[c];
+          src += ']';
+        }
+      }
+      if (f.anchor == kEnd) src += '$';
     }
+    return src;
   }
 
   // Expressions alone, each settled by the key and guarded by few enough
@@ -185,22 +222,36 @@ struct DevFilterHost {
       }
     }
     if (!fs.general.empty() || !fs.regexes.empty()) {
-      if (fs.general.size() > (size_t)kMaxPatterns)
-        return "at most 64 general (suffix, contains, class) filters per search";
       if (fs.regexes.size() > (size_t)kMaxRegexes)
         return "at most 16 regex: filters per search";
       mode |= kModePattern;
-      for (uint32_t i : fs.general) {
-        DevPattern p = dev_pattern(fs.filters[i]);
-        pats.push_back(p.len);
-        pats.push_back(p.first);
-        pats.push_back(p.last);
-        pats.push_back(p.key_last);
-        for (int s = 0; s < kAddressLen; ++s) pats.push_back(p.sets[s]);
+      bool dict = fs.general.size() > (size_t)kMaxPatterns;
+      if (!dict && !build_banks(fs, banks, &proxyless).empty()) dict = true;
+      if (dict) {
+        // too many for the banks: one automaton over every general filter
+        RegexLimits lim;
+        lim.program = 4000000;
+        lim.states = kRegexMaxStates;
+        lim.proxies = false;
+        const std::string e = Regex::compile(dictionary_source(fs), dictionary, lim);
+        if (!e.empty()) return "the general filters as one automaton: " + e;
+        dictionary_forms = fs.general.size();
+        banks.clear();
+        const std::string eb = build_banks(fs, banks, &proxyless, false);
+        if (!eb.empty()) return eb;
+        add_automaton(*dictionary);
+      } else {
+        for (uint32_t i : fs.general) {
+          DevPattern p = dev_pattern(fs.filters[i]);
+          pats.push_back(p.len);
+          pats.push_back(p.first);
+          pats.push_back(p.last);
+          pats.push_back(p.key_last);
+          for (int s = 0; s < kAddressLen; ++s) pats.push_back(p.sets[s]);
+        }
       }
-      const std::string e = build_banks(fs, banks, &regex_every);
-      if (!e.empty()) return e;
       add_automata(fs);
+      regex_every = proxyless || dictionary != nullptr;
     }
     if (!mode) return "the filter set is empty";
     return "";
@@ -229,6 +280,7 @@ struct DevFilterHost {
     if (e == hipSuccess) e = upload(&d_pats, pats, s);
     if (e == hipSuccess) e = upload(&d_regex, regex, s);
     if (e == hipSuccess) e = upload(&d_dfa, dfa, s);
+    if (e == hipSuccess) e = upload(&d_live, live, s);
     if (e == hipSuccess && !banks.empty()) {
       std::vector<uint32_t> words(banks.size() * kBankWords);
       std::memcpy(words.data(), banks.data(), words.size() * 4);
@@ -247,7 +299,8 @@ struct DevFilterHost {
     (void)hipFree(d_banks);
     (void)hipFree(d_regex);
     (void)hipFree(d_dfa);
-    d_bitmap = d_buckets = d_pats = d_banks = d_regex = nullptr;
+    (void)hipFree(d_live);
+    d_bitmap = d_buckets = d_pats = d_banks = d_regex = d_live = nullptr;
     d_dfa = nullptr;
     d_keys = nullptr;
   }
@@ -267,7 +320,8 @@ struct DevFilterHost {
     a.banks = d_banks;
     a.npat = (uint32_t)(pats.size() / kPatWords);
     a.pats = d_pats;
-    a.nregex = (uint32_t)(regex.size() / 2);
+    a.nregex = (uint32_t)(regex.size() / 3);
+    a.regex_live = d_live;
     a.regex_every = regex_every ? 1u : 0u;
     a.key_proxy = key_proxy ? 1u : 0u;
     a.regex = d_regex;

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "fe10.hpp"
@@ -14,6 +15,15 @@
 
 struct og_filterset {
   og::FilterSet set;
+  // the device tables, compiled once on the first host-side use
+  std::once_flag once;
+  og::DevFilterHost tables;
+  std::string tables_err;
+
+  const og::DevFilterHost &host_tables() {
+    std::call_once(once, [this] { tables_err = tables.compile(set); });
+    return tables;
+  }
 };
 
 namespace og {
@@ -262,6 +272,41 @@ int og_filters_match_address(const og_filterset *s, const uint8_t address[35]) {
   return s ? s->set.match_address(address) : -1;
 }
 
+int og_filters_scan_pair_host(const og_filterset *s, const uint8_t a[32], const uint8_t b[32],
+                              int key_hit[2], int need[2]) {
+  if (!s || !a || !b || !key_hit || !need) return -1;
+  const DevFilterHost &h = const_cast<og_filterset *>(s)->host_tables();
+  if (!s->tables_err.empty() || !(h.mode & kModePattern)) return -1;
+  const uint8_t *keys[2] = {a, b};
+  uint32_t kw[2][8];
+  bool kh[2], nd[2], maybe[2];
+  for (int i = 0; i < 2; ++i) {
+    Fe c;
+    for (int l = 0; l < 8; ++l)
+      c.v[l] = (uint32_t)keys[i][4 * l] | ((uint32_t)keys[i][4 * l + 1] << 8) |
+               ((uint32_t)keys[i][4 * l + 2] << 16) | ((uint32_t)keys[i][4 * l + 3] << 24);
+    key_words(c, kw[i]);
+    scan_key(kw[i], h.banks.data(), (uint32_t)h.banks.size(), kh[i], nd[i], maybe[i]);
+  }
+  const uint32_t nre = (uint32_t)(h.regex.size() / 3);
+  if (nre) {
+    for (int i = 0; i < 2; ++i) maybe[i] = (maybe[i] || h.regex_every) && !kh[i];
+    if (maybe[0] || maybe[1]) {
+      const uint32_t r = regex_key2(kw[0], kw[1], h.regex.data(), h.dfa.data(), h.live.data(), nre);
+      const uint32_t ri[2] = {maybe[0] ? (r & 3u) : 0u, maybe[1] ? (r >> 2) : 0u};
+      for (int i = 0; i < 2; ++i) {
+        kh[i] = kh[i] || ri[i] == 2;
+        nd[i] = nd[i] || ri[i] == 1;
+      }
+    }
+  }
+  for (int i = 0; i < 2; ++i) {
+    key_hit[i] = kh[i];
+    need[i] = nd[i] && !kh[i];
+  }
+  return 0;
+}
+
 size_t og_filters_describe(const og_filterset *s, char *out, size_t cap) {
   if (!s) return 0;
   const FilterSet &fs = s->set;
@@ -269,11 +314,16 @@ size_t og_filters_describe(const og_filterset *s, char *out, size_t cap) {
   if (!fs.fast.empty())
     t += "prefix table: " + std::to_string(fs.fast.size()) + " literal prefix(es)" +
          (fs.index_bits ? ", bitmap of 2^" + std::to_string(fs.index_bits) + " bits" : "") + "\n";
-  DevFilterHost h;
-  const std::string e = h.compile(fs);
-  if (!fs.general.empty() || (!fs.regexes.empty() && !h.key_proxy)) {
+  const DevFilterHost &h = const_cast<og_filterset *>(s)->host_tables();
+  const std::string &e = s->tables_err;
+  if (h.dictionary)
+    t += "dictionary automaton: " + std::to_string(h.dictionary->dfa_states()) + " states over " +
+         std::to_string(h.dictionary_forms) +
+         " general form(s), run on every candidate; the checksum only where a match can still "
+         "end in it\n";
+  if ((!fs.general.empty() && !h.dictionary) || (!fs.regexes.empty() && !h.key_proxy)) {
     t += "automaton banks: " + std::to_string(h.banks.size()) + " for " +
-         std::to_string(fs.general.size()) + " general form(s)";
+         std::to_string(h.dictionary ? 0 : fs.general.size()) + " general form(s)";
     if (!fs.regexes.empty()) t += " and the proxies of " + std::to_string(fs.regexes.size()) + " expression(s)";
     t += e.empty() ? "\n" : " (" + e + ")\n";
   }
@@ -287,7 +337,7 @@ size_t og_filters_describe(const og_filterset *s, char *out, size_t cap) {
                             : "; an open candidate costs a SHA3-256";
     t += "\n";
   }
-  if (h.regex_every)
+  if (h.proxyless)
     t += "an expression has no proxy in the banks: its automaton runs on every candidate\n";
   if (out && cap) {
     const size_t n = std::min(cap - 1, t.size());
@@ -301,10 +351,10 @@ int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int si
                          int *key_hit, int *need, int *hit) {
   if (!s || !packed || !key_hit || !need || !hit) return -1;
   // the tables the device reads, compiled the same way
-  DevFilterHost h;
-  if (!h.compile(s->set).empty() || !(h.mode & kModePattern || h.key_proxy)) return -1;
+  const DevFilterHost &h = const_cast<og_filterset *>(s)->host_tables();
+  if (!s->tables_err.empty() || !(h.mode & kModePattern || h.key_proxy)) return -1;
   const ScanBank *banks = h.banks.data();
-  const uint32_t nb = (uint32_t)h.banks.size(), nre = (uint32_t)(h.regex.size() / 2);
+  const uint32_t nb = (uint32_t)h.banks.size(), nre = (uint32_t)(h.regex.size() / 3);
   Fe c;
   for (int l = 0; l < 8; ++l) {
     c.v[l] = (uint32_t)packed[4 * l] | ((uint32_t)packed[4 * l + 1] << 8) |
@@ -317,14 +367,14 @@ int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int si
   if (h.key_proxy) {
     // the expressions' literal prefixes in the key tables, then the automata
     kh = h.proxy_set.match_key(packed) >= 0 &&
-         regex_key(kw, h.regex.data(), h.dfa.data(), nre) == 2;
+         regex_key(kw, h.regex.data(), h.dfa.data(), h.live.data(), nre) == 2;
   } else {
     scan_key(kw, banks, nb, kh, nd, maybe);
   }
   if (nre && !h.key_proxy) {
     maybe = (maybe || h.regex_every) && !kh;
     if (maybe) {
-      const uint32_t r = regex_key(kw, h.regex.data(), h.dfa.data(), nre);
+      const uint32_t r = regex_key(kw, h.regex.data(), h.dfa.data(), h.live.data(), nre);
       kh |= r == 2;
       nd |= r == 1;
     }

@@ -78,8 +78,9 @@ struct DevFilterArgs {
   uint32_t key_proxy;             //   (key forms) the key tables hold the
                                   //   expressions' literal prefixes: a key hit
                                   //   only sends the lane to the automata
-  const uint32_t *regex;          //   records and entries: og_scan.hpp
-  const uint16_t *dfa;
+  const uint32_t *regex;          //   records, entries and tail-liveness
+  const uint16_t *dfa;            //   bits: og_scan.hpp
+  const uint32_t *regex_live;
 };
 
 struct SearchArgs {

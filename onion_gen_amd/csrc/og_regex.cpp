@@ -14,7 +14,9 @@
 #include <bitset>
 #include <cctype>
 #include <cstring>
+#include <iterator>
 #include <map>
+#include <unordered_map>
 
 namespace og {
 
@@ -22,7 +24,6 @@ namespace {
 
 constexpr int kAddrLen = 56;
 constexpr int kKeySymbols = 49;
-constexpr size_t kMaxProgram = 20000;  // NFA instructions
 constexpr size_t kMaxLits = 64;        // class sequences the extraction keeps
 constexpr size_t kMaxLitLen = 32;      // one automaton bank
 constexpr uint32_t kAll = 0xffffffffu;
@@ -962,8 +963,9 @@ struct RegexBuilder {
   std::vector<Node> nodes;
   std::vector<Regex::Inst> &prog;
   std::string err;
+  size_t max_program;
 
-  explicit RegexBuilder(std::vector<Regex::Inst> &p) : prog(p) {}
+  RegexBuilder(std::vector<Regex::Inst> &p, size_t max) : prog(p), max_program(max) {}
 
   size_t push(uint8_t op) {
     prog.push_back(Regex::Inst());
@@ -973,9 +975,9 @@ struct RegexBuilder {
 
   void emit(int i) {
     if (!err.empty()) return;
-    if (prog.size() > kMaxProgram) {
-      err = "the expression is too large (its automaton exceeds " +
-            std::to_string(kMaxProgram) + " instructions)";
+    if (prog.size() > max_program) {
+      err = "the expression is too large (its program exceeds " + std::to_string(max_program) +
+            " instructions)";
       return;
     }
     const Node &x = nodes[i];
@@ -1104,39 +1106,57 @@ bool Regex::find(const char *text, int n) const {
 
 namespace {
 
+// Hash of a sorted thread set, for interning automaton states.
+struct SetHash {
+  size_t operator()(const std::vector<uint32_t> &v) const {
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (uint32_t x : v) h = (h ^ x) * 0x100000001b3ull;
+    return (size_t)(h ^ (h >> 29));
+  }
+};
+
 // The device automaton (see the header).  Raw states are the sets of threads
 // pending at a position; state 0 of the raw numbering is the one at symbol 0,
-// the only position where ^ holds.
-bool build_dfa(const std::vector<Regex::Inst> &prog, std::vector<uint16_t> &table,
-               uint32_t &start, std::string &err) {
+// the only position where ^ holds.  A match may start at every position, so
+// every state's closure includes the program start's; that part is the same
+// for every state past symbol 0 and is computed once (for a dictionary of
+// thousands of filters it is nearly all of the closure).
+bool build_dfa(const std::vector<Regex::Inst> &prog, uint32_t max_states,
+               std::vector<uint16_t> &table, uint32_t &start, std::string &err) {
   Closer c(prog);
-  std::map<std::vector<uint32_t>, uint32_t> ids;
+  std::unordered_map<std::vector<uint32_t>, uint32_t, SetHash> ids;
   std::vector<std::vector<uint32_t>> pend;
   std::vector<uint8_t> mid, end;  // a match ends here (not the last position) / at symbol 56
   std::vector<std::vector<uint32_t>> next;
-  std::vector<uint32_t> chars, seeds;
+  std::vector<uint32_t> chars;
+  const std::vector<uint32_t> zero{0};
 
-  auto accepts = [&](const std::vector<uint32_t> &p, bool initial, bool at_end) {
-    seeds = p;
-    seeds.push_back(0);
-    // all symbols are word characters: prev holds past 0, next before 56
-    return c.run(seeds, initial, at_end, !initial, !at_end, chars);
-  };
+  // all symbols are word characters: prev holds past 0, next before 56
+  const bool start_mid = c.run(zero, false, false, true, true, chars);
+  std::vector<uint32_t> start_targets[32];
+  for (uint32_t pc : chars)
+    for (uint32_t m = prog[pc].mask; m; m &= m - 1) start_targets[__builtin_ctz(m)].push_back(pc + 1);
+  for (auto &t : start_targets) {
+    std::sort(t.begin(), t.end());
+    t.erase(std::unique(t.begin(), t.end()), t.end());
+  }
+  const bool start_end = c.run(zero, false, true, true, false, chars);
+
   auto intern = [&](const std::vector<uint32_t> &p) -> uint32_t {
     auto it = ids.find(p);
     if (it != ids.end()) return it->second;
     const uint32_t id = (uint32_t)pend.size();
     ids.emplace(p, id);
     pend.push_back(p);
-    mid.push_back(accepts(p, false, false));
-    end.push_back(accepts(p, false, true));
+    mid.push_back(start_mid || c.run(p, false, false, true, true, chars));
+    end.push_back(start_end || c.run(p, false, true, true, false, chars));
     next.emplace_back();
     return id;
   };
   // the state at symbol 0 is not interned: its empty thread set also names
   // the state "nothing pending" at later symbols
   pend.push_back({});
-  mid.push_back(accepts({}, true, false));
+  mid.push_back(c.run(zero, true, false, false, true, chars));
   end.push_back(0);
   next.emplace_back();
   if (mid[0]) {
@@ -1146,24 +1166,33 @@ bool build_dfa(const std::vector<Regex::Inst> &prog, std::vector<uint16_t> &tabl
     start = kRegexMatched;
     return true;
   }
+  std::vector<uint32_t> own[32], merged;
   for (uint32_t u = 0; u < pend.size(); ++u) {
     if (u > 0 && mid[u]) continue;  // matched: absorbing, not expanded
-    if (pend.size() > kRegexMaxStates) {
-      err = "the expression needs more than " + std::to_string(kRegexMaxStates) +
-            " automaton states";
+    if (pend.size() > max_states) {
+      err = "the expression needs more than " + std::to_string(max_states) + " automaton states";
       return false;
     }
-    seeds = pend[u];
-    seeds.push_back(0);
-    c.run(seeds, u == 0, false, u != 0, true, chars);
-    std::vector<uint32_t> targets[32];
+    for (auto &t : own) t.clear();
+    if (u == 0) {
+      c.run(zero, true, false, false, true, chars);
+    } else {
+      c.run(pend[u], false, false, true, true, chars);
+    }
     for (uint32_t pc : chars)
-      for (uint32_t m = prog[pc].mask; m; m &= m - 1) targets[__builtin_ctz(m)].push_back(pc + 1);
+      for (uint32_t m = prog[pc].mask; m; m &= m - 1) own[__builtin_ctz(m)].push_back(pc + 1);
     next[u].resize(32);
     for (int s = 0; s < 32; ++s) {
-      std::sort(targets[s].begin(), targets[s].end());
-      targets[s].erase(std::unique(targets[s].begin(), targets[s].end()), targets[s].end());
-      const uint32_t v = intern(targets[s]);  // may grow `next`
+      std::sort(own[s].begin(), own[s].end());
+      merged.clear();
+      if (u == 0) {
+        merged = own[s];
+      } else {
+        std::set_union(own[s].begin(), own[s].end(), start_targets[s].begin(),
+                       start_targets[s].end(), std::back_inserter(merged));
+      }
+      merged.erase(std::unique(merged.begin(), merged.end()), merged.end());
+      const uint32_t v = intern(merged);  // may grow `next`
       next[u][s] = v;
     }
   }
@@ -1207,12 +1236,51 @@ bool build_dfa(const std::vector<Regex::Inst> &prog, std::vector<uint16_t> &tabl
   return true;
 }
 
+// The tail-liveness bits of the header, by dynamic programming backwards over
+// symbols 55..51 of the numbered automaton.
+std::vector<uint32_t> tail_liveness(const std::vector<uint16_t> &t) {
+  const uint32_t n = (uint32_t)(t.size() / 32);
+  const uint32_t aiqy = 0x01010101u, d = 1u << 3;
+  std::vector<uint8_t> acc(n), nxt(n);
+  // after symbol 54: symbol 55 is 'd', and the end decides
+  for (uint32_t u = 0; u < n; ++u) acc[u] = (t[(size_t)u * 32 + 3] & kRegexEndBit) != 0;
+  (void)d;
+  auto step = [&](uint32_t set) {
+    for (uint32_t u = 0; u < n; ++u) {
+      uint8_t a = 0;
+      for (uint32_t m = set; m && !a; m &= m - 1) {
+        const uint32_t v = t[(size_t)u * 32 + __builtin_ctz(m)] & 0x7fffu;
+        a = v == kRegexMatched || acc[v];
+      }
+      nxt[u] = a;
+    }
+    acc.swap(nxt);
+  };
+  step(aiqy);         // symbol 54
+  step(0xffffffffu);  // symbol 53
+  step(0xffffffffu);  // symbol 52
+  std::vector<uint32_t> live((2 * (size_t)n + 31) / 32, 0);
+  for (uint32_t b = 0; b < 2; ++b) {
+    const uint32_t set = b ? 0xffff0000u : 0x0000ffffu;  // symbol 51 by its top bit
+    for (uint32_t u = 0; u < n; ++u) {
+      bool a = false;
+      for (uint32_t m = set; m && !a; m &= m - 1) {
+        const uint32_t v = t[(size_t)u * 32 + __builtin_ctz(m)] & 0x7fffu;
+        a = v == kRegexMatched || acc[v];
+      }
+      if (a) live[(2 * u + b) >> 5] |= 1u << ((2 * u + b) & 31);
+    }
+  }
+  return live;
+}
+
 }  // namespace
 
-std::string Regex::compile(const std::string &source, std::shared_ptr<const Regex> &out) {
+std::string Regex::compile(const std::string &source, std::shared_ptr<const Regex> &out,
+                           const RegexLimits &limits) {
   auto r = std::make_shared<Regex>();
   r->source_ = source;
-  RegexBuilder b(r->prog_);
+  RegexBuilder b(r->prog_, limits.program);
   std::string perr;
   Parser p(source, b.nodes);
   const int root = p.parse(perr);
@@ -1239,9 +1307,15 @@ std::string Regex::compile(const std::string &source, std::shared_ptr<const Rege
   r->key_decidable_ = r->anchored_ && only_begin_looks(b.nodes, root) && longest >= 0 &&
                       longest <= kKeySymbols;
   std::string derr;
-  if (!build_dfa(r->prog_, r->dfa_, r->dfa_start_, derr))
+  if (!build_dfa(r->prog_, std::min(limits.states, kRegexMaxStates), r->dfa_, r->dfa_start_, derr))
     return "the expression is too complex for the device: " + derr;
   if (r->dfa_start_ == kRegexDead) return unreachable + "it accepts no address";
+  r->live_ = tail_liveness(r->dfa_);
+  if (!limits.proxies) {
+    r->why_none_ = "not extracted";
+    out = r;
+    return "";
+  }
 
   // proxies: the whole expression at symbol 0 when it is anchored, and any
   // suffix of its top-level sequence anywhere (a match of a.b contains a match

@@ -46,14 +46,24 @@ struct RegexProxy {
 constexpr uint32_t kRegexDead = 0;
 constexpr uint32_t kRegexMatched = 1;
 constexpr uint32_t kRegexEndBit = 0x8000u;
-constexpr uint32_t kRegexMaxStates = 4096;
+constexpr uint32_t kRegexMaxStates = 32767;
+
+// Size limits of one compiled expression.  The defaults suit an expression
+// typed by a user; a dictionary of general filters compiled into one
+// automaton (og_devfilter.hpp) raises them and skips the proxies.
+struct RegexLimits {
+  size_t program = 20000;  // program instructions
+  uint32_t states = 4096;  // automaton states
+  bool proxies = true;
+};
 
 class Regex {
  public:
   // Empty string on success, else the message (prefixed like the reference's
   // ExpressionError: "the regular expression does not parse: ..." or "no
   // address can ever match this expression: ...").
-  static std::string compile(const std::string &source, std::shared_ptr<const Regex> &out);
+  static std::string compile(const std::string &source, std::shared_ptr<const Regex> &out,
+                             const RegexLimits &limits = RegexLimits());
 
   // Whether the expression matches anywhere in `text` (any bytes).
   bool find(const char *text, int n) const;
@@ -69,6 +79,11 @@ class Regex {
   const std::vector<uint16_t> &dfa() const { return dfa_; }
   uint32_t dfa_start() const { return dfa_start_; }
   uint32_t dfa_states() const { return (uint32_t)(dfa_.size() / 32); }
+  // Bit 2s+b: from state s after symbol 50, a match can still be completed
+  // over symbols 51..55 when the top bit of symbol 51 is b (symbols 52 and 53
+  // free, 54 one of a/i/q/y, 55 'd').  The key stage asks it so that only a
+  // candidate whose match can still reach the checksum pays for it.
+  const std::vector<uint32_t> &tail_live() const { return live_; }
 
   // The proxies fitted into `budget` automaton-bank bits (shortened, then
   // merged, as the reference shortens literals past its limit), and the share
@@ -103,6 +118,7 @@ class Regex {
   bool key_decidable_ = false;
   std::vector<uint16_t> dfa_;
   uint32_t dfa_start_ = 0;
+  std::vector<uint32_t> live_;
   std::vector<RegexProxy> proxies_;
   double proxy_rate_ = 1.0;
   std::string why_none_;

@@ -169,30 +169,71 @@ OG_HD void address_words(const uint32_t kw[8], uint32_t sign, uint32_t chk, uint
 // Regular expressions (og_regex.hpp): one deterministic automaton each, as
 // [states][32] 16-bit entries: the next state, and in bit 15 whether the
 // address may end there.  State 0 can no longer match, state 1 has matched.
-// `recs` holds two words per expression: its entries' offset and first state.
+// `recs` holds three words per expression: its entries' offset, its first
+// state, and the word offset of its tail-liveness bits in `live` (bit 2s+b:
+// from state s after symbol 50 a match can still end by symbol 55 when the
+// top bit of symbol 51 is b).
+
+// From state s after symbol 48 of an open expression: whether a match can
+// still end in symbols 49..55.  Symbol 49 takes both values of the deferred
+// sign bit, symbol 50 is exact, and the tail-liveness bit covers the rest.
+OG_HD bool regex_open(const uint16_t *t, const uint32_t *lv, uint32_t s, const uint32_t w[8]) {
+  const uint32_t t49 = sym_at(w, 49), t50 = sym_at(w, 50), y248 = w[7] & 1u;
+  for (uint32_t sign = 0; sign < 4; sign += 2) {
+    const uint32_t e = t[s * 32u + (t49 | sign)] & 0x7fffu;
+    if (e == 1) return true;
+    const uint32_t e2 = t[e * 32u + t50] & 0x7fffu;
+    const uint32_t bit = 2 * e2 + y248;
+    if (e2 == 1 || ((lv[bit >> 5] >> (bit & 31)) & 1u)) return true;
+  }
+  return false;
+}
 
 // Symbols 0..48 (what the key settles) through every expression: 2 when one
-// has matched, 1 when one is still open, 0 when all are closed.
+// has matched, 1 when one may still match on the address, 0 otherwise.
 OG_HD uint32_t regex_key(const uint32_t w[8], const uint32_t *recs, const uint16_t *dfa,
-                         uint32_t n) {
+                         const uint32_t *live, uint32_t n) {
   uint32_t out = 0;
   for (uint32_t r = 0; r < n; ++r) {
-    const uint16_t *t = dfa + recs[2 * r];
-    uint32_t s = recs[2 * r + 1];
+    const uint16_t *t = dfa + recs[3 * r];
+    uint32_t s = recs[3 * r + 1];
 #pragma unroll
     for (int j = 0; j < 49; ++j) s = t[s * 32u + sym_at(w, j)] & 0x7fffu;
     if (s == 1) return 2;
-    if (s != 0) out = 1;
+    if (s != 0 && !out && regex_open(t, live + recs[3 * r + 2], s, w)) out = 1;
   }
   return out;
+}
+
+// The same for two keys at once, their walks interleaved so that each hides
+// the other's table latency: the first key's verdict in bits 0-1, the
+// second's in bits 2-3.
+OG_HD uint32_t regex_key2(const uint32_t wa[8], const uint32_t wb[8], const uint32_t *recs,
+                          const uint16_t *dfa, const uint32_t *live, uint32_t n) {
+  uint32_t ra = 0, rb = 0;
+  for (uint32_t r = 0; r < n && (ra != 2 || rb != 2); ++r) {
+    const uint16_t *t = dfa + recs[3 * r];
+    const uint32_t *lv = live + recs[3 * r + 2];
+    uint32_t sa = recs[3 * r + 1], sb = sa;
+#pragma unroll
+    for (int j = 0; j < 49; ++j) {
+      sa = t[sa * 32u + sym_at(wa, j)] & 0x7fffu;
+      sb = t[sb * 32u + sym_at(wb, j)] & 0x7fffu;
+    }
+    if (sa == 1) ra = 2;
+    else if (sa != 0 && !ra && regex_open(t, lv, sa, wa)) ra = 1;
+    if (sb == 1) rb = 2;
+    else if (sb != 0 && !rb && regex_open(t, lv, sb, wb)) rb = 1;
+  }
+  return ra | (rb << 2);
 }
 
 // All 56 symbols of the address words: whether an expression matches.
 OG_HD bool regex_address(const uint32_t w[9], const uint32_t *recs, const uint16_t *dfa,
                          uint32_t n) {
   for (uint32_t r = 0; r < n; ++r) {
-    const uint16_t *t = dfa + recs[2 * r];
-    uint32_t e = recs[2 * r + 1];
+    const uint16_t *t = dfa + recs[3 * r];
+    uint32_t e = recs[3 * r + 1];
 #pragma unroll
     for (int j = 0; j < 56; ++j) e = t[(e & 0x7fffu) * 32u + sym_at(w, j)];
     if (e >> 15) return true;

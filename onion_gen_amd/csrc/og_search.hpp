@@ -143,9 +143,23 @@ __device__ __noinline__ bool pattern_slow(uint32_t w0, uint32_t w1, uint32_t w2,
 __device__ __noinline__ uint32_t regex_key_slow(uint32_t w0, uint32_t w1, uint32_t w2,
                                                 uint32_t w3, uint32_t w4, uint32_t w5,
                                                 uint32_t w6, uint32_t w7, const uint32_t *recs,
-                                                const uint16_t *dfa, uint32_t n) {
+                                                const uint16_t *dfa, const uint32_t *live,
+                                                uint32_t n) {
   const uint32_t w[8] = {w0, w1, w2, w3, w4, w5, w6, w7};
-  return regex_key(w, recs, dfa, n);
+  return regex_key(w, recs, dfa, live, n);
+}
+
+__device__ __noinline__ uint32_t regex_key2_slow(uint32_t a0, uint32_t a1, uint32_t a2,
+                                                 uint32_t a3, uint32_t a4, uint32_t a5,
+                                                 uint32_t a6, uint32_t a7, uint32_t b0,
+                                                 uint32_t b1, uint32_t b2, uint32_t b3,
+                                                 uint32_t b4, uint32_t b5, uint32_t b6,
+                                                 uint32_t b7, const uint32_t *recs,
+                                                 const uint16_t *dfa, const uint32_t *live,
+                                                 uint32_t n) {
+  const uint32_t wa[8] = {a0, a1, a2, a3, a4, a5, a6, a7};
+  const uint32_t wb[8] = {b0, b1, b2, b3, b4, b5, b6, b7};
+  return regex_key2(wa, wb, recs, dfa, live, n);
 }
 
 __device__ __noinline__ bool regex_address_slow(uint32_t w0, uint32_t w1, uint32_t w2,
@@ -224,7 +238,7 @@ __device__ __forceinline__ void test_key_only(const Fe10 &y, uint64_t k, const D
       uint32_t w[8];
       key_words(c, w);
       hit = regex_key_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], f.regex, f.dfa,
-                           f.nregex) == 2;
+                           f.regex_live, f.nregex) == 2;
     }
   }
   report(hit, k, c, sink);
@@ -245,12 +259,45 @@ __device__ __forceinline__ void key_stage(const Fe &c, const uint32_t w[8], cons
     maybe = (maybe || f.regex_every != 0) && !hit;
     if (__ballot(maybe) && maybe) {
       const uint32_t r =
-          regex_key_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], f.regex, f.dfa, f.nregex);
+          regex_key_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], f.regex, f.dfa,
+                         f.regex_live, f.nregex);
       hit |= r == 2;
       need |= r == 1;
     }
   }
   need &= !hit;
+}
+
+// key_stage for the two candidates of a pair, with one interleaved walk of
+// the automata for both.
+template <uint32_t MODE>
+__device__ __forceinline__ void key_stage_pair(const Fe &ca, const uint32_t wa[8], const Fe &cb,
+                                               const uint32_t wb[8], const DevFilterArgs &f,
+                                               const ScanBank *banks, bool &ha, bool &na,
+                                               bool &hb, bool &nb) {
+  bool ma, mb;
+  scan_key(wa, banks, f.nbanks, ha, na, ma);
+  scan_key(wb, banks, f.nbanks, hb, nb, mb);
+  if (MODE & (kModePrefix | kModeBitmap)) {
+    ha |= exact_key<MODE>(ca, f);
+    hb |= exact_key<MODE>(cb, f);
+  }
+  if (f.nregex) {
+    ma = (ma || f.regex_every != 0) && !ha;
+    mb = (mb || f.regex_every != 0) && !hb;
+    if (__ballot(ma || mb) && (ma || mb)) {
+      const uint32_t r = regex_key2_slow(wa[0], wa[1], wa[2], wa[3], wa[4], wa[5], wa[6], wa[7],
+                                         wb[0], wb[1], wb[2], wb[3], wb[4], wb[5], wb[6], wb[7],
+                                         f.regex, f.dfa, f.regex_live, f.nregex);
+      const uint32_t ra = ma ? (r & 3u) : 0u, rb = mb ? (r >> 2) : 0u;
+      ha |= ra == 2;
+      na |= ra == 1;
+      hb |= rb == 2;
+      nb |= rb == 1;
+    }
+  }
+  na &= !ha;
+  nb &= !hb;
 }
 
 // The address stage is deferred.  A lane whose key pass leaves a checksum
@@ -390,8 +437,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         key_words(cm, wm);
         key_words(cp, wp);
         bool hm, nm, hp, np;
-        key_stage<MODE>(cm, wm, f, banks, hm, nm);
-        key_stage<MODE>(cp, wp, f, banks, hp, np);
+        key_stage_pair<MODE>(cm, wm, cp, wp, f, banks, hm, nm, hp, np);
         if (__ballot(nm | np)) {
           // x(P - Q) has the denominator dp, x(P + Q) the denominator dm
           const Fe10 XY = fe10_mul(P.X, yq);

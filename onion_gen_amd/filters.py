@@ -63,6 +63,14 @@ class FilterSet:
             raise OgError("the set has neither general filter nor expression, or does not compile")
         return bool(kh.value), bool(nd.value), bool(hit.value)
 
+    def scan_pair(self, a, b):
+        """The key stage for two keys at once, as the kernels run the two
+        candidates of a pair: ((key_hit_a, need_a), (key_hit_b, need_b))."""
+        kh, nd = (ctypes.c_int * 2)(), (ctypes.c_int * 2)()
+        if lib().og_filters_scan_pair_host(self.handle, bytes(a), bytes(b), kh, nd) != 0:
+            raise OgError("the set runs no automaton bank or expression")
+        return (bool(kh[0]), bool(nd[0])), (bool(kh[1]), bool(nd[1]))
+
     def close(self):
         if self.handle:
             lib().og_filters_free(self.handle)

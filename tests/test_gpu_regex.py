@@ -117,3 +117,36 @@ def test_batch_runs_expressions_beside_other_searches():
     for tag, _, python in mix:
         want = truth(key.start_point(hashlib.sha256(tag).digest()), python, n)
         assert sorted(got[tag]) == want and want, (tag, len(got[tag]), len(want))
+
+
+def test_dictionary_of_general_forms_matches_oracle():
+    """General forms beyond the automaton banks (a dictionary of substrings,
+    suffixes and class prefixes) run as one automaton on every candidate; the
+    engine's survivors equal the oracle's hits.  A tenth of the dictionary is
+    taken from addresses in the range, so every kind hits."""
+    import random
+    say("case: dictionary automaton, 1,500 general forms, n=2^17")
+    n = 1 << 17
+    start = key.start_point(hashlib.sha256(b"dictionary").digest())
+    every = search(start, 0, n, 256, OracleFilters(EVERY), cap=n)
+    rng = random.Random(9)
+    A = workload.ALPHABET
+    lines = ["contains:" + "".join(rng.choice(A) for _ in range(8)) for _ in range(1200)]
+    lines += ["contains:" + "".join(rng.choice(A) for _ in range(5)) + "?" for _ in range(100)]
+    lines += ["suffix:" + "".join(rng.choice(A) for _ in range(6)) + rng.choice("aiqy") + "d"
+              for _ in range(60)]
+    for _, _, addr in rng.sample(every, 150):
+        kind = rng.randrange(3)
+        if kind == 0:
+            s = rng.randint(0, 48)
+            lines.append("contains:" + addr[s:s + 8])
+        elif kind == 1:
+            lines.append("suffix:" + addr[56 - rng.randint(7, 10):56])
+        else:
+            lines.append("[" + addr[0] + "7]" + addr[1:7])
+    fs = filters.FilterSet.parse(lines)
+    assert any(l.startswith("dictionary automaton") for l in fs.describe())
+    got = run_engine(start, fs, n, threads=1024, half=64)
+    want = sorted((k, clear_sign(pk)) for k, pk, _ in search(start, 0, n, 256, OracleFilters(lines)))
+    assert len(want) >= 150
+    assert got == want, (len(got), len(want))

@@ -283,3 +283,92 @@ def test_key_files_in_tor_layout(tmp_path):
     created2, path2 = key.write_key(tmp_path / "keys", pk, bytes(64))
     assert not created2 and path2 == path
     assert open(os.path.join(path, "hs_ed25519_secret_key"), "rb").read() == raw
+
+
+def _dictionary_lines(rng, planted):
+    """A dictionary of general forms too large for the automaton banks:
+    substrings (some with '?' and classes), suffixes and class prefixes, with
+    some taken from the planted keys' addresses so that every kind hits."""
+    A = workload.ALPHABET
+    lines = []
+    for _ in range(700):
+        w = [rng.choice(A) for _ in range(rng.randint(4, 6))]
+        if rng.random() < 0.1:
+            w[rng.randrange(len(w))] = "?"
+        if rng.random() < 0.1:
+            w[rng.randrange(len(w))] = "[a-e]"
+        lines.append("contains:" + "".join(w))
+    for _ in range(200):
+        lines.append("suffix:" + "".join(rng.choice(A) for _ in range(rng.randint(2, 6))) +
+                     rng.choice("aiqy") + "d")
+    for _ in range(40):
+        lines.append("[a-d]" + "".join(rng.choice(A) for _ in range(4)))
+    for pk in planted:
+        text = _address_text(pk)
+        kind = rng.randrange(3)
+        if kind == 0:
+            s = rng.randint(0, 50)
+            lines.append("contains:" + text[s:s + 6])
+        elif kind == 1:
+            lines.append("suffix:" + text[56 - rng.randint(3, 8):])
+        else:
+            lines.append("[" + text[0] + "z]" + text[1:5])
+    return lines
+
+
+def _check_dictionary(lines, keys):
+    mine, theirs = filters.FilterSet.parse(lines), OracleFilters(lines)
+    assert any(l.startswith("dictionary automaton") for l in mine.describe())
+    hits = needed = key_hits = 0
+    prev = None
+    for pk in keys:
+        sign = pk[31] >> 7
+        clear = pk[:31] + bytes([pk[31] & 0x7F])
+        kh, need, hit = mine.scan(clear, sign)
+        if prev is not None:
+            # the kernels' pair form: the same verdicts as one key at a time
+            assert mine.scan_pair(prev[0], clear) == (prev[1], (kh, need))
+        prev = (clear, (kh, need))
+        want = theirs.match_address(key.address_bytes(pk)) >= 0
+        assert hit == want, pk.hex()
+        if kh:
+            assert hit and mine.scan(clear, 1 - sign)[0]
+        if hit and not kh:
+            assert need, pk.hex()
+        hits += hit
+        needed += need
+        key_hits += kh
+    return hits, needed, key_hits
+
+
+def test_dictionary_automaton_agrees_with_oracle():
+    """General forms beyond the banks run as one automaton over the address
+    (the reference's flat substring automaton and tail prefilter,
+    src/filter.rs:1044-1207): its verdict equals the oracle's on every key, a
+    key-stage hit is a hit whatever the sign, and an unsettled hit is always
+    sent to the address stage."""
+    rng = random.Random(77)
+    planted = [bytes(rng.getrandbits(8) for _ in range(32)) for _ in range(60)]
+    keys = planted + [bytes(rng.getrandbits(8) for _ in range(32)) for _ in range(1000)]
+    hits, needed, key_hits = _check_dictionary(_dictionary_lines(rng, planted), keys)
+    assert hits >= 60 and key_hits > 0
+
+
+def test_dictionary_tail_liveness_is_selective():
+    """Eight-symbol substrings can reach the checksum only from symbol 48 on,
+    where the key, the sign and the fixed tail rule nearly every placement out:
+    the automaton's tail liveness sends almost no key to the checksum.  (The
+    reference's tail prefilter passes every candidate once a needle of seven
+    symbols or fewer exists, src/filter.rs:1058-1085; here symbols 49-51 and the
+    a/i/q/y, d tail still count.)"""
+    rng = random.Random(78)
+    A = workload.ALPHABET
+    planted = [bytes(rng.getrandbits(8) for _ in range(32)) for _ in range(40)]
+    lines = ["contains:" + "".join(rng.choice(A) for _ in range(8)) for _ in range(1500)]
+    for pk in planted:
+        s = rng.randint(0, 41)
+        lines.append("contains:" + _address_text(pk)[s:s + 8])
+    keys = planted + [bytes(rng.getrandbits(8) for _ in range(32)) for _ in range(800)]
+    hits, needed, key_hits = _check_dictionary(lines, keys)
+    assert hits >= 40 and key_hits >= 40
+    assert needed < 0.05 * len(keys), needed
