@@ -62,7 +62,8 @@ struct BatchArgs {
 
 template <uint32_t MODE>
 __device__ __forceinline__ void batch_body(const BatchArgs &a, const SlotDesc &d, uint32_t slot,
-                                           uint32_t local, const ScanBank *banks) {
+                                           uint32_t local, const ScanBank *banks,
+                                           const uint16_t *dfa_lds) {
   const uint32_t chain = d.chain_lo + local;  // within the search
   Point10 P = ld_point(d.state, d.chains_total, chain);
   const cptr stride = (cptr)d.stride;
@@ -73,7 +74,7 @@ __device__ __forceinline__ void batch_body(const BatchArgs &a, const SlotDesc &d
   HitSink sink{a.counter, a.hits, a.cap, slot, a.counter + 1 + d.lead, d.budget};
   search_chain<MODE>(P, st, (cptr)a.table, a.scratch, a.nslots * a.chains,
                      slot * a.chains + local, a.half, a.rounds, d.chains_total, chain, d.k_base,
-                     d.f, banks, sink);
+                     d.f, banks, dfa_lds, sink);
   st_point(d.state, d.chains_total, chain, P);
 }
 
@@ -91,10 +92,11 @@ __global__ __launch_bounds__(256) OG_SEARCH_ATTR void og_batch_kernel(BatchArgs 
   constexpr uint32_t kAnyKey = kModePrefix | kModeBitmap;  // the key form read at run time
   if (kPatternFamily) {
     __shared__ ScanBank banks[kMaxBanks];
-    load_banks(banks, d.f);
-    batch_body<kModePattern | kAnyKey>(a, d, slot, local, banks);
+    __shared__ uint16_t dfa_lds[kLdsStates * 32];
+    load_filter(banks, dfa_lds, d.f);
+    batch_body<kModePattern | kAnyKey>(a, d, slot, local, banks, dfa_lds);
   } else if (d.f.mode) {
-    batch_body<kAnyKey>(a, d, slot, local, nullptr);
+    batch_body<kAnyKey>(a, d, slot, local, nullptr, nullptr);
   }
 }
 

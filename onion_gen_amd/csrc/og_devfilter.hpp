@@ -124,7 +124,10 @@ struct DevFilterHost {
            banks.size() * sizeof(ScanBank) + regex.size() * 4 + dfa.size() * 2 + live.size() * 4;
   }
 
+  uint32_t first_states = 0;  // states of the first automaton
+
   void add_automaton(const Regex &re) {
+    if (regex.empty()) first_states = re.dfa_states();
     regex.push_back((uint32_t)dfa.size());
     regex.push_back(re.dfa_start());
     regex.push_back((uint32_t)live.size());
@@ -322,6 +325,7 @@ struct DevFilterHost {
     a.pats = d_pats;
     a.nregex = (uint32_t)(regex.size() / 3);
     a.regex_live = d_live;
+    a.lds_states = !regex.empty() && first_states <= kLdsStates ? first_states : 0u;
     a.regex_every = regex_every ? 1u : 0u;
     a.key_proxy = key_proxy ? 1u : 0u;
     a.regex = d_regex;

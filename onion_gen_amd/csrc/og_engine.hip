@@ -41,10 +41,11 @@ __global__ __launch_bounds__(256) OG_SEARCH_ATTR void og_search_kernel(SearchArg
     st.t2d.v[l] = a.stride[20 + l];
   }
   __shared__ ScanBank banks[(MODE & kModePattern) ? kMaxBanks : 1];
-  if (MODE & kModePattern) load_banks(banks, a.f);
+  __shared__ uint16_t dfa_lds[(MODE & kModePattern) ? kLdsStates * 32 : 2];
+  if (MODE & kModePattern) load_filter(banks, dfa_lds, a.f);
   HitSink sink{a.counter, a.hits, a.cap, 0, nullptr, 0};
   search_chain<MODE>(P, st, (cptr)a.table, a.scratch, T, g, a.half, a.rounds, T, g, a.k_base, a.f,
-                     banks, sink);
+                     banks, dfa_lds, sink);
   st_point(a.state, T, g, P);
 }
 

@@ -292,7 +292,9 @@ int og_filters_scan_pair_host(const og_filterset *s, const uint8_t a[32], const 
   if (nre) {
     for (int i = 0; i < 2; ++i) maybe[i] = (maybe[i] || h.regex_every) && !kh[i];
     if (maybe[0] || maybe[1]) {
-      const uint32_t r = regex_key2(kw[0], kw[1], h.regex.data(), h.dfa.data(), h.live.data(), nre);
+      const uint32_t K = h.first_states <= kLdsStates ? h.first_states : 0u;
+      const uint32_t r = regex_key2(kw[0], kw[1], h.regex.data(), h.dfa.data(), h.live.data(),
+                                    nre, h.dfa.data() + h.regex[0], K);
       const uint32_t ri[2] = {maybe[0] ? (r & 3u) : 0u, maybe[1] ? (r >> 2) : 0u};
       for (int i = 0; i < 2; ++i) {
         kh[i] = kh[i] || ri[i] == 2;
@@ -367,14 +369,17 @@ int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int si
   if (h.key_proxy) {
     // the expressions' literal prefixes in the key tables, then the automata
     kh = h.proxy_set.match_key(packed) >= 0 &&
-         regex_key(kw, h.regex.data(), h.dfa.data(), h.live.data(), nre) == 2;
+         regex_key(kw, h.regex.data(), h.dfa.data(), h.live.data(), nre, h.dfa.data(), 0u) == 2;
   } else {
     scan_key(kw, banks, nb, kh, nd, maybe);
   }
   if (nre && !h.key_proxy) {
     maybe = (maybe || h.regex_every) && !kh;
     if (maybe) {
-      const uint32_t r = regex_key(kw, h.regex.data(), h.dfa.data(), h.live.data(), nre);
+      // the first automaton in LDS as the kernels hold it, when it fits
+      const uint32_t K = h.first_states <= kLdsStates ? h.first_states : 0u;
+      const uint32_t r = regex_key(kw, h.regex.data(), h.dfa.data(), h.live.data(), nre,
+                                   h.dfa.data() + h.regex[0], K);
       kh |= r == 2;
       nd |= r == 1;
     }

@@ -59,6 +59,10 @@ constexpr int kMaxBuckets = 16;
 constexpr int kMaxPatterns = 64;  // symbol-wise records; the banks bound the set
 constexpr int kPatWords = 4 + 56;  // DevPattern: len, first, last, key_last, sets[56]
 constexpr int kMaxRegexes = 16;
+// The first automaton is kept in each workgroup's LDS when it has at most
+// this many states (32 KiB).  A larger one is walked in global memory: a wave
+// waits whenever any lane leaves the LDS part, so a partial copy does not pay.
+constexpr uint32_t kLdsStates = 512;
 
 struct DevFilterArgs {
   uint32_t mode;
@@ -81,6 +85,8 @@ struct DevFilterArgs {
   const uint32_t *regex;          //   records, entries and tail-liveness
   const uint16_t *dfa;            //   bits: og_scan.hpp
   const uint32_t *regex_live;
+  uint32_t lds_states;            //   the first automaton's states, when it is
+                                  //   copied to LDS whole (0: it is not)
 };
 
 struct SearchArgs {

@@ -177,7 +177,10 @@ OG_HD void address_words(const uint32_t kw[8], uint32_t sign, uint32_t chk, uint
 // From state s after symbol 48 of an open expression: whether a match can
 // still end in symbols 49..55.  Symbol 49 takes both values of the deferred
 // sign bit, symbol 50 is exact, and the tail-liveness bit covers the rest.
-OG_HD bool regex_open(const uint16_t *t, const uint32_t *lv, uint32_t s, const uint32_t w[8]) {
+// The table pointers are template parameters so that the device can pass
+// them in the global address space (plain global loads, not flat ones).
+template <class T16, class T32>
+OG_HD bool regex_open(T16 t, T32 lv, uint32_t s, const uint32_t w[8]) {
   const uint32_t t49 = sym_at(w, 49), t50 = sym_at(w, 50), y248 = w[7] & 1u;
   for (uint32_t sign = 0; sign < 4; sign += 2) {
     const uint32_t e = t[s * 32u + (t49 | sign)] & 0x7fffu;
@@ -189,16 +192,37 @@ OG_HD bool regex_open(const uint16_t *t, const uint32_t *lv, uint32_t s, const u
   return false;
 }
 
+// One walk of an automaton over symbols 0..48 from state s, reading the
+// table at `t`.  TP is the pointer type: the device walks either the global
+// table or, when the whole automaton fits, its LDS copy.
+template <class TP>
+OG_HD uint32_t regex_walk49(TP t, uint32_t s, const uint32_t w[8]) {
+#pragma unroll
+  for (int j = 0; j < 49; ++j) s = t[s * 32u + sym_at(w, j)] & 0x7fffu;
+  return s;
+}
+
+template <class TP>
+OG_HD void regex_walk49x2(TP t, uint32_t &sa, uint32_t &sb, const uint32_t wa[8],
+                          const uint32_t wb[8]) {
+#pragma unroll
+  for (int j = 0; j < 49; ++j) {
+    sa = t[sa * 32u + sym_at(wa, j)] & 0x7fffu;
+    sb = t[sb * 32u + sym_at(wb, j)] & 0x7fffu;
+  }
+}
+
 // Symbols 0..48 (what the key settles) through every expression: 2 when one
 // has matched, 1 when one may still match on the address, 0 otherwise.
-OG_HD uint32_t regex_key(const uint32_t w[8], const uint32_t *recs, const uint16_t *dfa,
-                         const uint32_t *live, uint32_t n) {
+// `in_lds`: the first expression's whole table is also at `lds`.
+template <class T16, class T32, class TL>
+OG_HD uint32_t regex_key(const uint32_t w[8], const uint32_t *recs, T16 dfa, T32 live,
+                         uint32_t n, TL lds, uint32_t in_lds) {
   uint32_t out = 0;
   for (uint32_t r = 0; r < n; ++r) {
-    const uint16_t *t = dfa + recs[3 * r];
+    const T16 t = dfa + recs[3 * r];
     uint32_t s = recs[3 * r + 1];
-#pragma unroll
-    for (int j = 0; j < 49; ++j) s = t[s * 32u + sym_at(w, j)] & 0x7fffu;
+    s = (r == 0 && in_lds) ? regex_walk49(lds, s, w) : regex_walk49(t, s, w);
     if (s == 1) return 2;
     if (s != 0 && !out && regex_open(t, live + recs[3 * r + 2], s, w)) out = 1;
   }
@@ -208,18 +232,16 @@ OG_HD uint32_t regex_key(const uint32_t w[8], const uint32_t *recs, const uint16
 // The same for two keys at once, their walks interleaved so that each hides
 // the other's table latency: the first key's verdict in bits 0-1, the
 // second's in bits 2-3.
+template <class T16, class T32, class TL>
 OG_HD uint32_t regex_key2(const uint32_t wa[8], const uint32_t wb[8], const uint32_t *recs,
-                          const uint16_t *dfa, const uint32_t *live, uint32_t n) {
+                          T16 dfa, T32 live, uint32_t n, TL lds, uint32_t in_lds) {
   uint32_t ra = 0, rb = 0;
   for (uint32_t r = 0; r < n && (ra != 2 || rb != 2); ++r) {
-    const uint16_t *t = dfa + recs[3 * r];
-    const uint32_t *lv = live + recs[3 * r + 2];
+    const T16 t = dfa + recs[3 * r];
+    const T32 lv = live + recs[3 * r + 2];
     uint32_t sa = recs[3 * r + 1], sb = sa;
-#pragma unroll
-    for (int j = 0; j < 49; ++j) {
-      sa = t[sa * 32u + sym_at(wa, j)] & 0x7fffu;
-      sb = t[sb * 32u + sym_at(wb, j)] & 0x7fffu;
-    }
+    if (r == 0 && in_lds) regex_walk49x2(lds, sa, sb, wa, wb);
+    else regex_walk49x2(t, sa, sb, wa, wb);
     if (sa == 1) ra = 2;
     else if (sa != 0 && !ra && regex_open(t, lv, sa, wa)) ra = 1;
     if (sb == 1) rb = 2;
@@ -229,10 +251,10 @@ OG_HD uint32_t regex_key2(const uint32_t wa[8], const uint32_t wb[8], const uint
 }
 
 // All 56 symbols of the address words: whether an expression matches.
-OG_HD bool regex_address(const uint32_t w[9], const uint32_t *recs, const uint16_t *dfa,
-                         uint32_t n) {
+template <class T16>
+OG_HD bool regex_address(const uint32_t w[9], const uint32_t *recs, T16 dfa, uint32_t n) {
   for (uint32_t r = 0; r < n; ++r) {
-    const uint16_t *t = dfa + recs[3 * r];
+    const T16 t = dfa + recs[3 * r];
     uint32_t e = recs[3 * r + 1];
 #pragma unroll
     for (int j = 0; j < 56; ++j) e = t[(e & 0x7fffu) * 32u + sym_at(w, j)];

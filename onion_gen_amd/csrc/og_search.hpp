@@ -140,13 +140,17 @@ __device__ __noinline__ bool pattern_slow(uint32_t w0, uint32_t w1, uint32_t w2,
 
 // The expressions' automata, out of line like pattern_slow: over the key's
 // symbols 0..48 (2 matched, 1 open, 0 closed) and over the whole address.
+// Their tables are read as global memory (hipMalloc'd by the host).
+typedef const __attribute__((address_space(1))) uint16_t *gptr16;
+typedef const __attribute__((address_space(1))) uint32_t *gptr32;
+typedef const __attribute__((address_space(3))) uint16_t *lptr16;
 __device__ __noinline__ uint32_t regex_key_slow(uint32_t w0, uint32_t w1, uint32_t w2,
                                                 uint32_t w3, uint32_t w4, uint32_t w5,
                                                 uint32_t w6, uint32_t w7, const uint32_t *recs,
                                                 const uint16_t *dfa, const uint32_t *live,
-                                                uint32_t n) {
+                                                uint32_t n, lptr16 lds, uint32_t K) {
   const uint32_t w[8] = {w0, w1, w2, w3, w4, w5, w6, w7};
-  return regex_key(w, recs, dfa, live, n);
+  return regex_key(w, recs, (gptr16)dfa, (gptr32)live, n, lds, K);
 }
 
 __device__ __noinline__ uint32_t regex_key2_slow(uint32_t a0, uint32_t a1, uint32_t a2,
@@ -156,10 +160,10 @@ __device__ __noinline__ uint32_t regex_key2_slow(uint32_t a0, uint32_t a1, uint3
                                                  uint32_t b4, uint32_t b5, uint32_t b6,
                                                  uint32_t b7, const uint32_t *recs,
                                                  const uint16_t *dfa, const uint32_t *live,
-                                                 uint32_t n) {
+                                                 uint32_t n, lptr16 lds, uint32_t K) {
   const uint32_t wa[8] = {a0, a1, a2, a3, a4, a5, a6, a7};
   const uint32_t wb[8] = {b0, b1, b2, b3, b4, b5, b6, b7};
-  return regex_key2(wa, wb, recs, dfa, live, n);
+  return regex_key2(wa, wb, recs, (gptr16)dfa, (gptr32)live, n, lds, K);
 }
 
 __device__ __noinline__ bool regex_address_slow(uint32_t w0, uint32_t w1, uint32_t w2,
@@ -168,7 +172,7 @@ __device__ __noinline__ bool regex_address_slow(uint32_t w0, uint32_t w1, uint32
                                                 const uint32_t *recs, const uint16_t *dfa,
                                                 uint32_t n) {
   const uint32_t w[9] = {w0, w1, w2, w3, w4, w5, w6, w7, w8};
-  return regex_address(w, recs, dfa, n);
+  return regex_address(w, recs, (gptr16)dfa, n);
 }
 
 // Appends a wave's hits with one atomic per wave.  Records past `cap` are
@@ -238,7 +242,7 @@ __device__ __forceinline__ void test_key_only(const Fe10 &y, uint64_t k, const D
       uint32_t w[8];
       key_words(c, w);
       hit = regex_key_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], f.regex, f.dfa,
-                           f.regex_live, f.nregex) == 2;
+                           f.regex_live, f.nregex, nullptr, 0u) == 2;
     }
   }
   report(hit, k, c, sink);
@@ -251,7 +255,8 @@ __device__ __forceinline__ void test_key_only(const Fe10 &y, uint64_t k, const D
 // open asks for the address.
 template <uint32_t MODE>
 __device__ __forceinline__ void key_stage(const Fe &c, const uint32_t w[8], const DevFilterArgs &f,
-                                          const ScanBank *banks, bool &hit, bool &need) {
+                                          const ScanBank *banks, const uint16_t *dfa_lds,
+                                          bool &hit, bool &need) {
   bool maybe;
   scan_key(w, banks, f.nbanks, hit, need, maybe);
   if (MODE & (kModePrefix | kModeBitmap)) hit |= exact_key<MODE>(c, f);
@@ -260,7 +265,7 @@ __device__ __forceinline__ void key_stage(const Fe &c, const uint32_t w[8], cons
     if (__ballot(maybe) && maybe) {
       const uint32_t r =
           regex_key_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], f.regex, f.dfa,
-                         f.regex_live, f.nregex);
+                         f.regex_live, f.nregex, (lptr16)dfa_lds, f.lds_states);
       hit |= r == 2;
       need |= r == 1;
     }
@@ -273,8 +278,8 @@ __device__ __forceinline__ void key_stage(const Fe &c, const uint32_t w[8], cons
 template <uint32_t MODE>
 __device__ __forceinline__ void key_stage_pair(const Fe &ca, const uint32_t wa[8], const Fe &cb,
                                                const uint32_t wb[8], const DevFilterArgs &f,
-                                               const ScanBank *banks, bool &ha, bool &na,
-                                               bool &hb, bool &nb) {
+                                               const ScanBank *banks, const uint16_t *dfa_lds,
+                                               bool &ha, bool &na, bool &hb, bool &nb) {
   bool ma, mb;
   scan_key(wa, banks, f.nbanks, ha, na, ma);
   scan_key(wb, banks, f.nbanks, hb, nb, mb);
@@ -288,7 +293,8 @@ __device__ __forceinline__ void key_stage_pair(const Fe &ca, const uint32_t wa[8
     if (__ballot(ma || mb) && (ma || mb)) {
       const uint32_t r = regex_key2_slow(wa[0], wa[1], wa[2], wa[3], wa[4], wa[5], wa[6], wa[7],
                                          wb[0], wb[1], wb[2], wb[3], wb[4], wb[5], wb[6], wb[7],
-                                         f.regex, f.dfa, f.regex_live, f.nregex);
+                                         f.regex, f.dfa, f.regex_live, f.nregex,
+                                         (lptr16)dfa_lds, f.lds_states);
       const uint32_t ra = ma ? (r & 3u) : 0u, rb = mb ? (r >> 2) : 0u;
       ha |= ra == 2;
       na |= ra == 1;
@@ -362,7 +368,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
                                              uint32_t H, uint32_t rounds, uint32_t chains,
                                              uint32_t local, uint64_t k_base,
                                              const DevFilterArgs &f, const ScanBank *banks,
-                                             const HitSink &sink) {
+                                             const uint16_t *dfa_lds, const HitSink &sink) {
   constexpr bool kPat = (MODE & kModePattern) != 0;
   const uint64_t S = 2ull * H + 1;
   for (uint32_t r = 0; r < rounds; ++r) {
@@ -393,7 +399,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         uint32_t kw[8];
         key_words(c, kw);
         bool hit, need;
-        key_stage<MODE>(c, kw, f, banks, hit, need);
+        key_stage<MODE>(c, kw, f, banks, dfa_lds, hit, need);
         if (__ballot(need)) {
           const Fe10 x = fe10_mul(P.X, iz);
           if (need) park(scratch, top, T, g, parked, c, x, kc);
@@ -437,7 +443,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         key_words(cm, wm);
         key_words(cp, wp);
         bool hm, nm, hp, np;
-        key_stage_pair<MODE>(cm, wm, cp, wp, f, banks, hm, nm, hp, np);
+        key_stage_pair<MODE>(cm, wm, cp, wp, f, banks, dfa_lds, hm, nm, hp, np);
         if (__ballot(nm | np)) {
           // x(P - Q) has the denominator dp, x(P + Q) the denominator dm
           const Fe10 XY = fe10_mul(P.X, yq);
@@ -457,10 +463,19 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
   }
 }
 
-// The automaton banks of a filter into this workgroup's LDS copy.
-__device__ __forceinline__ void load_banks(ScanBank *lds, const DevFilterArgs &f) {
+// The automaton banks of a filter, and its first expression automaton when
+// that fits, into this workgroup's LDS copies.  The tables are whole rows of
+// 32 16-bit entries from a 256-byte aligned allocation, so they copy as
+// 32-bit words.
+__device__ __forceinline__ void load_filter(ScanBank *lds, uint16_t *dfa_lds,
+                                            const DevFilterArgs &f) {
   uint32_t *dst = reinterpret_cast<uint32_t *>(lds);
   for (uint32_t i = threadIdx.x; i < f.nbanks * kBankWords; i += blockDim.x) dst[i] = f.banks[i];
+  if (f.lds_states) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(f.dfa + f.regex[0]);
+    uint32_t *d = reinterpret_cast<uint32_t *>(dfa_lds);
+    for (uint32_t i = threadIdx.x; i < f.lds_states * 16; i += blockDim.x) d[i] = src[i];
+  }
   __syncthreads();
 }
 
