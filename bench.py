@@ -319,7 +319,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--filter", default="",
                     help="search this filter instead of the 6-symbol prefix (for profiling the "
-                         "other forms; the bench line then names it)")
+                         "other forms; the bench line then names it; dict:contains8 and "
+                         "dict:contains5 name the forms leg's substring dictionaries)")
     ap.add_argument("--mix-seconds", type=float, default=5.0,
                     help="wall-time window of each batched-mix run (0 skips the leg)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
@@ -350,7 +351,7 @@ def main():
     prefix, seed = c2["prefix"], c2["seed"]
     text = args.filter or prefix
     start = key.start_point(seed)
-    fs = filters.FilterSet.parse(text)
+    fs = filters.FilterSet.parse(form_lines(text))
     tuned = None
     if args.tuned:
         tuned = gpu.Engine.tune(start, fs)
