@@ -2,7 +2,7 @@
 
     python tests/golden/make_golden.py
 
-config1_hits.json: every hit of BASELINE configuration 1 - the secret seed is
+config1_hits.json: every hit of the CPU-sized workload - the secret seed is
 ChaCha20 seeded 0 (32 bytes), the filter a 4-symbol prefix drawn from ChaCha20
 seeded 1, BatchEngine width 256, candidates k = 0 .. 2^24-1.
 """

@@ -105,8 +105,8 @@ def test_device_checksum_matches_sha3():
 
 
 def test_config1_golden_hits():
-    """BASELINE configuration 1 on the device: 2^24 candidates, the 4-symbol
-    prefix, the committed hit list."""
+    """The CPU-sized workload on the device: 2^24 candidates, the 4-symbol
+    prefix drawn from ChaCha20 seeded 1, the committed hit list."""
     with open(os.path.join(GOLDEN, "config1_hits.json")) as f:
         g = json.load(f)
     say(f"case: config1 prefix={g['prefix']} n=2^24")
