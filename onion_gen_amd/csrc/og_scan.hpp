@@ -195,20 +195,54 @@ OG_HD bool regex_open(T16 t, T32 lv, uint32_t s, const uint32_t w[8]) {
 // One walk of an automaton over symbols 0..48 from state s, reading the
 // table at `t`.  TP is the pointer type: the device walks either the global
 // table or, when the whole automaton fits, its LDS copy.
+// The symbols of big-endian words in order, five bits at a time: a 64-bit
+// window and the words still to come.  The automaton walks read them in a
+// rolled loop, so that a walk holds a dozen registers rather than one per
+// symbol: the walks are called from the search loop, and every register a
+// callee uses is one the loop must give up around the call.
+struct SymStream {
+  uint64_t win;   // the next bits, left-aligned
+  int have;       // how many of them are valid (the same in every lane)
+  uint32_t q[7];  // the words after the window, then zeros
+
+  OG_HD SymStream(const uint32_t *w, int nwords) {
+    win = ((uint64_t)w[0] << 32) | w[1];
+    have = 64;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) q[i] = i + 2 < nwords ? w[i + 2] : 0u;
+  }
+
+  OG_HD uint32_t next() {
+    const uint32_t c = (uint32_t)(win >> 59);
+    win <<= 5;
+    have -= 5;
+    if (have < 32) {
+      win |= (uint64_t)q[0] << (32 - have);
+      have += 32;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) q[i] = q[i + 1];
+      q[6] = 0;
+    }
+    return c;
+  }
+};
+
 template <class TP>
 OG_HD uint32_t regex_walk49(TP t, uint32_t s, const uint32_t w[8]) {
-#pragma unroll
-  for (int j = 0; j < 49; ++j) s = t[s * 32u + sym_at(w, j)] & 0x7fffu;
+  SymStream st(w, 8);
+#pragma unroll 1
+  for (int j = 0; j < 49; ++j) s = t[s * 32u + st.next()] & 0x7fffu;
   return s;
 }
 
 template <class TP>
 OG_HD void regex_walk49x2(TP t, uint32_t &sa, uint32_t &sb, const uint32_t wa[8],
                           const uint32_t wb[8]) {
-#pragma unroll
+  SymStream a(wa, 8), b(wb, 8);
+#pragma unroll 1
   for (int j = 0; j < 49; ++j) {
-    sa = t[sa * 32u + sym_at(wa, j)] & 0x7fffu;
-    sb = t[sb * 32u + sym_at(wb, j)] & 0x7fffu;
+    sa = t[sa * 32u + a.next()] & 0x7fffu;
+    sb = t[sb * 32u + b.next()] & 0x7fffu;
   }
 }
 
@@ -256,8 +290,9 @@ OG_HD bool regex_address(const uint32_t w[9], const uint32_t *recs, T16 dfa, uin
   for (uint32_t r = 0; r < n; ++r) {
     const T16 t = dfa + recs[3 * r];
     uint32_t e = recs[3 * r + 1];
-#pragma unroll
-    for (int j = 0; j < 56; ++j) e = t[(e & 0x7fffu) * 32u + sym_at(w, j)];
+    SymStream st(w, 9);
+#pragma unroll 1
+    for (int j = 0; j < 56; ++j) e = t[(e & 0x7fffu) * 32u + st.next()];
     if (e >> 15) return true;
   }
   return false;
