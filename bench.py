@@ -217,6 +217,8 @@ FORMS = [
     ("dict:contains8", "1,000 eight-symbol substrings (Python random, seed 5): one automaton"),
     ("dict:contains5", "1,000 five-symbol substrings (Python random, seed 5): one automaton, "
                        "most candidates hashed (a match can still end in the checksum)"),
+    ("dict:contains8:10000", "10,000 eight-symbol substrings (Python random, seed 5): two "
+                             "automata"),
 ]
 
 
@@ -227,9 +229,10 @@ def form_lines(text):
     import random
     from onion_gen_amd import workload
     rng = random.Random(5)
-    length = int(text[len("dict:contains"):])
+    spec = text[len("dict:contains"):].split(":")
+    length, count = int(spec[0]), int(spec[1]) if len(spec) > 1 else 1000
     return ["contains:" + "".join(rng.choice(workload.ALPHABET) for _ in range(length))
-            for _ in range(1000)]
+            for _ in range(count)]
 
 
 def forms_leg(threads, half, rounds, launches):
@@ -319,8 +322,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--filter", default="",
                     help="search this filter instead of the 6-symbol prefix (for profiling the "
-                         "other forms; the bench line then names it; dict:contains8 and "
-                         "dict:contains5 name the forms leg's substring dictionaries)")
+                         "other forms; the bench line then names it; dict:contains<L>[:<count>] "
+                         "names a substring dictionary of the forms leg)")
     ap.add_argument("--mix-seconds", type=float, default=5.0,
                     help="wall-time window of each batched-mix run (0 skips the leg)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,

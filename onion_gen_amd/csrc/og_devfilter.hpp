@@ -105,9 +105,10 @@ struct DevFilterHost {
   std::vector<uint32_t> live;     // their tail-liveness bits
   bool regex_every = false;
   bool proxyless = false;         // an expression found no room for its proxies
-  // General filters beyond the banks, compiled into one automaton (a
-  // dictionary of substrings and suffixes): run on every candidate.
-  std::shared_ptr<const Regex> dictionary;
+  // General filters beyond the banks, compiled into automata (a dictionary
+  // of substrings and suffixes): one, or several when one would exceed the
+  // state limit; run on every candidate.
+  std::vector<std::shared_ptr<const Regex>> dictionaries;
   size_t dictionary_forms = 0;
   // A set of key-decidable expressions only: their literal prefixes, which
   // the key tables hold in place of the set's own (key_proxy).
@@ -144,10 +145,11 @@ struct DevFilterHost {
   // sets, anchored as their forms are (the reference's flat substring
   // automaton, src/filter.rs:1124-1207, here over the whole address with the
   // suffixes and class prefixes in the same automaton).
-  static std::string dictionary_source(const FilterSet &fs) {
+  static std::string dictionary_source(const FilterSet &fs, size_t lo, size_t hi) {
     static const char kSym[] = "abcdefghijklmnopqrstuvwxyz234567";
     std::string src;
-    for (uint32_t idx : fs.general) {
+    for (size_t n = lo; n < hi; ++n) {
+      const uint32_t idx = fs.general[n];
       const Filter &f = fs.filters[idx];
       if (!src.empty()) src += '|';
       if (f.anchor == kStart) src += '^';
@@ -188,6 +190,26 @@ struct DevFilterHost {
     key_proxy = true;
     add_automata(fs);
     return "";
+  }
+
+  // General filters [lo, hi) as one automaton, or, when that exceeds the state
+  // limit, as the automata of its two halves; at most `budget` in all.
+  std::string compile_dictionary(const FilterSet &fs, size_t lo, size_t hi, size_t budget) {
+    RegexLimits lim;
+    lim.program = 4000000;
+    lim.states = kRegexMaxStates;
+    lim.proxies = false;
+    std::shared_ptr<const Regex> re;
+    const std::string e = Regex::compile(dictionary_source(fs, lo, hi), re, lim);
+    if (e.empty()) {
+      if (dictionaries.size() >= budget) return "more automata than the device holds (16)";
+      dictionaries.push_back(re);
+      return "";
+    }
+    if (hi - lo < 2 || e.find("automaton states") == std::string::npos) return e;
+    const size_t mid = lo + (hi - lo) / 2;
+    const std::string a = compile_dictionary(fs, lo, mid, budget);
+    return a.empty() ? compile_dictionary(fs, mid, hi, budget) : a;
   }
 
   // Empty string on success.
@@ -231,18 +253,15 @@ struct DevFilterHost {
       bool dict = fs.general.size() > (size_t)kMaxPatterns;
       if (!dict && !build_banks(fs, banks, &proxyless).empty()) dict = true;
       if (dict) {
-        // too many for the banks: one automaton over every general filter
-        RegexLimits lim;
-        lim.program = 4000000;
-        lim.states = kRegexMaxStates;
-        lim.proxies = false;
-        const std::string e = Regex::compile(dictionary_source(fs), dictionary, lim);
-        if (!e.empty()) return "the general filters as one automaton: " + e;
+        // too many for the banks: automata over the general filters
+        const std::string e =
+            compile_dictionary(fs, 0, fs.general.size(), kMaxRegexes - fs.regexes.size());
+        if (!e.empty()) return "the general filters as automata: " + e;
         dictionary_forms = fs.general.size();
         banks.clear();
         const std::string eb = build_banks(fs, banks, &proxyless, false);
         if (!eb.empty()) return eb;
-        add_automaton(*dictionary);
+        for (const auto &d : dictionaries) add_automaton(*d);
       } else {
         for (uint32_t i : fs.general) {
           DevPattern p = dev_pattern(fs.filters[i]);
@@ -254,7 +273,7 @@ struct DevFilterHost {
         }
       }
       add_automata(fs);
-      regex_every = proxyless || dictionary != nullptr;
+      regex_every = proxyless || !dictionaries.empty();
     }
     if (!mode) return "the filter set is empty";
     return "";

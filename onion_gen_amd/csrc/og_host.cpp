@@ -318,14 +318,20 @@ size_t og_filters_describe(const og_filterset *s, char *out, size_t cap) {
          (fs.index_bits ? ", bitmap of 2^" + std::to_string(fs.index_bits) + " bits" : "") + "\n";
   const DevFilterHost &h = const_cast<og_filterset *>(s)->host_tables();
   const std::string &e = s->tables_err;
-  if (h.dictionary)
-    t += "dictionary automaton: " + std::to_string(h.dictionary->dfa_states()) + " states over " +
-         std::to_string(h.dictionary_forms) +
+  if (!h.dictionaries.empty()) {
+    size_t states = 0;
+    for (const auto &d : h.dictionaries) states += d->dfa_states();
+    t += (h.dictionaries.size() == 1 ? "dictionary automaton: "
+                                     : "dictionary automata: " +
+                                           std::to_string(h.dictionaries.size()) + " of ") +
+         std::to_string(states) + " states over " + std::to_string(h.dictionary_forms) +
          " general form(s), run on every candidate; the checksum only where a match can still "
          "end in it\n";
-  if ((!fs.general.empty() && !h.dictionary) || (!fs.regexes.empty() && !h.key_proxy)) {
+  }
+  const bool dict = !h.dictionaries.empty();
+  if ((!fs.general.empty() && !dict) || (!fs.regexes.empty() && !h.key_proxy)) {
     t += "automaton banks: " + std::to_string(h.banks.size()) + " for " +
-         std::to_string(h.dictionary ? 0 : fs.general.size()) + " general form(s)";
+         std::to_string(dict ? 0 : fs.general.size()) + " general form(s)";
     if (!fs.regexes.empty()) t += " and the proxies of " + std::to_string(fs.regexes.size()) + " expression(s)";
     t += e.empty() ? "\n" : " (" + e + ")\n";
   }

@@ -150,3 +150,28 @@ def test_dictionary_of_general_forms_matches_oracle():
     want = sorted((k, clear_sign(pk)) for k, pk, _ in search(start, 0, n, 256, OracleFilters(lines)))
     assert len(want) >= 150
     assert got == want, (len(got), len(want))
+
+
+def test_dictionary_split_across_automata_matches_every_address():
+    """10,000 eight-symbol substrings exceed one automaton's states and are
+    split across two; the engine reports exactly the candidates whose address
+    holds one of the words (decided here by a set of the words over every
+    8-symbol window of the oracle's addresses)."""
+    import random
+    say("case: 10,000 substrings in two automata, n=2^16")
+    n = 1 << 16
+    start = key.start_point(hashlib.sha256(b"split dictionary").digest())
+    every = search(start, 0, n, 256, OracleFilters(EVERY), cap=n)
+    rng = random.Random(10)
+    A = workload.ALPHABET
+    words = {"".join(rng.choice(A) for _ in range(8)) for _ in range(10000)}
+    for _, _, addr in rng.sample(every, 60):
+        s = rng.randint(0, 48)
+        words.add(addr[s:s + 8])
+    fs = filters.FilterSet.parse(["contains:" + w for w in sorted(words)])
+    assert any(l.startswith("dictionary automata: 2 of") for l in fs.describe())
+    got = run_engine(start, fs, n, threads=1024, half=64)
+    want = sorted((k, clear_sign(pk)) for k, pk, addr in every
+                  if any(addr[s:s + 8] in words for s in range(49)))
+    assert len(want) >= 60
+    assert got == want, (len(got), len(want))
