@@ -1169,7 +1169,10 @@ bool build_dfa(const std::vector<Regex::Inst> &prog, uint32_t max_states,
   std::vector<uint32_t> own[32], merged;
   for (uint32_t u = 0; u < pend.size(); ++u) {
     if (u > 0 && mid[u]) continue;  // matched: absorbing, not expanded
-    if (pend.size() > max_states) {
+    // the numbered table holds the raw states plus the two absorbing ones,
+    // and a state number must fit the entries' 15 bits; an expansion adds at
+    // most 32 states
+    if (pend.size() + 32 + 2 > max_states) {
       err = "the expression needs more than " + std::to_string(max_states) + " automaton states";
       return false;
     }
