@@ -74,7 +74,7 @@ __device__ __forceinline__ void batch_body(const BatchArgs &a, const SlotDesc &d
   HitSink sink{a.counter, a.hits, a.cap, slot, a.counter + 1 + d.lead, d.budget};
   search_chain<MODE>(P, st, (cptr)a.table, a.scratch, a.nslots * a.chains,
                      slot * a.chains + local, a.half, a.rounds, d.chains_total, chain, d.k_base,
-                     d.f, banks, dfa_lds, sink);
+                     d.f, banks, dfa_lds, nullptr, sink);
   st_point(d.state, d.chains_total, chain, P);
 }
 

@@ -45,8 +45,22 @@ __global__ __launch_bounds__(256) OG_SEARCH_ATTR void og_search_kernel(SearchArg
   if (MODE & kModePattern) load_filter(banks, dfa_lds, a.f);
   HitSink sink{a.counter, a.hits, a.cap, 0, nullptr, 0};
   search_chain<MODE>(P, st, (cptr)a.table, a.scratch, T, g, a.half, a.rounds, T, g, a.k_base, a.f,
-                     banks, dfa_lds, sink);
+                     banks, dfa_lds, a.parked, sink);
   st_point(a.state, T, g, P);
+}
+
+// The deferred address stage of a one-round pattern launch: per chain, the
+// candidates the key stage parked (sign of x, SHA3-256, the exact automaton
+// pass).  Without the search loop's registers it runs at four waves per SIMD
+// instead of two.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void
+og_address_kernel(SearchArgs a) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ ScanBank banks[kMaxBanks];
+  __shared__ uint16_t dfa_lds[kLdsStates * 32];
+  load_filter(banks, dfa_lds, a.f);
+  HitSink sink{a.counter, a.hits, a.cap, 0, nullptr, 0};
+  address_stage(a.scratch, a.half, a.threads, g, a.parked[g], a.f, banks, sink);
 }
 
 __global__ __launch_bounds__(256) void og_setup_kernel(SetupArgs a) {
@@ -175,6 +189,7 @@ struct og_engine {
   DevFilterHost filt;
   uint32_t *d_counter = nullptr;
   HitRec *d_hits = nullptr;
+  uint32_t *d_parked = nullptr;  // per chain, for the address kernel
   uint32_t *h_counter = nullptr;
   HitRec *h_hits = nullptr;
   SearchArgs args;
@@ -210,6 +225,7 @@ void free_engine(og_engine *e) {
   e->filt.free_device();
   (void)hipFree(e->d_counter);
   (void)hipFree(e->d_hits);
+  (void)hipFree(e->d_parked);
   (void)hipHostFree(e->h_counter);
   (void)hipHostFree(e->h_hits);
   if (e->ev0) (void)hipEventDestroy(e->ev0);
@@ -222,6 +238,8 @@ template <uint32_t MODE>
 hipError_t launch_search(og_engine *e) {
   const uint32_t grid = e->threads / e->block;
   hipLaunchKernelGGL(og_search_kernel<MODE>, dim3(grid), dim3(e->block), 0, e->stream, e->args);
+  if ((MODE & kModePattern) && e->args.parked)
+    hipLaunchKernelGGL(og_address_kernel, dim3(grid), dim3(e->block), 0, e->stream, e->args);
   return hipGetLastError();
 }
 
@@ -380,6 +398,13 @@ int create_engine(int ordinal, const uint8_t start_xyzt[128], const og_engine_op
   a.rounds = o.rounds;
   a.f = e->filt.args();
   a.counter = e->d_counter;
+  // A one-round pattern launch leaves its address stage to its own kernel.
+  if ((e->filt.mode & kModePattern) && o.rounds == 1) {
+    OG_TRY(hipMalloc(&e->d_parked, (size_t)o.threads * 4));
+    a.parked = e->d_parked;
+  } else {
+    a.parked = nullptr;
+  }
   a.hits = e->d_hits;
   a.cap = e->cap;
 

@@ -100,6 +100,8 @@ struct SearchArgs {
   uint32_t *counter;      // survivors found (may exceed cap)
   HitRec *hits;
   uint32_t cap;
+  uint32_t *parked;       // per chain: candidates left for the address kernel
+                          // (one round per launch), or null: drained in place
 };
 
 struct SetupArgs {

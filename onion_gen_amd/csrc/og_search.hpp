@@ -310,23 +310,42 @@ __device__ __forceinline__ void key_stage_pair(const Fe &ca, const uint32_t wa[8
 // placement possible computes the sign of x (bit 0 of its canonical form) and
 // parks the candidate - packed key with the sign in bit 255, and its index -
 // in a scratch plane the backward pass has already consumed, counting down
-// from the plane past the window's 3H.  After the backward pass the window
-// drains the parked candidates (SHA3-256, the exact automaton pass) with
-// little else live, so the Keccak state never competes with the chain's.
-// Entry j lands in plane 3H - j; after the centre and pairs H-1..m at most
-// 2(H-m)+1 entries exist, and 3H - 2(H-m) >= 3m, so an entry never reaches a
-// plane the backward pass has yet to read.
+// from the plane past the window's 3H.  After the backward pass the parked
+// candidates are drained (SHA3-256, the exact automaton pass) with little
+// else live, so the Keccak state never competes with the chain's.
+//
+// Parking is compacted per wave.  The wave keeps one count Q; the lanes that
+// park at a step take entries Q, Q+1, ... in lane order, and entry q lands in
+// plane 3H - floor(q/64), in the column of the wave's lane q mod 64.  So one
+// step's stores are contiguous (lanes' own counts would diverge within a few
+// steps and scatter every store over 64 planes), and the drain gives every
+// lane the same share: lane i takes entries i, i+64, ..., in planes 3H, 3H-1,
+// ... of its own column.  After the centre and pairs H-1..m the wave has at
+// most 64(2(H-m)+1) entries, in at most 2(H-m)+1 planes, and
+// 3H - 2(H-m) >= 3m, so an entry never reaches a plane the backward pass has
+// yet to read.  `mask` is the wave's ballot of `want`.
 __device__ __forceinline__ void park(uint32_t *scratch, uint32_t top, uint32_t T, uint32_t g,
-                                     uint32_t &n, const Fe &c, const Fe10 &x, uint64_t k) {
-  const uint32_t sign = fe10_canon(x).v[0] & 1u;
-  Fe10 e;
+                                     uint32_t &Q, uint64_t mask, bool want, const Fe &c,
+                                     const Fe10 &x, uint64_t k) {
+  if (want) {
+    const uint32_t lane = __lane_id();
+    const uint32_t q = Q + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+    const uint32_t sign = fe10_canon(x).v[0] & 1u;
+    Fe10 e;
 #pragma unroll
-  for (int l = 0; l < 8; ++l) e.v[l] = c.v[l];
-  e.v[7] |= sign << 31;
-  e.v[8] = (uint32_t)k;
-  e.v[9] = (uint32_t)(k >> 32);
-  st10(scratch, top - n, T, g, e);
-  ++n;
+    for (int l = 0; l < 8; ++l) e.v[l] = c.v[l];
+    e.v[7] |= sign << 31;
+    e.v[8] = (uint32_t)k;
+    e.v[9] = (uint32_t)(k >> 32);
+    st10(scratch, top - (q >> 6), T, g - lane + (q & 63u), e);
+  }
+  Q += (uint32_t)__popcll(mask);
+}
+
+// A lane's share of its wave's Q parked entries.
+__device__ __forceinline__ uint32_t parked_share(uint32_t Q) {
+  const uint32_t lane = __lane_id();
+  return Q > lane ? (Q - lane + 63u) >> 6 : 0u;
 }
 
 template <uint32_t MODE>
@@ -358,6 +377,14 @@ __device__ __forceinline__ void drain(const uint32_t *scratch, uint32_t top, uin
   }
 }
 
+// The address stage of one chain's parked candidates, as its own kernel runs
+// it: the same drain, with the window's parking planes left by the search.
+__device__ __forceinline__ void address_stage(const uint32_t *scratch, uint32_t half, uint32_t T,
+                                              uint32_t g, uint32_t n, const DevFilterArgs &f,
+                                              const ScanBank *banks, const HitSink &sink) {
+  drain<kModePattern>(scratch, kPlanesPerPair * half, T, g, n, f, banks, sink);
+}
+
 // `rounds` windows of one chain (the method is described in og_kernels.hpp).
 // Every multiplication takes its looser operand first (fe10_mul's bounds).  g
 // indexes the SoA buffers (stride T); the chain's windows are numbered
@@ -368,7 +395,8 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
                                              uint32_t H, uint32_t rounds, uint32_t chains,
                                              uint32_t local, uint64_t k_base,
                                              const DevFilterArgs &f, const ScanBank *banks,
-                                             const uint16_t *dfa_lds, const HitSink &sink) {
+                                             const uint16_t *dfa_lds, uint32_t *parked_out,
+                                             const HitSink &sink) {
   constexpr bool kPat = (MODE & kModePattern) != 0;
   const uint64_t S = 2ull * H + 1;
   for (uint32_t r = 0; r < rounds; ++r) {
@@ -390,7 +418,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
     // the backward pass
     Fe10 inv = fe10_invert(fe10_mul(D, P.Z));
     const uint32_t top = kPlanesPerPair * H;  // the parking plane past the window's
-    uint32_t parked = 0;
+    uint32_t parked = 0;  // the wave's parked entries (uniform)
     {
       const Fe10 iz = fe10_mul(inv, D);
       const Fe10 y = fe10_mul(P.Y, iz);
@@ -400,10 +428,8 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         key_words(c, kw);
         bool hit, need;
         key_stage<MODE>(c, kw, f, banks, dfa_lds, hit, need);
-        if (__ballot(need)) {
-          const Fe10 x = fe10_mul(P.X, iz);
-          if (need) park(scratch, top, T, g, parked, c, x, kc);
-        }
+        const uint64_t mn = __ballot(need);
+        if (mn) park(scratch, top, T, g, parked, mn, need, c, fe10_mul(P.X, iz), kc);
         report(hit, kc, c, sink);
       } else {
         test_key_only<MODE>(y, kc, f, sink);
@@ -444,12 +470,13 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         key_words(cp, wp);
         bool hm, nm, hp, np;
         key_stage_pair<MODE>(cm, wm, cp, wp, f, banks, dfa_lds, hm, nm, hp, np);
-        if (__ballot(nm | np)) {
+        const uint64_t mm = __ballot(nm), mp = __ballot(np);
+        if (mm | mp) {
           // x(P - Q) has the denominator dp, x(P + Q) the denominator dm
           const Fe10 XY = fe10_mul(P.X, yq);
           const Fe10 YX = fe10_mul(P.Y, xq);
-          if (nm) park(scratch, top, T, g, parked, cm, fe10_mul(fe10_sub(XY, YX), idp), km);
-          if (np) park(scratch, top, T, g, parked, cp, fe10_mul(fe10_add(XY, YX), idm), kp);
+          if (mm) park(scratch, top, T, g, parked, mm, nm, cm, fe10_mul(fe10_sub(XY, YX), idp), km);
+          if (mp) park(scratch, top, T, g, parked, mp, np, cp, fe10_mul(fe10_add(XY, YX), idm), kp);
         }
         report(hm, km, cm, sink);
         report(hp, kp, cp, sink);
@@ -458,7 +485,13 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         test_key_only<MODE>(yp, kp, f, sink);
       }
     }
-    if (kPat) drain<MODE>(scratch, top, T, g, parked, f, banks, sink);
+    if (kPat) {
+      // the address stage: here, or (one round per launch) in the address
+      // kernel, which runs it at an occupancy the search loop cannot have
+      const uint32_t share = parked_share(parked);
+      if (parked_out) parked_out[g] = share;
+      else drain<MODE>(scratch, top, T, g, share, f, banks, sink);
+    }
     P = pt10_add1(P, st);
   }
 }

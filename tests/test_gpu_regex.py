@@ -175,3 +175,19 @@ def test_dictionary_split_across_automata_matches_every_address():
                   if any(addr[s:s + 8] in words for s in range(49)))
     assert len(want) >= 60
     assert got == want, (len(got), len(want))
+
+
+def test_address_stage_in_place_and_deferred_agree():
+    """A one-round launch leaves the address stage (sign, SHA3, exact pass) to
+    its own kernel; a launch of several rounds drains it in place.  Over the
+    same candidates both report the same survivors, and those equal the
+    oracle's."""
+    say("case: address stage deferred (rounds 1) vs in place (rounds 2)")
+    lines = ["suffix:[a-e]aqd", "contains:zz7", "regex:^zz.*qd$"]
+    n = 1 << 17
+    start = key.start_point(hashlib.sha256(b"address stage").digest())
+    fs = filters.FilterSet.parse(lines)
+    deferred = run_engine(start, fs, n, threads=512, half=64, rounds=1)
+    in_place = run_engine(start, fs, n, threads=512, half=64, rounds=2)
+    want = truth(start, ["[a-e]aqd$", "zz7", "^zz.*qd$"], n)
+    assert want and deferred == want and in_place == want, (len(deferred), len(in_place), len(want))
