@@ -58,6 +58,8 @@ struct BatchArgs {
   uint32_t *counter;    // [0]: ring records; [1 + slot]: every hit of the search led there
   HitRec *hits;
   uint32_t cap;
+  uint32_t *parked;     // one round per step: per column, the candidates left for
+                        // the address kernel; null: drained in the search kernel
 };
 
 template <uint32_t MODE>
@@ -74,7 +76,7 @@ __device__ __forceinline__ void batch_body(const BatchArgs &a, const SlotDesc &d
   HitSink sink{a.counter, a.hits, a.cap, slot, a.counter + 1 + d.lead, d.budget};
   search_chain<MODE>(P, st, (cptr)a.table, a.scratch, a.nslots * a.chains,
                      slot * a.chains + local, a.half, a.rounds, d.chains_total, chain, d.k_base,
-                     d.f, banks, dfa_lds, nullptr, sink);
+                     d.f, banks, dfa_lds, a.parked, sink);
   st_point(d.state, d.chains_total, chain, P);
 }
 
@@ -98,6 +100,23 @@ __global__ __launch_bounds__(256) OG_SEARCH_ATTR void og_batch_kernel(BatchArgs 
   } else if (d.f.mode) {
     batch_body<kAnyKey>(a, d, slot, local, nullptr, nullptr);
   }
+}
+
+// The deferred address stage of the pattern family's slots (a one-round
+// step), at the occupancy the search loop's registers deny the search kernel.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void
+og_batch_address_kernel(BatchArgs a) {
+  const uint32_t blocks_per_slot = a.chains / 256;
+  const uint32_t slot = blockIdx.x / blocks_per_slot;
+  const uint32_t local = (blockIdx.x % blocks_per_slot) * 256 + threadIdx.x;
+  const SlotDesc d = a.slots[slot];  // block-uniform
+  if (!(d.f.mode & kModePattern)) return;
+  __shared__ ScanBank banks[kMaxBanks];
+  __shared__ uint16_t dfa_lds[kLdsStates * 32];
+  load_filter(banks, dfa_lds, d.f);
+  HitSink sink{a.counter, a.hits, a.cap, slot, a.counter + 1 + d.lead, d.budget};
+  const uint32_t g = slot * a.chains + local;
+  address_stage(a.scratch, a.half, a.nslots * a.chains, g, a.parked[g], d.f, banks, sink);
 }
 
 // Clears the ring counter and the per-slot counts (a kernel rather than a
@@ -157,7 +176,7 @@ struct og_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   og_batch_opts o{};
   uint64_t per_step = 0;  // candidates one step examines per scheduled search
-  uint32_t *d_table = nullptr, *d_scratch = nullptr;
+  uint32_t *d_table = nullptr, *d_scratch = nullptr, *d_parked = nullptr;
   SlotDesc *d_slots = nullptr, *h_slots = nullptr;
   uint32_t *d_counter = nullptr;
   HitRec *d_hits = nullptr;
@@ -199,6 +218,7 @@ void free_batch(og_batch *b) {
   for (uint32_t *p : b->d_stride) (void)hipFree(p);
   (void)hipFree(b->d_table);
   (void)hipFree(b->d_scratch);
+  (void)hipFree(b->d_parked);
   (void)hipFree(b->d_slots);
   (void)hipFree(b->d_counter);
   (void)hipFree(b->d_hits);
@@ -353,9 +373,15 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
   a.counter = b->d_counter;
   a.hits = b->d_hits;
   a.cap = o.ring_cap;
+  if (o.rounds == 1) {
+    // one round per step: the pattern family's address stage gets its own kernel
+    OG_TRY(hipMalloc(&b->d_parked, (size_t)T * 4));
+    a.parked = b->d_parked;
+  }
 
   // Capture one step: clear, then the two family kernels as parallel branches
-  // (fork/join through events), then the counters back to the host.
+  // (fork/join through events; the pattern branch followed by its address
+  // kernel), then the counters back to the host.
   OG_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(og_batch_clear, dim3(1), dim3(256), 0, b->stream, b->d_counter,
                      1u + o.slots);
@@ -364,6 +390,9 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
   hipLaunchKernelGGL(og_batch_kernel<false>, dim3((unsigned)(T / 256)), dim3(256), 0, b->stream,
                      a);
   hipLaunchKernelGGL(og_batch_kernel<true>, dim3((unsigned)(T / 256)), dim3(256), 0, b->side, a);
+  if (a.parked)
+    hipLaunchKernelGGL(og_batch_address_kernel, dim3((unsigned)(T / 256)), dim3(256), 0, b->side,
+                       a);
   (void)hipEventRecord(b->join, b->side);
   (void)hipStreamWaitEvent(b->stream, b->join, 0);
   (void)hipMemcpyAsync(b->h_counter, b->d_counter, (1 + (size_t)o.slots) * 4,

@@ -178,3 +178,28 @@ def test_searches_widen_into_idle_slots_and_narrow_under_load():
         assert sorted(got[i]) == oracle(seeds[i], ["2x"], n), i
     assert total == b.examined()
     b.close()
+
+
+def test_two_round_steps_drain_the_address_stage_in_place():
+    """A step of one round leaves the pattern family's address stage to its own
+    kernel; a step of two rounds drains it inside the search kernel.  Both
+    report the oracle's hits."""
+    say("case: batch steps of 2 rounds, suffix and contains searches")
+    mix = [(b"s2", ["suffix:aqd"]), (b"c2", ["contains:abcd"])]
+    for rounds in (1, 2):
+        b = batch.Batch(slots=2, chains=256, half=32, rounds=rounds)
+        ids = {tag: b.admit(key.start_point(hashlib.sha256(tag).digest()),
+                            filters.FilterSet.parse(lines), i)
+               for i, (tag, lines) in enumerate(mix)}
+        n = 1 << 16
+        got = {tag: [] for tag, _ in mix}
+        for _ in range(-(-n // b.per_step())):
+            for h in b.step():
+                tag = mix[h.tag][0]
+                assert h.search == ids[tag]
+                if h.offset // 8 < n:
+                    got[tag].append((h.offset // 8, h.packed))
+        b.close()
+        for tag, lines in mix:
+            want = oracle(hashlib.sha256(tag).digest(), lines, n)
+            assert sorted(got[tag]) == want and want, (rounds, tag, len(got[tag]), len(want))
