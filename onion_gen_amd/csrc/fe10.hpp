@@ -63,6 +63,95 @@ OG_HD Fe10 fe10_carry64(uint64_t h[10]) {
   return r;
 }
 
+// The product by a tight, wave-uniform g (a table constant): the same
+// partial products as fe10_mul, with the doubling of odd-by-odd terms moved
+// onto g, so that every scaled form of g (2g, 19g, 38g) is scalar work and
+// v_mad_u64_u32 reads it from an SGPR; f needs no preparation at all.
+// 38 * 2^25 < 2^31, so every scaled limb fits 32 bits.
+OG_HD Fe10 fe10_mulc(const Fe10 &f, const Fe10 &g) {
+  uint64_t h[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      const int j = k - i;
+      const int jj = j < 0 ? j + 10 : j;
+      const uint32_t scale = ((i & 1) && (jj & 1) ? 2u : 1u) * (j < 0 ? 19u : 1u);
+      acc = mad64(f.v[i], scale * g.v[jj], acc);
+    }
+    h[k] = acc;
+  }
+  return fe10_carry64(h);
+}
+
+#ifdef __HIP_DEVICE_COMPILE__
+// The device product runs column by column, each column's ten partial
+// products accumulated on top of the carry out of the column before, so that
+// the carry costs no 64-bit add of its own and only one column sum is live at
+// a time (the key forms' search loop then fits three waves per SIMD).  Written
+// in plain C the compiler reassociates every column sum and adds the incoming
+// carry last; one asm block per column keeps the order: ten dependent
+// v_mad_u64_u32 (the compiler emits such dependent sequences back to back
+// itself, with no wait states).  The carry-out SGPR pair of each is unused.
+__device__ __forceinline__ uint64_t fe10_column(const uint32_t a[10], const uint32_t b[10],
+                                                uint64_t c) {
+  uint64_t r, cc;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %22\n\t"
+      "v_mad_u64_u32 %0, %1, %4, %5, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %6, %7, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %8, %9, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %10, %11, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %12, %13, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %14, %15, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %16, %17, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %18, %19, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %20, %21, %0"
+      : "=&v"(r), "=&s"(cc)
+      : "v"(a[0]), "v"(b[0]), "v"(a[1]), "v"(b[1]), "v"(a[2]), "v"(b[2]), "v"(a[3]), "v"(b[3]),
+        "v"(a[4]), "v"(b[4]), "v"(a[5]), "v"(b[5]), "v"(a[6]), "v"(b[6]), "v"(a[7]), "v"(b[7]),
+        "v"(a[8]), "v"(b[8]), "v"(a[9]), "v"(b[9]), "v"(c));
+  return r;
+}
+
+OG_HD Fe10 fe10_mul(const Fe10 &f, const Fe10 &g) {
+  uint32_t g19[10], f2[10];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    g19[i] = 19u * g.v[i];
+    f2[i] = (i & 1) ? 2u * f.v[i] : f.v[i];
+  }
+  Fe10 r;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    uint32_t a[10], b[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      const int j = k - i;
+      const int jj = j < 0 ? j + 10 : j;
+      a[i] = ((i & 1) && (jj & 1)) ? f2[i] : f.v[i];
+      b[i] = j < 0 ? g19[jj] : g.v[j];
+    }
+    if (k == 0) {  // no carry in: any order of the sum will do
+#pragma unroll
+      for (int i = 0; i < 10; ++i) acc = mad64(a[i], b[i], acc);
+    } else {
+      acc = fe10_column(a, b, acc);
+    }
+    const int w = fe10_width(k);
+    r.v[k] = (uint32_t)acc & ((1u << w) - 1);
+    acc >>= w;
+  }
+  const uint64_t h0 = (uint64_t)r.v[0] + 19 * acc;
+  r.v[1] += (uint32_t)(h0 >> 26);
+  r.v[0] = (uint32_t)h0 & ((1u << 26) - 1);
+  return r;
+}
+#else
+// The host product: every column summed on its own, then one carry pass.  The
+// same partial products and the same carries as the device form, so the two
+// agree bit for bit.
 OG_HD Fe10 fe10_mul(const Fe10 &f, const Fe10 &g) {
   uint32_t g19[10], f2[10];
 #pragma unroll
@@ -89,6 +178,7 @@ OG_HD Fe10 fe10_mul(const Fe10 &f, const Fe10 &g) {
   }
   return fe10_carry64(h);
 }
+#endif
 
 OG_HD Fe10 fe10_sq(const Fe10 &f) { return fe10_mul(f, f); }
 

@@ -85,7 +85,8 @@ __device__ __forceinline__ void batch_body(const BatchArgs &a, const SlotDesc &d
 // checksum path.  Both run over every slot; a block whose slot belongs to the
 // other family (or is empty) returns at once.
 template <bool kPatternFamily>
-__global__ __launch_bounds__(256) OG_SEARCH_ATTR void og_batch_kernel(BatchArgs a) {
+__global__ __launch_bounds__(256) OG_SEARCH_ATTR(kPatternFamily ? kModePattern : 0u) void
+og_batch_kernel(BatchArgs a) {
   const uint32_t blocks_per_slot = a.chains / 256;
   const uint32_t slot = blockIdx.x / blocks_per_slot;
   const uint32_t local = (blockIdx.x % blocks_per_slot) * 256 + threadIdx.x;

@@ -29,7 +29,7 @@ using namespace og;
 namespace {
 
 template <uint32_t MODE>
-__global__ __launch_bounds__(256) OG_SEARCH_ATTR void og_search_kernel(SearchArgs a) {
+__global__ __launch_bounds__(256) OG_SEARCH_ATTR(MODE) void og_search_kernel(SearchArgs a) {
   const uint32_t T = a.threads;
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   Point10 P = ld_point(a.state, T, g);

@@ -126,9 +126,9 @@ struct Cached10 {
 
 // P + Q for a cached Q with Z = 1, seven multiplications (src/curve.rs:130-175).
 __device__ __forceinline__ Point10 pt10_add1(const Point10 &p, const Cached10 &q) {
-  const Fe10 a = fe10_mul(fe10_add(p.Y, p.X), q.ypx);
-  const Fe10 b = fe10_mul(fe10_sub(p.Y, p.X), q.ymx);
-  const Fe10 c = fe10_mul(p.T, q.t2d);
+  const Fe10 a = fe10_mulc(fe10_add(p.Y, p.X), q.ypx);
+  const Fe10 b = fe10_mulc(fe10_sub(p.Y, p.X), q.ymx);
+  const Fe10 c = fe10_mulc(p.T, q.t2d);
   const Fe10 d = fe10_add(p.Z, p.Z);
   const Fe10 e = fe10_sub(a, b), h = fe10_add(a, b);
   const Fe10 f = fe10_sub(d, c), g = fe10_add(d, c);

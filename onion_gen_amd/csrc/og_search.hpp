@@ -8,17 +8,22 @@ namespace og {
 
 typedef const __attribute__((address_space(4))) uint32_t *cptr;
 
-// Occupancy of the search kernels, in waves per SIMD.  Left alone the compiler
-// spends about 300 VGPRs on the ten-limb products and runs one wave per SIMD;
-// held to two waves (256 VGPRs, a few spills) the kernel measured 20% faster.
-#ifndef OG_SEARCH_WAVES
-#define OG_SEARCH_WAVES 2
+// Occupancy of the search kernels, in waves per SIMD.  With each product
+// column summed as one dependent chain (fe10_mul) the key forms' loop fits 168
+// VGPRs and runs three waves per SIMD (measured 5.7% over two); the pattern
+// forms carry the automata's registers as well and stay at two (248 VGPRs),
+// where three would spill inside the loop.
+#ifndef OG_SEARCH_WAVES_KEY
+#define OG_SEARCH_WAVES_KEY 3
 #endif
-#if OG_SEARCH_WAVES > 0
-#define OG_SEARCH_ATTR __attribute__((amdgpu_waves_per_eu(OG_SEARCH_WAVES, OG_SEARCH_WAVES)))
-#else
-#define OG_SEARCH_ATTR
+#ifndef OG_SEARCH_WAVES_PATTERN
+#define OG_SEARCH_WAVES_PATTERN 2
 #endif
+template <uint32_t MODE>
+constexpr unsigned kSearchWaves =
+    (MODE & kModePattern) ? OG_SEARCH_WAVES_PATTERN : OG_SEARCH_WAVES_KEY;
+#define OG_SEARCH_ATTR(MODE) \
+  __attribute__((amdgpu_waves_per_eu(kSearchWaves<MODE>, kSearchWaves<MODE>)))
 
 __device__ __forceinline__ Fe10 ld10_const(cptr p) {
   Fe10 r;
@@ -29,27 +34,35 @@ __device__ __forceinline__ Fe10 ld10_const(cptr p) {
 
 // Element `plane` of a structure-of-arrays buffer of 10-limb values: limb l of
 // chain g sits at (plane*10 + l)*T + g, so a wave touches 64 consecutive words.
-// The plane base is wave-uniform and the in-plane index fits 32 bits, so each
-// access is a scalar base plus a 32-bit lane offset (no 64-bit address math
-// per load).
-__device__ __forceinline__ const uint32_t *soa_at(const uint32_t *base, size_t plane, uint32_t T,
-                                                  uint32_t g, int l) {
-  const char *pl = reinterpret_cast<const char *>(base + plane * 10 * T);
-  return reinterpret_cast<const uint32_t *>(pl + (uint32_t)((l * T + g) * 4u));
+// Each access is a raw buffer op: the plane base in a scalar descriptor (built
+// by SALU, wave-uniform), the limb offset l*T*4 as the scalar offset and g*4 as
+// the one lane offset, so no load or store spends a VALU slot on 64-bit
+// address arithmetic.  T is at most 2^22 (both engines refuse more), so every
+// in-plane offset stays below 40 * 2^22 bytes, inside the descriptor's range.
+constexpr int kBufWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format
+constexpr int kBufNT = 2;              // the nontemporal cache policy bit
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t soa_plane(const uint32_t *base, size_t plane,
+                                                            uint32_t T) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(base + plane * 10 * T), 0,
+                                           0x7fffffff, kBufWord3);
 }
 
 __device__ __forceinline__ Fe10 ld10(const uint32_t *base, size_t plane, uint32_t T, uint32_t g) {
-  Fe10 r;
+  const __amdgpu_buffer_rsrc_t r = soa_plane(base, plane, T);
+  Fe10 v;
 #pragma unroll
-  for (int l = 0; l < 10; ++l) r.v[l] = __builtin_nontemporal_load(soa_at(base, plane, T, g, l));
-  return r;
+  for (int l = 0; l < 10; ++l)
+    v.v[l] = __builtin_amdgcn_raw_buffer_load_b32(r, (int)(g * 4u), (int)(l * T * 4u), kBufNT);
+  return v;
 }
 
 __device__ __forceinline__ void st10(uint32_t *base, size_t plane, uint32_t T, uint32_t g,
                                      const Fe10 &v) {
+  const __amdgpu_buffer_rsrc_t r = soa_plane(base, plane, T);
 #pragma unroll
   for (int l = 0; l < 10; ++l)
-    __builtin_nontemporal_store(v.v[l], const_cast<uint32_t *>(soa_at(base, plane, T, g, l)));
+    __builtin_amdgcn_raw_buffer_store_b32(v.v[l], r, (int)(g * 4u), (int)(l * T * 4u), kBufNT);
 }
 
 __device__ __forceinline__ uint32_t bswap32(uint32_t x) { return __builtin_bswap32(x); }
@@ -405,7 +418,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
     Fe10 D = fe10_one();
     for (uint32_t m = 0; m < H; ++m) {
       const Fe10 dq = ld10_const(table + m * 30 + 20);
-      const Fe10 C = fe10_mul(P.T, dq);
+      const Fe10 C = fe10_mulc(P.T, dq);
       const Fe10 dp = fe10_sub(P.Z, C);
       const Fe10 dm = fe10_add(P.Z, C);
       st10(scratch, kPlanesPerPair * m, T, g, D);
@@ -458,8 +471,8 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
       }
       const Fe10 xq = ld10_const(table + m * 30);
       const Fe10 yq = ld10_const(table + m * 30 + 10);
-      const Fe10 A = fe10_mul(P.Y, yq);
-      const Fe10 B = fe10_mul(P.X, xq);
+      const Fe10 A = fe10_mulc(P.Y, yq);
+      const Fe10 B = fe10_mulc(P.X, xq);
       const Fe10 ym = fe10_mul(fe10_sub(A, B), idm);  // y(P - Q)
       const Fe10 yp = fe10_mul(fe10_add(A, B), idp);  // y(P + Q)
       const uint64_t km = kc - (m + 1), kp = kc + (m + 1);
