@@ -309,8 +309,9 @@ def main():
     ap.add_argument("--steps", type=int, default=0, help="0: cover 2^34 candidates")
     ap.add_argument("--warmup", type=int, default=2)
     # The shape the autotune walk (og_hip_engine_tune) picks for this workload on
-    # MI355X: 2^19 chains, 1024 pairs per window, one window per launch.
-    ap.add_argument("--threads", type=int, default=1 << 19)
+    # MI355X: 2^20 chains, 1024 pairs per window, one window per launch (128 GB
+    # of scratch planes: three 40-byte elements per pair).
+    ap.add_argument("--threads", type=int, default=1 << 20)
     ap.add_argument("--half", type=int, default=1024)
     ap.add_argument("--rounds", type=int, default=1)
     ap.add_argument("--no-dict", action="store_true",

@@ -87,7 +87,7 @@ int og_hip_engine_create_ex(int ordinal, const uint8_t start_xyzt[128], const og
 
 /* Replaces the automatic choice of Engine::new with retune = true
  * (src/gpu/mod.rs:354-412): times one launch of every shape of a walk over
- * chains {2^19 .. 2^16} x half {1024 .. 128} (rounds for about 2^28
+ * chains {2^20 .. 2^16} x half {1024 .. 128} (rounds for about 2^28
  * candidates a launch) with the given filters, records the fastest in the
  * tuning cache under device/<name>/hip/0/<version>, and writes it. */
 int og_hip_engine_tune(int ordinal, const uint8_t start_xyzt[128], const og_filterset *filters,

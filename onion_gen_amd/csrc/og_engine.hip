@@ -468,10 +468,11 @@ std::string tuning_key(int ordinal) {
 
 // The shapes the walk times, largest first (src/gpu/mod.rs:315-340 walks its
 // batch sizes down the same way): chains in flight times pairs per window,
-// with enough rounds for about 2^28 candidates per launch.
+// with enough rounds for about 2^28 candidates per launch.  The largest,
+// 2^20 chains of 1024 pairs, holds 128 GB of scratch planes.
 std::vector<og_engine_opts> tuning_walk() {
   std::vector<og_engine_opts> out;
-  for (uint32_t chains : {1u << 19, 1u << 18, 1u << 17, 1u << 16})
+  for (uint32_t chains : {1u << 20, 1u << 19, 1u << 18, 1u << 17, 1u << 16})
     for (uint32_t half : {1024u, 512u, 256u, 128u}) {
       og_engine_opts o{};
       o.threads = chains;

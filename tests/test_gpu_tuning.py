@@ -16,13 +16,13 @@ def say(msg):
 
 def test_walk_records_and_the_tuned_engine_uses_it(tmp_path, monkeypatch):
     monkeypatch.setenv("OG_TUNING_PATH", str(tmp_path / "tuning.yaml"))
-    say("case: tuning walk over 16 shapes, 4-symbol prefix")
+    say("case: tuning walk over 20 shapes, 4-symbol prefix")
     seed = workload.chacha_stream(0, 32)
     start = key.start_point(seed)
     fs = filters.FilterSet.parse("2xeq")
     best = gpu.Engine.tune(start, fs)
     say(f"best: {best}")
-    assert best["threads"] in (1 << 19, 1 << 18, 1 << 17, 1 << 16)
+    assert best["threads"] in (1 << 20, 1 << 19, 1 << 18, 1 << 17, 1 << 16)
     assert best["half"] in (1024, 512, 256, 128) and best["rounds"] >= 1
     assert best["rate"] > 1e9
     dev = gpu.look()[0]
