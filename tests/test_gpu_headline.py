@@ -1,0 +1,53 @@
+"""The default launch shape itself: 2^20 chains, 1024 pairs per window, one
+window per launch, 2,148,532,224 candidates.
+
+The oracle pins the small shapes bit for bit (test_gpu_parity); a CPU oracle
+over 2^31 candidates would take many minutes.  So the full-size launch is
+checked by shape invariance, which carries that pin to the full size: every
+hit of one full-size launch must be found, and only those, by a shape of 2^18
+chains and four rounds over the same candidates (for the pattern form, the
+address stage then drains in place instead of in its own kernel).  A sample of
+the hits is re-derived from its offset on the host.
+"""
+import hashlib
+
+import pytest
+
+from onion_gen_amd import filters, key
+from tests.test_gpu_parity import run_engine
+
+pytestmark = pytest.mark.gpu
+
+FULL = dict(threads=1 << 20, half=1024, rounds=1, hit_cap=1 << 20)
+SMALL = dict(threads=1 << 18, half=256, rounds=4, hit_cap=1 << 20)
+N = (1 << 20) * 2049
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def check(seed, line, expect_lo, expect_hi):
+    start = key.start_point(seed)
+    fs = filters.FilterSet.parse(line)
+    say(f"case: {line}, one launch at {FULL}")
+    full = run_engine(start, fs, N, **FULL)
+    say(f"case: {line}, the same candidates at {SMALL}")
+    small = run_engine(start, fs, N, **SMALL)
+    assert expect_lo < len(full) < expect_hi
+    assert full == small
+    for k, packed in full[:: max(1, len(full) // 500)]:
+        pk, addr = key.finish_hit(seed, 8 * k)
+        assert fs.match_address(key.address_bytes(pk)) >= 0, (k, addr)
+        assert pk[:31] == packed[:31] and (pk[31] & 0x7F) == packed[31]
+
+
+def test_full_launch_prefix():
+    # a 3-symbol prefix: 2^31 / 2^15 = 65,536 hits expected
+    check(hashlib.sha256(b"full prefix").digest(), "2xe", 60000, 71000)
+
+
+def test_full_launch_suffix():
+    # a 5-symbol suffix: symbol 55 is always d and q is one of the four
+    # possible symbols 54, so 2^31 / (2^15 * 4) = 16,384 hits expected
+    check(hashlib.sha256(b"full suffix").digest(), "suffix:2xaqd", 14500, 18500)
