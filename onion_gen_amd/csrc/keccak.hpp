@@ -123,6 +123,47 @@ __device__ __forceinline__ void ktheta(const KLane a[25], KLane d[5]) {
   for (int x = 0; x < 5; ++x) d[x] = kx(c[(x + 4) % 5], krotl<1>(c[(x + 1) % 5]));
 }
 
+// One round of Keccak-f[1600] on the split-lane state.
+__device__ __forceinline__ void kround(KLane a[25], uint64_t rc) {
+  KLane d[5], b[25];
+  ktheta(a, d);
+  b[0] = kx(a[0], d[0]);
+  b[10] = krotl<1>(kx(a[1], d[1]));
+  b[20] = krotl<62>(kx(a[2], d[2]));
+  b[5] = krotl<28>(kx(a[3], d[3]));
+  b[15] = krotl<27>(kx(a[4], d[4]));
+  b[16] = krotl<36>(kx(a[5], d[0]));
+  b[1] = krotl<44>(kx(a[6], d[1]));
+  b[11] = krotl<6>(kx(a[7], d[2]));
+  b[21] = krotl<55>(kx(a[8], d[3]));
+  b[6] = krotl<20>(kx(a[9], d[4]));
+  b[7] = krotl<3>(kx(a[10], d[0]));
+  b[17] = krotl<10>(kx(a[11], d[1]));
+  b[2] = krotl<43>(kx(a[12], d[2]));
+  b[12] = krotl<25>(kx(a[13], d[3]));
+  b[22] = krotl<39>(kx(a[14], d[4]));
+  b[23] = krotl<41>(kx(a[15], d[0]));
+  b[8] = krotl<45>(kx(a[16], d[1]));
+  b[18] = krotl<15>(kx(a[17], d[2]));
+  b[3] = krotl<21>(kx(a[18], d[3]));
+  b[13] = krotl<8>(kx(a[19], d[4]));
+  b[14] = krotl<18>(kx(a[20], d[0]));
+  b[24] = krotl<2>(kx(a[21], d[1]));
+  b[9] = krotl<61>(kx(a[22], d[2]));
+  b[19] = krotl<56>(kx(a[23], d[3]));
+  b[4] = krotl<14>(kx(a[24], d[4]));
+#pragma unroll
+  for (int y = 0; y < 25; y += 5) {
+#pragma unroll
+    for (int x = 0; x < 5; ++x) {
+      a[y + x].lo = chi3(b[y + x].lo, b[y + (x + 1) % 5].lo, b[y + (x + 2) % 5].lo);
+      a[y + x].hi = chi3(b[y + x].hi, b[y + (x + 1) % 5].hi, b[y + (x + 2) % 5].hi);
+    }
+  }
+  a[0].lo ^= (uint32_t)rc;
+  a[0].hi ^= (uint32_t)(rc >> 32);
+}
+
 __device__ __forceinline__ uint32_t onion_checksum_gfx950(const uint32_t pk[8]) {
   KLane a[25];
 #pragma unroll
@@ -138,47 +179,11 @@ __device__ __forceinline__ uint32_t onion_checksum_gfx950(const uint32_t pk[8]) 
   a[5] = {__builtin_amdgcn_alignbit(pk[7], pk[6], 8), (pk[7] >> 8) | (0x03u << 24)};
   a[6] = {0x06u, 0u};
   a[16] = {0u, 0x80000000u};
+  // Round 0 on its own: 17 of its lanes are zero and three constant, which
+  // the compiler folds (about 50 of the round's 190 instructions).
+  kround(a, keccak_rc(0));
 #pragma unroll 1
-  for (int round = 0; round < 23; ++round) {
-    KLane d[5], b[25];
-    ktheta(a, d);
-    b[0] = kx(a[0], d[0]);
-    b[10] = krotl<1>(kx(a[1], d[1]));
-    b[20] = krotl<62>(kx(a[2], d[2]));
-    b[5] = krotl<28>(kx(a[3], d[3]));
-    b[15] = krotl<27>(kx(a[4], d[4]));
-    b[16] = krotl<36>(kx(a[5], d[0]));
-    b[1] = krotl<44>(kx(a[6], d[1]));
-    b[11] = krotl<6>(kx(a[7], d[2]));
-    b[21] = krotl<55>(kx(a[8], d[3]));
-    b[6] = krotl<20>(kx(a[9], d[4]));
-    b[7] = krotl<3>(kx(a[10], d[0]));
-    b[17] = krotl<10>(kx(a[11], d[1]));
-    b[2] = krotl<43>(kx(a[12], d[2]));
-    b[12] = krotl<25>(kx(a[13], d[3]));
-    b[22] = krotl<39>(kx(a[14], d[4]));
-    b[23] = krotl<41>(kx(a[15], d[0]));
-    b[8] = krotl<45>(kx(a[16], d[1]));
-    b[18] = krotl<15>(kx(a[17], d[2]));
-    b[3] = krotl<21>(kx(a[18], d[3]));
-    b[13] = krotl<8>(kx(a[19], d[4]));
-    b[14] = krotl<18>(kx(a[20], d[0]));
-    b[24] = krotl<2>(kx(a[21], d[1]));
-    b[9] = krotl<61>(kx(a[22], d[2]));
-    b[19] = krotl<56>(kx(a[23], d[3]));
-    b[4] = krotl<14>(kx(a[24], d[4]));
-#pragma unroll
-    for (int y = 0; y < 25; y += 5) {
-#pragma unroll
-      for (int x = 0; x < 5; ++x) {
-        a[y + x].lo = chi3(b[y + x].lo, b[y + (x + 1) % 5].lo, b[y + (x + 2) % 5].lo);
-        a[y + x].hi = chi3(b[y + x].hi, b[y + (x + 1) % 5].hi, b[y + (x + 2) % 5].hi);
-      }
-    }
-    const uint64_t rc = keccak_rc(round);
-    a[0].lo ^= (uint32_t)rc;
-    a[0].hi ^= (uint32_t)(rc >> 32);
-  }
+  for (int round = 1; round < 23; ++round) kround(a, keccak_rc(round));
   // round 23, cut to the low half of lane 0: chi of b[0], b[1], b[2] and iota
   KLane d[5];
   ktheta(a, d);
