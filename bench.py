@@ -114,8 +114,11 @@ def batched_leg(seconds):
     for _, text, _ in mix:
         k = text.split(":")[0] if ":" in text else "prefix"
         kinds[k] = kinds.get(k, 0) + 1
-    batched = run_mix(mix, seconds, 64, 2048, 256, batch.LEAST_SERVED)
-    plain = run_mix(mix, seconds, 1, 2048, 256, batch.ROUND_ROBIN)
+    # the unit shape: 64 slots of 16384 chains, 256 pairs per window (2^20
+    # chains in all; a sweep of 2048..32768 chains and half 256..1024 ranked it
+    # first for this mix, at a median time to first hit under 0.1 s)
+    batched = run_mix(mix, seconds, 64, 16384, 256, batch.LEAST_SERVED)
+    plain = run_mix(mix, seconds, 1, 16384, 256, batch.ROUND_ROBIN)
     full = run_mix(mix, seconds, 1, 1 << 18, 256, batch.ROUND_ROBIN, ring_cap=1 << 21)
     return {
         "workload": "64 searches (search_mix seed 4: " + ", ".join(f"{v} {k}" for k, v in
@@ -139,7 +142,7 @@ def served_leg(seconds, rate=200.0):
     reqs = [(t, seed, text, want) for t, (seed, text, want) in zip(arrivals, mix)]
     keep = ("keys_per_s", "completed", "running", "queued", "rejected", "max_resident",
             "p50_time_to_n", "p99_time_to_n", "p50_time_to_first", "p99_time_to_first")
-    st, res = serve.run(reqs, duration=seconds, slots=64, chains=2048, half=256,
+    st, res = serve.run(reqs, duration=seconds, slots=64, chains=16384, half=256,
                         max_queue=1 << 16, policy=batch.LEAST_SERVED)
     cont = {k: st[k] for k in keep}
     with batch.Batch(slots=1, chains=1 << 18, half=256) as b:

@@ -278,7 +278,7 @@ typedef struct og_batch og_batch;
 
 typedef struct {
   uint32_t slots;    /* searches per step; 0 = 64 */
-  uint32_t chains;   /* chains per search, a multiple of 256; 0 = 2048 */
+  uint32_t chains;   /* chains per search, a multiple of 256; 0 = 16384 */
   uint32_t half;     /* pairs per window; 0 = 256 */
   uint32_t rounds;   /* windows per chain per step; 0 = 1 */
   uint32_t ring_cap; /* survivor records per step; 0 = 65536 */

@@ -32,7 +32,7 @@ class BatchHit:
 
 
 class Batch:
-    def __init__(self, slots=64, chains=2048, half=256, rounds=1, ring_cap=0,
+    def __init__(self, slots=64, chains=16384, half=256, rounds=1, ring_cap=0,
                  policy=LEAST_SERVED, device=0):
         h = ctypes.c_void_p()
         e = err_buf()

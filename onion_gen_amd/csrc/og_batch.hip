@@ -316,7 +316,7 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
   }
   og_batch_opts o = opts ? *opts : og_batch_opts{};
   if (!o.slots) o.slots = 64;
-  if (!o.chains) o.chains = 2048;
+  if (!o.chains) o.chains = 16384;
   if (!o.half) o.half = 256;
   if (!o.rounds) o.rounds = 1;
   if (!o.ring_cap) o.ring_cap = 1u << 16;

@@ -35,7 +35,7 @@ class Stats(ctypes.Structure):
 STATUS = {0: "not_arrived", 1: "queued", 2: "running", 3: "done", 4: "rejected"}
 
 
-def run(requests, duration, slots=64, chains=2048, half=256, rounds=1, max_queue=256,
+def run(requests, duration, slots=64, chains=16384, half=256, rounds=1, max_queue=256,
         device_budget=0, policy=LEAST_SERVED, ring_cap=1 << 20, device=0):
     """``requests``: list of (arrival_s, seed32, filter_text, want_hits), sorted
     by arrival.  Returns (stats dict, list of result dicts)."""
