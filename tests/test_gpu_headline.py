@@ -7,7 +7,8 @@ checked by shape invariance, which carries that pin to the full size: every
 hit of one full-size launch must be found, and only those, by a shape of 2^18
 chains and four rounds over the same candidates (for the pattern form, the
 address stage then drains in place instead of in its own kernel).  A sample of
-the hits is re-derived from its offset on the host.
+the hits is re-derived from its offset on the host.  The slot engine's default
+unit shape is checked the same way against the single engine.
 """
 import hashlib
 
@@ -51,3 +52,29 @@ def test_full_launch_suffix():
     # a 5-symbol suffix: symbol 55 is always d and q is one of the four
     # possible symbols 54, so 2^31 / (2^15 * 4) = 16,384 hits expected
     check(hashlib.sha256(b"full suffix").digest(), "suffix:2xaqd", 14500, 18500)
+
+
+def test_slot_engine_default_shape():
+    """The slot engine at its default unit shape (64 slots x 16384 chains x
+    256 pairs): a prefix search and a suffix search resident together, each
+    laid out over half the slots, report over their examined prefix [0, n)
+    exactly what the single engine reports over the same candidates."""
+    from onion_gen_amd import batch
+    cases = [(hashlib.sha256(b"slot prefix").digest(), "2xe"),
+             (hashlib.sha256(b"slot suffix").digest(), "suffix:2xaqd")]
+    say("case: slot engine 64 x 16384 x 256, a prefix and a suffix, 2 steps")
+    got = [[] for _ in cases]
+    with batch.Batch(slots=64, chains=16384, half=256, rounds=1, ring_cap=1 << 18) as b:
+        ids = [b.admit(key.start_point(seed), filters.FilterSet.parse(line), i)
+               for i, (seed, line) in enumerate(cases)]
+        for _ in range(2):
+            for h in b.step():
+                got[h.tag].append((h.offset // 8, h.packed))
+        examined = [b.search_state(i)["examined"] for i in ids]
+    for (seed, line), hits, n in zip(cases, got, examined):
+        assert n >= (1 << 29), (line, n)
+        say(f"case: {line} alone over the {n} candidates the slot engine examined")
+        alone = run_engine(key.start_point(seed), filters.FilterSet.parse(line), n,
+                           threads=1 << 18, half=256, rounds=1, hit_cap=1 << 20)
+        assert sorted(h for h in hits if h[0] < n) == alone and alone, (line, len(hits))
+        assert all(h[0] < n for h in hits)
