@@ -71,58 +71,67 @@ OG_HD void key_words(const Fe &c, uint32_t w[8]) {
   }
 }
 
-// The key pass.  `hit`: a placement inside symbols 0..48 matched.  `need`:
-// some placement reaching past 48 is still possible (or a truncated pattern
-// reached its last kept bit), so the full address decides.  `maybe`: an
-// expression proxy occurred, or may reach past 48.
+// The key pass of one bank.  `hit`: a placement inside symbols 0..48
+// matched.  `need`: some placement reaching past 48 is still possible (or a
+// truncated pattern reached its last kept bit), so the full address decides.
+// `maybe`: an expression proxy occurred, or may reach past 48.  Each flag is
+// or-ed in, so the banks of a set accumulate.
+OG_HD void scan_bank_key(const ScanBank &s, const uint32_t w[8], bool &hit, bool &need,
+                         bool &maybe) {
+  const uint32_t y248 = w[7] & 1u;  // stream bit 255: the top bit of symbol 51
+  const uint32_t js = s.jstart, iany = s.iany, fm = s.fmaybe;
+  uint32_t D = 0, acc = 0;
+  // A bank whose patterns all start past the key span skips the key pass.
+  // Otherwise every symbol runs, with the injection masked before jstart
+  // (no branch per symbol): D stays 0 until a pattern may start.
+  if (js < 49) {
+#pragma unroll
+    for (int j = 0; j < 49; ++j) {
+      const uint32_t inj = (j == 0 ? s.istart : 0u) | ((uint32_t)j >= js ? iany : 0u);
+      D = ((D << 1) | inj) & s.b[sym_at(w, j)];
+      acc |= D;
+    }
+  }
+  hit |= (acc & s.fany) != 0;
+  uint32_t e = acc & s.flong, m = acc;
+  const uint32_t t49 = sym_at(w, 49);  // sign bit clear: value 2 of the symbol
+  D = ((D << 1) | iany) & (s.b[t49] | s.b[t49 | 2u]);
+  e |= D & (s.fany | s.flong);
+  m |= D;
+  D = ((D << 1) | iany) & s.b[sym_at(w, 50)];
+  e |= D & (s.fany | s.flong);
+  m |= D;
+  D = ((D << 1) | iany) & s.u51[y248];
+  e |= D & (s.fany | s.flong);
+  m |= D;
+  D = ((D << 1) | iany) & s.uall;
+  e |= D & (s.fany | s.flong);
+  m |= D;
+  D = ((D << 1) | iany) & s.uall;
+  e |= D & (s.fany | s.flong);
+  m |= D;
+  D = ((D << 1) | iany) & s.u54;
+  e |= D & (s.fany | s.flong);
+  m |= D;
+  D = ((D << 1) | iany) & s.b[3];  // 'd'
+  e |= D & (s.fany | s.flong | s.fend);
+  m |= D;
+  need |= e != 0;
+  maybe |= (m & fm) != 0;
+}
+
+// The key pass over every bank of a set.  The first bank runs on its own:
+// its table address is then a constant, so every symbol's read takes the
+// table offset as an immediate instead of an address add (a set of one bank
+// is the common case).
 OG_HD void scan_key(const uint32_t w[8], const ScanBank *banks, uint32_t nbanks, bool &hit,
                     bool &need, bool &maybe) {
   hit = false;
   need = false;
   maybe = false;
-  const uint32_t y248 = w[7] & 1u;  // stream bit 255: the top bit of symbol 51
-  for (uint32_t k = 0; k < nbanks; ++k) {
-    const ScanBank &s = banks[k];
-    const uint32_t js = s.jstart, iany = s.iany, fm = s.fmaybe;
-    uint32_t D = 0, acc = 0;
-    // A bank whose patterns all start past the key span skips the key pass.
-    // Otherwise every symbol runs, with the injection masked before jstart
-    // (no branch per symbol): D stays 0 until a pattern may start.
-    if (js < 49) {
-#pragma unroll
-      for (int j = 0; j < 49; ++j) {
-        const uint32_t inj = (j == 0 ? s.istart : 0u) | ((uint32_t)j >= js ? iany : 0u);
-        D = ((D << 1) | inj) & s.b[sym_at(w, j)];
-        acc |= D;
-      }
-    }
-    hit |= (acc & s.fany) != 0;
-    uint32_t e = acc & s.flong, m = acc;
-    const uint32_t t49 = sym_at(w, 49);  // sign bit clear: value 2 of the symbol
-    D = ((D << 1) | iany) & (s.b[t49] | s.b[t49 | 2u]);
-    e |= D & (s.fany | s.flong);
-    m |= D;
-    D = ((D << 1) | iany) & s.b[sym_at(w, 50)];
-    e |= D & (s.fany | s.flong);
-    m |= D;
-    D = ((D << 1) | iany) & s.u51[y248];
-    e |= D & (s.fany | s.flong);
-    m |= D;
-    D = ((D << 1) | iany) & s.uall;
-    e |= D & (s.fany | s.flong);
-    m |= D;
-    D = ((D << 1) | iany) & s.uall;
-    e |= D & (s.fany | s.flong);
-    m |= D;
-    D = ((D << 1) | iany) & s.u54;
-    e |= D & (s.fany | s.flong);
-    m |= D;
-    D = ((D << 1) | iany) & s.b[3];  // 'd'
-    e |= D & (s.fany | s.flong | s.fend);
-    m |= D;
-    need |= e != 0;
-    maybe |= (m & fm) != 0;
-  }
+  if (nbanks == 0) return;
+  scan_bank_key(banks[0], w, hit, need, maybe);
+  for (uint32_t k = 1; k < nbanks; ++k) scan_bank_key(banks[k], w, hit, need, maybe);
 }
 
 // The exact pass over the 35-byte address as nine big-endian words (the last
