@@ -217,8 +217,14 @@ OG_HD uint32_t fe10_low32(const Fe10 &a) { return a.v[0] + (a.v[1] << 26); }
 // The canonical representative below p as eight 32-bit words.  q = [x >= p]
 // comes from propagating x + 19 through the limbs; then x + 19q - 2^255 q is
 // carried out exactly and packed (the technique of ref10's fe_tobytes).
-OG_HD Fe fe10_canon(const Fe10 &a) {
-  Fe10 t = fe10_carry(a);
+//
+// fe10_canon_product takes a value whose limbs are below 2^31 and whose
+// integer is below 2p, with no carry pass first: the output of fe10_mul (below
+// 2^255 + 2^44; only limb 1 can exceed its width, by at most 2^18).  The ripple
+// for q then carries exactly whatever the limbs hold, q = floor((x + 19) /
+// 2^255) is still [x >= p], and the final ripple normalises every limb.
+// fe10_canon tightens an arbitrary loose value first.
+OG_HD Fe fe10_canon_product(const Fe10 &t) {
   uint32_t q = (t.v[0] + 19u) >> 26;
 #pragma unroll
   for (int i = 1; i < 10; ++i) q = (t.v[i] + q) >> fe10_width(i);
@@ -245,9 +251,12 @@ OG_HD Fe fe10_canon(const Fe10 &a) {
   return r;
 }
 
-// From any 8-word value below 2^256 (canonicalised first).
-OG_HD Fe10 fe10_from(const Fe &x) {
-  const Fe c = fe_canon(x);
+OG_HD Fe fe10_canon(const Fe10 &a) { return fe10_canon_product(fe10_carry(a)); }
+
+// The limbs of x mod 2^255 as they stand, with no reduction mod p: a value in
+// [p, 2^255) keeps its non-canonical form (the field tests feed these to
+// fe10_canon_product).
+OG_HD Fe10 fe10_split(const Fe &c) {
   Fe10 r;
   r.v[0] = c.v[0] & 0x3ffffff;
   r.v[1] = ((c.v[0] >> 26) | (c.v[1] << 6)) & 0x1ffffff;
@@ -261,6 +270,9 @@ OG_HD Fe10 fe10_from(const Fe &x) {
   r.v[9] = (c.v[7] >> 6) & 0x1ffffff;
   return r;
 }
+
+// From any 8-word value below 2^256 (canonicalised first).
+OG_HD Fe10 fe10_from(const Fe &x) { return fe10_split(fe_canon(x)); }
 
 OG_HD Fe fe10_to(const Fe10 &a) { return fe10_canon(a); }
 

@@ -107,9 +107,14 @@ __host__ __device__ inline void field_op_one(int op, const uint32_t *a, const ui
       case 14: r10 = fe10_invert(x10); break;
       // loose inputs at the bounds the search reaches: (2x - y) * (x - y)
       case 16: r10 = fe10_mul(fe10_sub(fe10_add(x10, x10), y10), fe10_sub(x10, y10)); break;
+      // a product's output canonicalised as the search kernels do it: x times
+      // one with x mod 2^255 split as it stands (values in [p, 2^255) stay
+      // above p), and x times y
+      case 17: r = fe10_canon_product(fe10_mul(fe10_split(x), fe10_one())); break;
+      case 18: r = fe10_canon_product(fe10_mul(x10, y10)); break;
       default: r10 = x10; break;
     }
-    r = fe10_canon(r10);
+    if (op != 17 && op != 18) r = fe10_canon(r10);
   }
   for (int l = 0; l < 8; ++l) out[l] = r.v[l];
 }
@@ -714,8 +719,8 @@ int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
   return 0;
 }
 
-// ops 0..5, 10..16 and 20, as include/onion_gen_hip.h lists them
-static bool field_op_known(int op) { return (op >= 0 && op <= 5) || (op >= 10 && op <= 16) || op == 20; }
+// ops 0..5, 10..18 and 20, as include/onion_gen_hip.h lists them
+static bool field_op_known(int op) { return (op >= 0 && op <= 5) || (op >= 10 && op <= 18) || op == 20; }
 
 int og_field_op_host(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, size_t n) {
   if (!field_op_known(op) || !a || !b || !out) return -1;

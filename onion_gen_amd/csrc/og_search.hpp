@@ -248,7 +248,7 @@ __device__ __forceinline__ void test_key_only(const Fe10 &y, uint64_t k, const D
   bool hit = false;
   Fe c;
   if (maybe) {
-    c = fe10_canon(y);
+    c = fe10_canon_product(y);
     hit = exact_key<MODE>(c, f);
     if (f.key_proxy && hit) {
       // an expression's prefix: its automata settle the key's 49 symbols
@@ -343,7 +343,7 @@ __device__ __forceinline__ void park(uint32_t *scratch, uint32_t top, uint32_t T
   if (want) {
     const uint32_t lane = __lane_id();
     const uint32_t q = Q + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
-    const uint32_t sign = fe10_canon(x).v[0] & 1u;
+    const uint32_t sign = fe10_canon_product(x).v[0] & 1u;
     Fe10 e;
 #pragma unroll
     for (int l = 0; l < 8; ++l) e.v[l] = c.v[l];
@@ -436,7 +436,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
       const Fe10 iz = fe10_mul(inv, D);
       const Fe10 y = fe10_mul(P.Y, iz);
       if (kPat) {
-        const Fe c = fe10_canon(y);
+        const Fe c = fe10_canon_product(y);
         uint32_t kw[8];
         key_words(c, kw);
         bool hit, need;
@@ -477,7 +477,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
       const Fe10 yp = fe10_mul(fe10_add(A, B), idp);  // y(P + Q)
       const uint64_t km = kc - (m + 1), kp = kc + (m + 1);
       if (kPat) {
-        const Fe cm = fe10_canon(ym), cp = fe10_canon(yp);
+        const Fe cm = fe10_canon_product(ym), cp = fe10_canon_product(yp);
         uint32_t wm[8], wp[8];
         key_words(cm, wm);
         key_words(cp, wp);

@@ -95,6 +95,12 @@ def test_field_arithmetic_both_forms_on_the_host():
         nz = [x for x in a if x % P][:150]
         assert host_field_op(base + 4, nz, nz) == [pow(x, P - 2, P) for x in nz]
     assert host_field_op(16, a, b) == [(x - y) * (2 * x - y) % P for x, y in zip(a, b)]
+    # a product's output canonicalised without a carry pass first: values in
+    # [p, 2^255) kept as they stand (op 17), and products (op 18)
+    above = [P, P + 1, P + 18, 2 ** 255 - 1, 2 ** 255 + P, 2 ** 256 - 1]
+    a17 = edges + above + [rng.getrandbits(256) for _ in range(500)]
+    assert host_field_op(17, a17, a17) == [(x % 2 ** 255) % P for x in a17]
+    assert host_field_op(18, a, b) == [x * y % P for x, y in zip(a, b)]
     # the largest limbs a tight value can carry: all-ones limbs
     big = [sum(((1 << (26 if i % 2 == 0 else 25)) - 1) << [0, 26, 51, 77, 102, 128, 153, 179, 204, 230][i]
                for i in range(10))]
