@@ -10,9 +10,10 @@ typedef const __attribute__((address_space(4))) uint32_t *cptr;
 
 // Occupancy of the search kernels, in waves per SIMD.  With each product
 // column summed as one dependent chain (fe10_mul) the key forms' loop fits 168
-// VGPRs and runs three waves per SIMD (measured 5.7% over two); the pattern
-// forms carry the automata's registers as well and stay at two (248 VGPRs),
-// where three would spill inside the loop.
+// VGPRs and runs three waves per SIMD (measured 2% over two with the same
+// products); the pattern forms carry the automata's registers as well and stay
+// at two (256 VGPRs), where three spill inside the loop and measured 5-25%
+// slower.
 #ifndef OG_SEARCH_WAVES_KEY
 #define OG_SEARCH_WAVES_KEY 3
 #endif
