@@ -137,7 +137,9 @@ void og_hip_engine_destroy(og_engine *e);
  * the output); op 16 is (2x - y)(x - y) in the 10-limb
  * form, the loosest operands the search multiplies; ops 17 and 18 canonicalise
  * a product's output as the search kernels do (17: x mod 2^255, split into
- * limbs without reduction, times one; 18: x times y).  Inputs are n raw 32-byte little-endian
+ * limbs without reduction, times one; 18: x times y) and op 19 is bit 0 of the
+ * canonical (x mod 2^255, split the same way) times y, as the search takes the
+ * sign of x.  Inputs are n raw 32-byte little-endian
  * words (any value below 2^256); outputs are canonical encodings.  The _host
  * variant runs the same code on the CPU. */
 int og_hip_field_op(int ordinal, int op, const uint8_t *a, const uint8_t *b, uint8_t *out,

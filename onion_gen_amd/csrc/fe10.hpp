@@ -253,6 +253,16 @@ OG_HD Fe fe10_canon_product(const Fe10 &t) {
 
 OG_HD Fe fe10_canon(const Fe10 &a) { return fe10_canon_product(fe10_carry(a)); }
 
+// Bit 0 of the canonical form of a product's output (the sign of x in a
+// packed key), without forming it: the canonical value is x - qp with p odd,
+// so its parity is that of x (bit 0 of limb 0, which no carry reaches) xor q.
+OG_HD uint32_t fe10_parity_product(const Fe10 &t) {
+  uint32_t q = (t.v[0] + 19u) >> 26;
+#pragma unroll
+  for (int i = 1; i < 10; ++i) q = (t.v[i] + q) >> fe10_width(i);
+  return (t.v[0] ^ q) & 1u;
+}
+
 // The limbs of x mod 2^255 as they stand, with no reduction mod p: a value in
 // [p, 2^255) keeps its non-canonical form (the field tests feed these to
 // fe10_canon_product).

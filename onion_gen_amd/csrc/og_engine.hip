@@ -112,9 +112,12 @@ __host__ __device__ inline void field_op_one(int op, const uint32_t *a, const ui
       // above p), and x times y
       case 17: r = fe10_canon_product(fe10_mul(fe10_split(x), fe10_one())); break;
       case 18: r = fe10_canon_product(fe10_mul(x10, y10)); break;
+      // bit 0 of the canonical product, as the search takes the sign of x
+      // (x mod 2^255 split as it stands, as for op 17)
+      case 19: r = fe_zero(); r.v[0] = fe10_parity_product(fe10_mul(fe10_split(x), y10)); break;
       default: r10 = x10; break;
     }
-    if (op != 17 && op != 18) r = fe10_canon(r10);
+    if (op < 17) r = fe10_canon(r10);
   }
   for (int l = 0; l < 8; ++l) out[l] = r.v[l];
 }
@@ -719,8 +722,8 @@ int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
   return 0;
 }
 
-// ops 0..5, 10..18 and 20, as include/onion_gen_hip.h lists them
-static bool field_op_known(int op) { return (op >= 0 && op <= 5) || (op >= 10 && op <= 18) || op == 20; }
+// ops 0..5, 10..20, as include/onion_gen_hip.h lists them
+static bool field_op_known(int op) { return (op >= 0 && op <= 5) || (op >= 10 && op <= 20); }
 
 int og_field_op_host(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, size_t n) {
   if (!field_op_known(op) || !a || !b || !out) return -1;

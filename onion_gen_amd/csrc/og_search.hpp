@@ -344,7 +344,7 @@ __device__ __forceinline__ void park(uint32_t *scratch, uint32_t top, uint32_t T
   if (want) {
     const uint32_t lane = __lane_id();
     const uint32_t q = Q + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
-    const uint32_t sign = fe10_canon_product(x).v[0] & 1u;
+    const uint32_t sign = fe10_parity_product(x);
     Fe10 e;
 #pragma unroll
     for (int l = 0; l < 8; ++l) e.v[l] = c.v[l];

@@ -93,6 +93,8 @@ def test_field_kernels_match_big_integers():
     above = [P, P + 1, P + 18, 2 ** 255 - 1, 2 ** 255 + P, 2 ** 256 - 1]
     assert field_op(17, a + above, a + above) == [(x % 2 ** 255) % P for x in a + above]
     assert field_op(18, a, b) == [x * y % P for x, y in zip(a, b)]
+    assert field_op(19, a, b) == [(x % 2 ** 255) * y % P & 1 for x, y in zip(a, b)]
+    assert field_op(19, above, [1] * len(above)) == [(x % 2 ** 255) % P & 1 for x in above]
 
 
 def test_device_checksum_matches_sha3():

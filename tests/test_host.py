@@ -101,6 +101,9 @@ def test_field_arithmetic_both_forms_on_the_host():
     a17 = edges + above + [rng.getrandbits(256) for _ in range(500)]
     assert host_field_op(17, a17, a17) == [(x % 2 ** 255) % P for x in a17]
     assert host_field_op(18, a, b) == [x * y % P for x, y in zip(a, b)]
+    assert host_field_op(19, a, b) == [(x % 2 ** 255) * y % P & 1 for x, y in zip(a, b)]
+    # products that land on [p, 2^255) before reduction: x times one
+    assert host_field_op(19, above, [1] * len(above)) == [(x % 2 ** 255) % P & 1 for x in above]
     # the largest limbs a tight value can carry: all-ones limbs
     big = [sum(((1 << (26 if i % 2 == 0 else 25)) - 1) << [0, 26, 51, 77, 102, 128, 153, 179, 204, 230][i]
                for i in range(10))]
