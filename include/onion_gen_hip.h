@@ -75,6 +75,12 @@ typedef struct {
   uint32_t half;    /* pairs per window: one window is 2*half+1 candidates */
   uint32_t rounds;  /* windows per chain per advance */
   uint32_t hit_cap; /* survivor room per advance; 0 = og_hit_room */
+  uint32_t sign_stage; /* general forms: where the sign of x is formed for the
+                          candidates the key alone cannot settle.  0: chosen
+                          from the filters (the address kernel when under 1 key
+                          in 8 needs it), 1: in the search loop, 2: in the
+                          address kernel.  Launches of more than one round, or
+                          of half below 8, always form it in the loop. */
 } og_engine_opts;
 
 /* The extended constructor: the filter set compiled by og_filters_parse,
@@ -95,6 +101,11 @@ int og_hip_engine_tune(int ordinal, const uint8_t start_xyzt[128], const og_filt
 
 /* The shape an engine runs: chains, pairs per window, windows per launch. */
 void og_hip_engine_shape(const og_engine *e, uint32_t *threads, uint32_t *half, uint32_t *rounds);
+
+/* Where an engine forms the sign of x (og_engine_opts::sign_stage): 1 in the
+ * search loop, 2 in the address kernel; -1 for a null engine.  No counterpart
+ * in the reference, whose CPU path forms it per candidate (src/batch.rs). */
+int og_hip_engine_sign_stage(const og_engine *e);
 
 /* The device ordinal an engine runs on. */
 int og_hip_engine_device(const og_engine *e);

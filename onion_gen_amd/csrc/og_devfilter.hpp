@@ -349,7 +349,43 @@ struct DevFilterHost {
     a.key_proxy = key_proxy ? 1u : 0u;
     a.regex = d_regex;
     a.dfa = d_dfa;
+    a.defer_sign = 0;
     return a;
+  }
+
+  // The fraction of keys whose key stage leaves a checksum placement possible
+  // (the lanes that park), over `samples` pseudo-random keys: the key stage of
+  // the search kernels run on the host (the general forms and expressions).
+  double need_rate(uint32_t samples) const {
+    if (!(mode & kModePattern) || samples == 0) return 0.0;
+    const uint32_t nb = (uint32_t)banks.size(), nre = (uint32_t)(regex.size() / 3);
+    const uint32_t K = !regex.empty() && first_states <= kLdsStates ? first_states : 0u;
+    uint64_t st = 0x243f6a8885a308d3ull;
+    uint32_t need = 0;
+    for (uint32_t i = 0; i < samples; ++i) {
+      uint32_t kw[8];
+      for (int l = 0; l < 8; ++l) {
+        st += 0x9e3779b97f4a7c15ull;  // splitmix64
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        kw[l] = (uint32_t)(z ^ (z >> 31));
+      }
+      kw[7] &= ~0x80u;  // the sign bit of the packed key is clear in the key stage
+      bool kh, nd, maybe;
+      scan_key(kw, banks.data(), nb, kh, nd, maybe);
+      if (nre) {
+        maybe = (maybe || regex_every) && !kh;
+        if (maybe) {
+          const uint32_t r = regex_key(kw, regex.data(), dfa.data(), live.data(), nre,
+                                       dfa.data() + regex[0], K);
+          kh |= r == 2;
+          nd |= r == 1;
+        }
+      }
+      need += nd && !kh;
+    }
+    return (double)need / samples;
   }
 };
 

@@ -50,17 +50,25 @@ __global__ __launch_bounds__(256) OG_SEARCH_ATTR(MODE) void og_search_kernel(Sea
 }
 
 // The deferred address stage of a one-round pattern launch: per chain, the
-// candidates the key stage parked (sign of x, SHA3-256, the exact automaton
-// pass).  Without the search loop's registers it runs at four waves per SIMD
-// instead of two.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void
+// candidates the key stage parked (SHA3-256, the exact automaton pass, and
+// with kSign the sign of x first).  Without the search loop's registers it
+// runs at four waves per SIMD instead of two; the form that derives the signs
+// (products, an inversion per chunk) is its own instantiation, held to two
+// waves so that it does not spill, and the plain one keeps its registers.
+template <bool kSign>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSign ? 2 : 4, 8))) void
 og_address_kernel(SearchArgs a) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ ScanBank banks[kMaxBanks];
   __shared__ uint16_t dfa_lds[kLdsStates * 32];
   load_filter(banks, dfa_lds, a.f);
   HitSink sink{a.counter, a.hits, a.cap, 0, nullptr, 0};
-  address_stage(a.scratch, a.half, a.threads, g, a.parked[g], a.f, banks, sink);
+  const uint32_t top = kPlanesPerPair * a.half;
+  if (kSign)
+    drain_deferred(a.scratch, top, a.half, a.threads, g, a.parked[g], a.k_base, (cptr)a.table,
+                   a.f, banks, sink);
+  else
+    drain<kModePattern>(a.scratch, top, a.threads, g, a.parked[g], a.f, banks, sink);
 }
 
 __global__ __launch_bounds__(256) void og_setup_kernel(SetupArgs a) {
@@ -246,8 +254,14 @@ template <uint32_t MODE>
 hipError_t launch_search(og_engine *e) {
   const uint32_t grid = e->threads / e->block;
   hipLaunchKernelGGL(og_search_kernel<MODE>, dim3(grid), dim3(e->block), 0, e->stream, e->args);
-  if ((MODE & kModePattern) && e->args.parked)
-    hipLaunchKernelGGL(og_address_kernel, dim3(grid), dim3(e->block), 0, e->stream, e->args);
+  if ((MODE & kModePattern) && e->args.parked) {
+    if (e->args.f.defer_sign)
+      hipLaunchKernelGGL(og_address_kernel<true>, dim3(grid), dim3(e->block), 0, e->stream,
+                         e->args);
+    else
+      hipLaunchKernelGGL(og_address_kernel<false>, dim3(grid), dim3(e->block), 0, e->stream,
+                         e->args);
+  }
   return hipGetLastError();
 }
 
@@ -406,10 +420,14 @@ int create_engine(int ordinal, const uint8_t start_xyzt[128], const og_engine_op
   a.rounds = o.rounds;
   a.f = e->filt.args();
   a.counter = e->d_counter;
-  // A one-round pattern launch leaves its address stage to its own kernel.
+  // A one-round pattern launch leaves its address stage to its own kernel,
+  // and, unless many lanes park (the key stage run on sample keys), the sign
+  // of x as well: then the search loop never forms x.
   if ((e->filt.mode & kModePattern) && o.rounds == 1) {
     OG_TRY(hipMalloc(&e->d_parked, (size_t)o.threads * 4));
     a.parked = e->d_parked;
+    if (o.half >= 8 && o.sign_stage != 1)
+      a.f.defer_sign = o.sign_stage == 2 || e->filt.need_rate(4096) < 0.125 ? 1u : 0u;
   } else {
     a.parked = nullptr;
   }
@@ -550,7 +568,10 @@ int create_tuned(int ordinal, const uint8_t start_xyzt[128], const og_engine_opt
                          err, errcap);
   if (check_device(ordinal, err, errcap)) return -1;
   og_engine_opts o{};
-  if (opts) o.hit_cap = opts->hit_cap;
+  if (opts) {
+    o.hit_cap = opts->hit_cap;
+    o.sign_stage = opts->sign_stage;
+  }
   og_tuning_shape s{};
   if (og_tuning_get(tuning_key(ordinal).c_str(), &s) == 1 && s.first && s.second) {
     o.threads = s.second;
@@ -608,6 +629,11 @@ void og_hip_engine_shape(const og_engine *e, uint32_t *threads, uint32_t *half,
   if (threads) *threads = e ? e->threads : 0;
   if (half) *half = e ? e->half : 0;
   if (rounds) *rounds = e ? e->rounds : 0;
+}
+
+int og_hip_engine_sign_stage(const og_engine *e) {
+  if (!e) return -1;
+  return e->args.parked && e->args.f.defer_sign ? 2 : 1;
 }
 
 uint32_t og_hip_engine_threads(const og_engine *e) { return e ? e->threads : 0; }

@@ -87,6 +87,9 @@ struct DevFilterArgs {
   const uint32_t *regex_live;
   uint32_t lds_states;            //   the first automaton's states, when it is
                                   //   copied to LDS whole (0: it is not)
+  uint32_t defer_sign;            // (kModePattern, one-round launches) the key
+                                  //   stage parks candidates without the sign
+                                  //   of x, and the address kernel derives it
 };
 
 struct SearchArgs {

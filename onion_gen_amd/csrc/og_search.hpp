@@ -337,14 +337,14 @@ __device__ __forceinline__ void key_stage_pair(const Fe &ca, const uint32_t wa[8
 // ... of its own column.  After the centre and pairs H-1..m the wave has at
 // most 64(2(H-m)+1) entries, in at most 2(H-m)+1 planes, and
 // 3H - 2(H-m) >= 3m, so an entry never reaches a plane the backward pass has
-// yet to read.  `mask` is the wave's ballot of `want`.
+// yet to read.  `mask` is the wave's ballot of `want`.  `sign` is the sign of
+// x, or 0 when the address kernel is to derive it (DevFilterArgs::defer_sign).
 __device__ __forceinline__ void park(uint32_t *scratch, uint32_t top, uint32_t T, uint32_t g,
                                      uint32_t &Q, uint64_t mask, bool want, const Fe &c,
-                                     const Fe10 &x, uint64_t k) {
+                                     uint32_t sign, uint64_t k) {
   if (want) {
     const uint32_t lane = __lane_id();
     const uint32_t q = Q + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
-    const uint32_t sign = fe10_parity_product(x);
     Fe10 e;
 #pragma unroll
     for (int l = 0; l < 8; ++l) e.v[l] = c.v[l];
@@ -362,32 +362,105 @@ __device__ __forceinline__ uint32_t parked_share(uint32_t Q) {
   return Q > lane ? (Q - lane + 63u) >> 6 : 0u;
 }
 
+// The address stage of one parked entry: packed key (sign in bit 255) and index.
+__device__ __forceinline__ void drain_one(const Fe10 &e, const DevFilterArgs &f,
+                                          const ScanBank *banks, const HitSink &sink) {
+  uint32_t pk[8];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) pk[l] = e.v[l];
+  uint32_t w[9];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) w[l] = __builtin_bswap32(pk[l]);  // big-endian, sign included
+  const uint32_t chk = onion_checksum(pk);
+  w[8] = ((chk & 0xffu) << 24) | (((chk >> 8) & 0xffu) << 16) | (0x03u << 8);
+  const uint32_t r = scan_address(w, banks, f.nbanks);
+  const bool hit =
+      r == 1 ||
+      (r == 2 && pattern_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8], f.pats,
+                              f.npat)) ||
+      (f.nregex && regex_address_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8],
+                                      f.regex, f.dfa, f.nregex));
+  Fe c;
+#pragma unroll
+  for (int l = 0; l < 8; ++l) c.v[l] = pk[l];
+  c.v[7] &= 0x7fffffffu;  // survivors carry the key with the sign clear
+  report(hit, (uint64_t)e.v[8] | ((uint64_t)e.v[9] << 32), c, sink);
+}
+
 template <uint32_t MODE>
 __device__ __forceinline__ void drain(const uint32_t *scratch, uint32_t top, uint32_t T,
                                       uint32_t g, uint32_t n, const DevFilterArgs &f,
                                       const ScanBank *banks, const HitSink &sink) {
-  for (uint32_t j = 0; j < n; ++j) {
-    const Fe10 e = ld10(scratch, top - j, T, g);  // j is uniform over the active lanes
-    uint32_t pk[8];
-#pragma unroll
-    for (int l = 0; l < 8; ++l) pk[l] = e.v[l];
-    uint32_t w[9];
-#pragma unroll
-    for (int l = 0; l < 8; ++l) w[l] = __builtin_bswap32(pk[l]);  // big-endian, sign included
-    const uint32_t chk = onion_checksum(pk);
-    w[8] = ((chk & 0xffu) << 24) | (((chk >> 8) & 0xffu) << 16) | (0x03u << 8);
-    const uint32_t r = scan_address(w, banks, f.nbanks);
-    const bool hit =
-        r == 1 ||
-        (r == 2 && pattern_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8], f.pats,
-                                f.npat)) ||
-        (f.nregex && regex_address_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8],
-                                        f.regex, f.dfa, f.nregex));
-    Fe c;
-#pragma unroll
-    for (int l = 0; l < 8; ++l) c.v[l] = pk[l];
-    c.v[7] &= 0x7fffffffu;  // survivors carry the key with the sign clear
-    report(hit, (uint64_t)e.v[8] | ((uint64_t)e.v[9] << 32), c, sink);
+  for (uint32_t j = 0; j < n; ++j)
+    drain_one(ld10(scratch, top - j, T, g), f, banks, sink);  // j is uniform over the active lanes
+}
+
+// The sign of x for parked entries that left it out (defer_sign), then the
+// address stage of each.  Parking is compacted per wave, so a lane drains
+// entries of any chain of its wave: an entry's index k names its chain c
+// ((k - k_base) / (2H+1) in a one-round launch), whose window centre P sits
+// in planes 0..3 of column c, and the candidate: P - Q_m (k = kc - m - 1) has
+// x = (X yq - Y xq) / (Z - T dq), P + Q_m has x = (X yq + Y xq) / (Z + T dq),
+// the centre kc has x = X / Z.  The denominators of up to half - 4 entries at
+// a time share one inversion (Montgomery's trick), their prefix products held
+// in planes 4.. of the lane's own column (the backward pass has consumed every
+// plane below the parking planes, which start at plane half).
+__device__ __forceinline__ void sign_terms(const Fe10 &e, const uint32_t *scratch, uint32_t T,
+                                           uint64_t k_base, uint32_t half, cptr table, Fe10 &num,
+                                           Fe10 &den) {
+  const uint64_t k = (uint64_t)e.v[8] | ((uint64_t)e.v[9] << 32);
+  const uint64_t S = 2ull * half + 1;
+  uint64_t c = (k - k_base) / S;
+  c = c < T ? c : T - 1u;  // an entry's chain is below T; this only bounds the reads
+  const uint64_t kc = k_base + c * S + half;
+  Point10 P;
+  P.X = ld10(scratch, 0, T, (uint32_t)c);
+  P.Y = ld10(scratch, 1, T, (uint32_t)c);
+  P.Z = ld10(scratch, 2, T, (uint32_t)c);
+  P.T = ld10(scratch, 3, T, (uint32_t)c);
+  if (k == kc) {
+    num = P.X;
+    den = P.Z;
+    return;
+  }
+  const bool minus = k < kc;
+  // an entry's index is within half of its centre; the clamp only keeps the
+  // table read in bounds
+  uint32_t m = (uint32_t)(minus ? kc - k : k - kc) - 1u;
+  m = m < half ? m : half - 1u;
+  const Fe10 xq = ld10_const(table + m * 30), yq = ld10_const(table + m * 30 + 10);
+  const Fe10 dq = ld10_const(table + m * 30 + 20);
+  const Fe10 C = fe10_mul(P.T, dq);
+  const Fe10 XY = fe10_mul(P.X, yq), YX = fe10_mul(P.Y, xq);
+  num = minus ? fe10_sub(XY, YX) : fe10_add(XY, YX);
+  den = minus ? fe10_sub(P.Z, C) : fe10_add(P.Z, C);
+}
+
+__device__ __forceinline__ void drain_deferred(uint32_t *scratch, uint32_t top, uint32_t half,
+                                               uint32_t T, uint32_t g, uint32_t n,
+                                               uint64_t k_base, cptr table,
+                                               const DevFilterArgs &f, const ScanBank *banks,
+                                               const HitSink &sink) {
+  const uint32_t K = half - 4;  // entries per inversion
+  for (uint32_t j0 = 0; j0 < n; j0 += K) {
+    const uint32_t cnt = n - j0 < K ? n - j0 : K;
+    Fe10 D = fe10_one();
+    for (uint32_t i = 0; i < cnt; ++i) {
+      st10(scratch, 4 + i, T, g, D);  // the product of the denominators before entry i
+      Fe10 num, den;
+      sign_terms(ld10(scratch, top - (j0 + i), T, g), scratch, T, k_base, half, table, num, den);
+      D = fe10_mul(den, D);
+    }
+    Fe10 inv = fe10_invert(D);
+    for (uint32_t i = cnt; i-- > 0;) {
+      Fe10 e = ld10(scratch, top - (j0 + i), T, g);
+      Fe10 num, den;
+      sign_terms(e, scratch, T, k_base, half, table, num, den);
+      const Fe10 x = fe10_mul(num, fe10_mul(ld10(scratch, 4 + i, T, g), inv));
+      inv = fe10_mul(den, inv);
+      e.v[7] |= fe10_parity_product(x) << 31;
+      drain_one(e, f, banks, sink);
+    }
   }
 }
 
@@ -412,6 +485,9 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
                                              const uint16_t *dfa_lds, uint32_t *parked_out,
                                              const HitSink &sink) {
   constexpr bool kPat = (MODE & kModePattern) != 0;
+  // the sign of x is left to the address kernel (set only for one-round
+  // launches with an address kernel: parked_out non-null)
+  const bool defer = kPat && f.defer_sign != 0 && parked_out != nullptr;
   const uint64_t S = 2ull * H + 1;
   for (uint32_t r = 0; r < rounds; ++r) {
     const uint64_t w = (uint64_t)r * chains + local;
@@ -443,7 +519,11 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         bool hit, need;
         key_stage<MODE>(c, kw, f, banks, dfa_lds, hit, need);
         const uint64_t mn = __ballot(need);
-        if (mn) park(scratch, top, T, g, parked, mn, need, c, fe10_mul(P.X, iz), kc);
+        if (mn) {
+          uint32_t sign = 0;
+          if (!defer) sign = fe10_parity_product(fe10_mul(P.X, iz));
+          park(scratch, top, T, g, parked, mn, need, c, sign, kc);
+        }
         report(hit, kc, c, sink);
       } else {
         test_key_only<MODE>(y, kc, f, sink);
@@ -485,12 +565,19 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         bool hm, nm, hp, np;
         key_stage_pair<MODE>(cm, wm, cp, wp, f, banks, dfa_lds, hm, nm, hp, np);
         const uint64_t mm = __ballot(nm), mp = __ballot(np);
-        if (mm | mp) {
+        if (defer) {
+          if (mm) park(scratch, top, T, g, parked, mm, nm, cm, 0u, km);
+          if (mp) park(scratch, top, T, g, parked, mp, np, cp, 0u, kp);
+        } else if (mm | mp) {
           // x(P - Q) has the denominator dp, x(P + Q) the denominator dm
           const Fe10 XY = fe10_mul(P.X, yq);
           const Fe10 YX = fe10_mul(P.Y, xq);
-          if (mm) park(scratch, top, T, g, parked, mm, nm, cm, fe10_mul(fe10_sub(XY, YX), idp), km);
-          if (mp) park(scratch, top, T, g, parked, mp, np, cp, fe10_mul(fe10_add(XY, YX), idm), kp);
+          if (mm)
+            park(scratch, top, T, g, parked, mm, nm, cm,
+                 fe10_parity_product(fe10_mul(fe10_sub(XY, YX), idp)), km);
+          if (mp)
+            park(scratch, top, T, g, parked, mp, np, cp,
+                 fe10_parity_product(fe10_mul(fe10_add(XY, YX), idm)), kp);
         }
         report(hm, km, cm, sink);
         report(hp, kp, cp, sink);
@@ -503,8 +590,18 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
       // the address stage: here, or (one round per launch) in the address
       // kernel, which runs it at an occupancy the search loop cannot have
       const uint32_t share = parked_share(parked);
-      if (parked_out) parked_out[g] = share;
-      else drain<MODE>(scratch, top, T, g, share, f, banks, sink);
+      if (parked_out) {
+        parked_out[g] = share;
+        // the centre, from which the address kernel derives the deferred signs
+        if (defer) {
+          st10(scratch, 0, T, g, P.X);
+          st10(scratch, 1, T, g, P.Y);
+          st10(scratch, 2, T, g, P.Z);
+          st10(scratch, 3, T, g, P.T);
+        }
+      } else {
+        drain<MODE>(scratch, top, T, g, share, f, banks, sink);
+      }
     }
     P = pt10_add1(P, st);
   }

@@ -16,7 +16,8 @@ class _Survivor(ctypes.Structure):
 
 class _Opts(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_uint32), ("half", ctypes.c_uint32),
-                ("rounds", ctypes.c_uint32), ("hit_cap", ctypes.c_uint32)]
+                ("rounds", ctypes.c_uint32), ("hit_cap", ctypes.c_uint32),
+                ("sign_stage", ctypes.c_uint32)]
 
 
 @dataclass(frozen=True)
@@ -83,8 +84,11 @@ class Engine:
         return cls(h)
 
     @classmethod
-    def with_filters(cls, start_xyzt, filters, threads=0, half=0, rounds=0, hit_cap=0, device=0):
-        opts = _Opts(threads, half, rounds, hit_cap)
+    def with_filters(cls, start_xyzt, filters, threads=0, half=0, rounds=0, hit_cap=0, device=0,
+                     sign_stage=0):
+        """``sign_stage`` (og_engine_opts): 0 chosen from the filters, 1 the
+        search loop, 2 the address kernel."""
+        opts = _Opts(threads, half, rounds, hit_cap, sign_stage)
         h = ctypes.c_void_p()
         e = err_buf()
         rc = lib().og_hip_engine_create_ex(device, bytes(start_xyzt), ctypes.byref(opts),
@@ -111,6 +115,11 @@ class Engine:
         t, h, r = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         lib().og_hip_engine_shape(self._h, ctypes.byref(t), ctypes.byref(h), ctypes.byref(r))
         return t.value, h.value, r.value
+
+    def sign_stage(self):
+        """Where this engine forms the sign of x: 1 the search loop, 2 the
+        address kernel (og_hip_engine_sign_stage)."""
+        return lib().og_hip_engine_sign_stage(self._h)
 
     def restart(self, start_xyzt):
         """Starts over from another start point (og_hip_engine_restart)."""

@@ -216,3 +216,47 @@ def test_digest_over_2p20_matches_oracle():
                      hit_cap=1 << 16)
     want = oracle_hits(start, "2", 1 << 20)
     assert got == want
+
+
+@pytest.mark.parametrize("lines,python,n", [
+    (["contains:2xeq"], None, 1 << 16),
+    (["suffix:aqd"], None, 1 << 15),
+    (["suffix:[a-e]id", "contains:zz7", "ab"], None, 1 << 15),
+    (["regex:(cat|dog)f"], ["(cat|dog)f"], 1 << 16),
+])
+def test_sign_of_x_in_the_loop_or_the_address_kernel(lines, python, n):
+    """The sign of x formed in the search loop (stage 1) or derived in the
+    address kernel from the window's centre, with one inversion per chunk of
+    parked entries (stage 2): the same hits as the oracle either way (for an
+    expression, the oracle's addresses decided by Python's `re`).  Parking is
+    compacted over a wave, so a lane derives signs for entries of its
+    neighbours' chains.  At half 64 a chunk holds 60 entries, so the suffix
+    sets span several chunks."""
+    start = key.start_point(hashlib.sha256(b"sign" + repr(lines).encode()).digest())
+    if python:
+        from tests.test_gpu_regex import truth
+        want = truth(start, python, n)
+    else:
+        want = oracle_hits(start, lines, n)
+    assert want
+    for stage in (1, 2):
+        say(f"case: sign stage {stage}, {lines} n={n}")
+        fs = filters.FilterSet.parse(lines)
+        with gpu.Engine.with_filters(start, fs, threads=256, half=64, sign_stage=stage) as eng:
+            assert eng.sign_stage() == stage
+        assert run_engine(start, fs, n, threads=256, half=64, sign_stage=stage) == want
+
+
+def test_sign_stage_chosen_from_the_filters():
+    say("case: automatic sign stage: a substring defers, a short suffix does not")
+    start = key.start_point(bytes(32))
+    with gpu.Engine.with_filters(start, filters.FilterSet.parse("contains:q2x7a"),
+                                 threads=256, half=64) as eng:
+        assert eng.sign_stage() == 2
+    with gpu.Engine.with_filters(start, filters.FilterSet.parse("suffix:2xaqd"),
+                                 threads=256, half=64) as eng:
+        assert eng.sign_stage() == 1
+    # two rounds per launch drain in place, so the loop forms the sign
+    with gpu.Engine.with_filters(start, filters.FilterSet.parse("contains:q2x7a"),
+                                 threads=256, half=64, rounds=2, sign_stage=2) as eng:
+        assert eng.sign_stage() == 1
