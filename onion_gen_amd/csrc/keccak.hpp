@@ -86,7 +86,7 @@ OG_HD void keccak_f1600(uint64_t a[25]) {
 // The checksum permutation for gfx950: each 64-bit lane as two 32-bit halves,
 // rotations as two v_alignbit_b32 (funnel shifts), chi and the five-way column
 // XOR as v_bitop3_b32 (any three-input bitwise function), and a last round cut
-// to the one lane the checksum reads.  About 190 VALU instructions a round
+// to the one lane the checksum reads.  About 180 VALU instructions a round
 // where the 64-bit form compiles to 288.
 struct KLane {
   uint32_t lo, hi;
@@ -110,48 +110,61 @@ __device__ __forceinline__ uint32_t chi3(uint32_t a, uint32_t b, uint32_t c) {
   return __builtin_amdgcn_bitop3_b32(a, b, c, (unsigned char)0xd2);  // a ^ (~b & c)
 }
 
-__device__ __forceinline__ KLane kx(KLane a, KLane b) { return {a.lo ^ b.lo, a.hi ^ b.hi}; }
+// Theta: the column parities c and their rotations r.  Lane a[x + 5y] then
+// takes a ^ c[x-1] ^ rotl1(c[x+1]) as one three-input XOR a half, rather than
+// forming that sum once per column and XORing it in (10 fewer instructions a
+// round).
+struct KTheta {
+  KLane c[5], r[5];
+};
 
-__device__ __forceinline__ void ktheta(const KLane a[25], KLane d[5]) {
-  KLane c[5];
+__device__ __forceinline__ KTheta ktheta(const KLane a[25]) {
+  KTheta t;
 #pragma unroll
   for (int x = 0; x < 5; ++x) {
-    c[x].lo = xor3(xor3(a[x].lo, a[x + 5].lo, a[x + 10].lo), a[x + 15].lo, a[x + 20].lo);
-    c[x].hi = xor3(xor3(a[x].hi, a[x + 5].hi, a[x + 10].hi), a[x + 15].hi, a[x + 20].hi);
+    t.c[x].lo = xor3(xor3(a[x].lo, a[x + 5].lo, a[x + 10].lo), a[x + 15].lo, a[x + 20].lo);
+    t.c[x].hi = xor3(xor3(a[x].hi, a[x + 5].hi, a[x + 10].hi), a[x + 15].hi, a[x + 20].hi);
   }
 #pragma unroll
-  for (int x = 0; x < 5; ++x) d[x] = kx(c[(x + 4) % 5], krotl<1>(c[(x + 1) % 5]));
+  for (int x = 0; x < 5; ++x) t.r[x] = krotl<1>(t.c[x]);
+  return t;
+}
+
+// Lane a of column x after theta.
+__device__ __forceinline__ KLane ktd(const KLane &a, const KTheta &t, int x) {
+  const KLane &c = t.c[(x + 4) % 5], &r = t.r[(x + 1) % 5];
+  return {xor3(a.lo, c.lo, r.lo), xor3(a.hi, c.hi, r.hi)};
 }
 
 // One round of Keccak-f[1600] on the split-lane state.
 __device__ __forceinline__ void kround(KLane a[25], uint64_t rc) {
-  KLane d[5], b[25];
-  ktheta(a, d);
-  b[0] = kx(a[0], d[0]);
-  b[10] = krotl<1>(kx(a[1], d[1]));
-  b[20] = krotl<62>(kx(a[2], d[2]));
-  b[5] = krotl<28>(kx(a[3], d[3]));
-  b[15] = krotl<27>(kx(a[4], d[4]));
-  b[16] = krotl<36>(kx(a[5], d[0]));
-  b[1] = krotl<44>(kx(a[6], d[1]));
-  b[11] = krotl<6>(kx(a[7], d[2]));
-  b[21] = krotl<55>(kx(a[8], d[3]));
-  b[6] = krotl<20>(kx(a[9], d[4]));
-  b[7] = krotl<3>(kx(a[10], d[0]));
-  b[17] = krotl<10>(kx(a[11], d[1]));
-  b[2] = krotl<43>(kx(a[12], d[2]));
-  b[12] = krotl<25>(kx(a[13], d[3]));
-  b[22] = krotl<39>(kx(a[14], d[4]));
-  b[23] = krotl<41>(kx(a[15], d[0]));
-  b[8] = krotl<45>(kx(a[16], d[1]));
-  b[18] = krotl<15>(kx(a[17], d[2]));
-  b[3] = krotl<21>(kx(a[18], d[3]));
-  b[13] = krotl<8>(kx(a[19], d[4]));
-  b[14] = krotl<18>(kx(a[20], d[0]));
-  b[24] = krotl<2>(kx(a[21], d[1]));
-  b[9] = krotl<61>(kx(a[22], d[2]));
-  b[19] = krotl<56>(kx(a[23], d[3]));
-  b[4] = krotl<14>(kx(a[24], d[4]));
+  KLane b[25];
+  const KTheta t = ktheta(a);
+  b[0] = ktd(a[0], t, 0);
+  b[10] = krotl<1>(ktd(a[1], t, 1));
+  b[20] = krotl<62>(ktd(a[2], t, 2));
+  b[5] = krotl<28>(ktd(a[3], t, 3));
+  b[15] = krotl<27>(ktd(a[4], t, 4));
+  b[16] = krotl<36>(ktd(a[5], t, 0));
+  b[1] = krotl<44>(ktd(a[6], t, 1));
+  b[11] = krotl<6>(ktd(a[7], t, 2));
+  b[21] = krotl<55>(ktd(a[8], t, 3));
+  b[6] = krotl<20>(ktd(a[9], t, 4));
+  b[7] = krotl<3>(ktd(a[10], t, 0));
+  b[17] = krotl<10>(ktd(a[11], t, 1));
+  b[2] = krotl<43>(ktd(a[12], t, 2));
+  b[12] = krotl<25>(ktd(a[13], t, 3));
+  b[22] = krotl<39>(ktd(a[14], t, 4));
+  b[23] = krotl<41>(ktd(a[15], t, 0));
+  b[8] = krotl<45>(ktd(a[16], t, 1));
+  b[18] = krotl<15>(ktd(a[17], t, 2));
+  b[3] = krotl<21>(ktd(a[18], t, 3));
+  b[13] = krotl<8>(ktd(a[19], t, 4));
+  b[14] = krotl<18>(ktd(a[20], t, 0));
+  b[24] = krotl<2>(ktd(a[21], t, 1));
+  b[9] = krotl<61>(ktd(a[22], t, 2));
+  b[19] = krotl<56>(ktd(a[23], t, 3));
+  b[4] = krotl<14>(ktd(a[24], t, 4));
 #pragma unroll
   for (int y = 0; y < 25; y += 5) {
 #pragma unroll
@@ -180,16 +193,15 @@ __device__ __forceinline__ uint32_t onion_checksum_gfx950(const uint32_t pk[8]) 
   a[6] = {0x06u, 0u};
   a[16] = {0u, 0x80000000u};
   // Round 0 on its own: 17 of its lanes are zero and three constant, which
-  // the compiler folds (163 instructions instead of 190).
+  // the compiler folds.
   kround(a, keccak_rc(0));
 #pragma unroll 1
   for (int round = 1; round < 23; ++round) kround(a, keccak_rc(round));
   // round 23, cut to the low half of lane 0: chi of b[0], b[1], b[2] and iota
-  KLane d[5];
-  ktheta(a, d);
-  const uint32_t b0 = a[0].lo ^ d[0].lo;
-  const uint32_t b1 = krotl<44>(kx(a[6], d[1])).lo;
-  const uint32_t b2 = krotl<43>(kx(a[12], d[2])).lo;
+  const KTheta t = ktheta(a);
+  const uint32_t b0 = ktd(a[0], t, 0).lo;
+  const uint32_t b1 = krotl<44>(ktd(a[6], t, 1)).lo;
+  const uint32_t b2 = krotl<43>(ktd(a[12], t, 2)).lo;
   const uint32_t lane0 = chi3(b0, b1, b2) ^ (uint32_t)keccak_rc(23);
   return lane0 & 0xffffu;
 }
