@@ -177,6 +177,29 @@ def test_dictionary_split_across_automata_matches_every_address():
     assert got == want, (len(got), len(want))
 
 
+def test_dictionary_that_reaches_the_checksum_matches_every_address():
+    """1,000 five-symbol substrings: a word can still end in the checksum for
+    most candidates, so most park, and each lane drains a share of many
+    entries.  The engine reports exactly the candidates whose address holds a
+    word anywhere in its 56 symbols, in one-round launches (the address
+    kernel) and in two-round launches (drained in place)."""
+    import random
+    say("case: 1,000 five-symbol substrings, n=2^17")
+    n = 1 << 17
+    start = key.start_point(hashlib.sha256(b"checksum dictionary").digest())
+    every = search(start, 0, n, 256, OracleFilters(EVERY), cap=n)
+    rng = random.Random(11)
+    A = workload.ALPHABET
+    words = {"".join(rng.choice(A) for _ in range(5)) for _ in range(1000)}
+    fs = filters.FilterSet.parse(["contains:" + w for w in sorted(words)])
+    want = sorted((k, clear_sign(pk)) for k, pk, addr in every
+                  if any(addr[s:s + 5] in words for s in range(52)))
+    assert len(want) > 100
+    for rounds in (1, 2):
+        got = run_engine(start, fs, n, threads=256, half=64, rounds=rounds)
+        assert got == want, (rounds, len(got), len(want))
+
+
 def test_address_stage_in_place_and_deferred_agree():
     """A one-round launch leaves the address stage (sign, SHA3, exact pass) to
     its own kernel; a launch of several rounds drains it in place.  Over the
