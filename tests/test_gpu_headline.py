@@ -8,7 +8,8 @@ hit of one full-size launch must be found, and only those, by a shape of 2^18
 chains and four rounds over the same candidates (for the pattern form, the
 address stage then drains in place instead of in its own kernel).  A sample of
 the hits is re-derived from its offset on the host.  The slot engine's default
-unit shape is checked the same way against the single engine.
+unit shape is checked the same way against the single engine, and the sign
+of x derived in the address kernel against the sign formed in the loop.
 """
 import hashlib
 
@@ -78,3 +79,29 @@ def test_slot_engine_default_shape():
                            threads=1 << 18, half=256, rounds=1, hit_cap=1 << 20)
         assert sorted(h for h in hits if h[0] < n) == alone and alone, (line, len(hits))
         assert all(h[0] < n for h in hits)
+
+
+def test_full_launch_substring_with_the_sign_deferred():
+    """At the default shape a rare substring leaves the sign of x to the
+    address kernel, which finds each parked entry's chain from its 64-bit
+    index (past 2^31 here) and derives x from that chain's window centre.  The
+    launch reports exactly what the same launch reports with the sign formed
+    in the search loop; a sample of the hits is re-derived on the host."""
+    seed = hashlib.sha256(b"full substring").digest()
+    start = key.start_point(seed)
+    line = "contains:q2x7a"
+    fs = filters.FilterSet.parse(line)
+    from onion_gen_amd import gpu
+    with gpu.Engine.with_filters(start, fs, **FULL) as eng:
+        assert eng.sign_stage() == 2
+    say(f"case: {line}, one launch at {FULL}, sign in the address kernel")
+    deferred = run_engine(start, fs, N, **FULL)
+    say(f"case: {line}, one launch at {FULL}, sign in the loop")
+    in_loop = run_engine(start, fs, N, sign_stage=1, **FULL)
+    # 52 placements of a 5-symbol word: about 2^31 * 52 / 2^25 = 3,328 hits
+    assert 2600 < len(deferred) < 4200
+    assert deferred == in_loop
+    for k, packed in deferred[:: max(1, len(deferred) // 200)]:
+        pk, addr = key.finish_hit(seed, 8 * k)
+        assert fs.match_address(key.address_bytes(pk)) >= 0, (k, addr)
+        assert pk[:31] == packed[:31] and (pk[31] & 0x7F) == packed[31]
