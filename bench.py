@@ -4,8 +4,9 @@ Workload (the single-GPU configuration of the headline metric): one 6-symbol
 base32 prefix drawn from ChaCha20 seeded 1, the secret seed from ChaCha20
 seeded 0, 2^34 sequential candidates P0 + k*8G.  A step is one engine advance
 (one launch over threads * (2*half+1) * rounds candidates); the default step
-count covers 2^34 candidates.  Every survivor is re-derived from its secret
-scalar on the host and checked against the prefix.
+count covers 2^34 candidates.  Every survivor goes through the hit sink (its
+secret at the offset, its public key re-derived from the scalar alone) and its
+address is matched on the host; the run exits non-zero if one fails.
 
 Prints one JSON line (rank 0).  `value` is candidates/s with the start state
 already on the device; `roofline` prices the search kernel's field
@@ -23,8 +24,24 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# MI355X: 256 CUs x 4 SIMD x 16 lanes at 2.4 GHz = one VALU op per lane per clock.
+# MI355X: 256 CUs x 4 SIMD x 16 lanes at the published 2.4 GHz peak clock, one
+# VALU op per lane per clock: the nominal ceiling, reported beside the
+# measured peak that prices the roofline.
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
+
+
+def progress(msg):
+    """A line on stderr as each part of the run ends (the JSON line stays alone
+    on stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_mad_peak():
+    """P_mul: the peak v_mad_u64_u32 issue rate measured on this device now
+    (og_hip_issue_bench, eight independent chains per lane over 2^20 lanes),
+    the best of three, with the shader clock it ran at."""
+    from onion_gen_amd.radix_bench import issue_rate
+    return max((issue_rate(0, 1 << 20, 2048) for _ in range(3)), key=lambda rc: rc[0])
 
 
 def field_muls_per_candidate(half, with_x=False):
@@ -118,8 +135,10 @@ def batched_leg(seconds):
     # chains in all; a sweep of 2048..32768 chains and half 256..1024 ranked it
     # first for this mix, at a median time to first hit under 0.1 s)
     batched = run_mix(mix, seconds, 64, 16384, 256, batch.LEAST_SERVED)
+    progress("batched mix done")
     plain = run_mix(mix, seconds, 1, 16384, 256, batch.ROUND_ROBIN)
     full = run_mix(mix, seconds, 1, 1 << 18, 256, batch.ROUND_ROBIN, ring_cap=1 << 21)
+    progress("one search per launch done")
     return {
         "workload": "64 searches (search_mix seed 4: " + ", ".join(f"{v} {k}" for k, v in
                                                                    sorted(kinds.items())) +
@@ -132,30 +151,80 @@ def batched_leg(seconds):
     }
 
 
-def served_leg(seconds, rate=200.0):
+def _family(text):
+    return text.split(":")[0] if ":" in text else "prefix"
+
+
+def served_leg(seconds, rate=200.0, fifo_seconds=60.0):
     """Poisson arrivals at `rate`/s for `seconds`, each request a filter of
-    the batched mix wanting 1, 10 or 100 hits: the continuous-batching loop
-    against one search at a time on the whole GPU (first come, first served)."""
+    the batched mix wanting 1, 10 or 100 hits, through the continuous-batching
+    loop.  The comparator is like for like: the requests the loop completed,
+    all available at once, served one at a time on the whole GPU (first come,
+    first served) for at most `fifo_seconds`; both are reported as requests
+    completed per second and as candidates per second per filter family."""
     from onion_gen_amd import batch, filters, serve, workload
     arrivals = workload.poisson_arrivals(rate, seconds, 5)
     mix = workload.search_mix(len(arrivals), 6, hits="choice")
     reqs = [(t, seed, text, want) for t, (seed, text, want) in zip(arrivals, mix)]
-    keep = ("keys_per_s", "completed", "running", "queued", "rejected", "max_resident",
-            "p50_time_to_n", "p99_time_to_n", "p50_time_to_first", "p99_time_to_first")
+    keep = ("wall_s", "keys_per_s", "completed", "running", "queued", "rejected",
+            "max_resident", "p50_time_to_n", "p99_time_to_n", "p50_time_to_first",
+            "p99_time_to_first")
     st, res = serve.run(reqs, duration=seconds, slots=64, chains=16384, half=256,
                         max_queue=1 << 16, policy=batch.LEAST_SERVED)
+    progress(f"served, continuous: {st['completed']} of {len(reqs)} completed")
     cont = {k: st[k] for k in keep}
+    done = [i for i, r in enumerate(res) if r["status"] == "done"]
+    cont["completed_per_s"] = len(done) / st["wall_s"]
+
+    def by_family(idx, results, wall_of):
+        fam = {}
+        for i in idx:
+            f = fam.setdefault(_family(reqs[i][2]), {"requests": 0, "examined": 0, "done": 0})
+            f["requests"] += 1
+            f["examined"] += results[i]["examined"]
+            f["done"] += results[i]["status"] == "done"
+        for name, f in fam.items():
+            f["keys_per_s"] = f["examined"] / wall_of(name) if wall_of(name) else None
+        return fam
+
+    # continuous: every family runs the whole window side by side
+    cont["families"] = by_family(range(len(reqs)), res, lambda name: st["wall_s"])
+    # the same completed requests, one at a time on the whole GPU
+    same = [(0.0, reqs[i][1], reqs[i][2], reqs[i][3]) for i in done]
     with batch.Batch(slots=1, chains=1 << 18, half=256) as b:
-        one = max(b.footprint(filters.FilterSet.parse(t)) for t in {r[2] for r in reqs})
-    st2, _ = serve.run(reqs, duration=seconds, slots=1, chains=1 << 18, half=256,
-                       max_queue=1 << 16, device_budget=one, policy=batch.ROUND_ROBIN)
+        one = max(b.footprint(filters.FilterSet.parse(t)) for t in {r[2] for r in same})
+    st2, res2 = serve.run(same, duration=fifo_seconds, slots=1, chains=1 << 18, half=256,
+                          max_queue=1 << 16, device_budget=one, policy=batch.ROUND_ROBIN)
+    progress(f"served, one at a time: {st2['completed']} of {len(same)} completed")
     fifo = {k: st2[k] for k in keep}
+    fifo["completed_per_s"] = st2["completed"] / st2["wall_s"]
+    # one at a time: a family's rate is over the time its requests held the GPU
+    held = {}
+    for j, r in enumerate(res2):
+        if r["admitted_s"] >= 0:
+            end = r["done_s"] if r["done_s"] >= 0 else st2["wall_s"]
+            name = _family(same[j][2])
+            held[name] = held.get(name, 0.0) + end - r["admitted_s"]
+    fam2 = {}
+    for j, r in enumerate(res2):
+        f = fam2.setdefault(_family(same[j][2]), {"requests": 0, "examined": 0, "done": 0})
+        f["requests"] += 1
+        f["examined"] += r["examined"]
+        f["done"] += r["status"] == "done"
+    for name, f in fam2.items():
+        f["keys_per_s"] = f["examined"] / held[name] if held.get(name) else None
+    fifo["families"] = fam2
     return {
         "workload": f"Poisson {rate:.0f} req/s for {seconds:.0f} s ({len(reqs)} requests), "
                     "filters of the batched mix, n in {1, 10, 100}",
         "continuous": cont,
-        "plain_fifo_full_gpu": fifo,
-        "keys_per_s_vs_plain": cont["keys_per_s"] / fifo["keys_per_s"] if fifo["keys_per_s"] else None,
+        "fifo_same_completed": fifo,
+        "fifo_workload": f"the {len(same)} requests the continuous loop completed, all at t = 0, "
+                         f"one at a time on 2^18 chains, at most {fifo_seconds:.0f} s",
+        "completed_per_s_vs_fifo": (cont["completed_per_s"] / fifo["completed_per_s"]
+                                    if fifo["completed_per_s"] else None),
+        "keys_per_s_vs_fifo": (cont["keys_per_s"] / fifo["keys_per_s"]
+                               if fifo["keys_per_s"] else None),
     }
 
 
@@ -277,33 +346,79 @@ def forms_leg(threads, half, rounds, launches):
     return out
 
 
-def cpu_baseline(prefix, seconds):
-    """The reference's CPU engines restated in C (the oracle), on the host's
-    cores, one block per thread: PairEngine at its default batch of 8192
-    (src/pairs.rs, the reference's default and fastest CPU path, with its limb
-    probe) for `seconds`, and BatchEngine at width 256 (src/batch.rs) for half
-    that beside it."""
-    from tests.oracle_lib import OracleFilters, load
-    import ctypes
-    from onion_gen_amd import workload
-    L = load()
+def host_cpus():
+    """(threads to use, description): the CPUs this process may run on, at most
+    the CPU quota of its control group (a container's share of a larger host),
+    with the processor model and the host's count beside them."""
     try:
-        cores = len(os.sched_getaffinity(0))
+        usable = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    fs = OracleFilters(prefix)
+        usable = os.cpu_count() or 1
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = max(1, int(int(q) / int(period)))
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    threads = min(usable, quota) if quota else usable
+    return threads, {"cpu_model": model, "host_cpus": os.cpu_count(), "affinity": usable,
+                     "cgroup_quota": quota}
+
+
+def cpu_baseline(prefix, seconds):
+    """The reference's CPU engines restated in C (the oracle), built for this
+    host (-O3 -march=native) where it is timed, on every CPU this process may
+    use, one block per thread: PairEngine at its default batch of 8192
+    (src/pairs.rs, the reference's default and fastest CPU path, with its limb
+    probe) and BatchEngine at width 256 (src/batch.rs), each the median of
+    three runs of `seconds` / 3 (BatchEngine: half that)."""
+    import ctypes
+    import statistics
+    import subprocess
+    import tempfile
+    from tests.oracle_lib import load
+    from onion_gen_amd import workload
+    out_dir = tempfile.mkdtemp(prefix="og_oracle_native_")
+    lib_path = os.path.join(out_dir, "liboracle_native.so")
+    flags = "-O3 -march=native"
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native",
+                        f"NATIVE_OUT={lib_path}"], check=True, capture_output=True)
+        L = load(lib_path)
+    except (subprocess.CalledProcessError, OSError):
+        L, flags = load(), "-O2 -march=x86-64-v2 (the native build failed)"
+    cores, cpu = host_cpus()
+    err = ctypes.create_string_buffer(256)
+    fs = L.ofilter_parse(prefix.encode(), err, 256)
     seed = workload.chacha_stream(0, 32)
-    ex_p, ex_b = ctypes.c_uint64(), ctypes.c_uint64()
-    pairs = L.obench_pairs(seed, 8192, cores, seconds, fs.h, ctypes.byref(ex_p))
-    batch = L.obench_batch(seed, 256, cores, seconds / 2, fs.h, ctypes.byref(ex_b))
-    return {"value": pairs, "unit": "keys/s", "cores": cores, "kind": "port",
-            "sample": f"oracle PairEngine batch 8192 (src/pairs.rs restated in C, limb probe), one "
-                      f"block per thread, prefix '{prefix}', {seconds:.0f} s wall, "
-                      f"{ex_p.value} candidates",
-            "batch_engine": {"value": batch, "sample": f"oracle BatchEngine width 256 "
-                                                       f"(src/batch.rs), {seconds / 2:.0f} s, "
-                                                       f"{ex_b.value} candidates"}}
+    ex = ctypes.c_uint64()
+    pairs, batch, n_pairs, n_batch = [], [], 0, 0
+    for _ in range(3):
+        pairs.append(L.obench_pairs(seed, 8192, cores, seconds / 3, fs, ctypes.byref(ex)))
+        n_pairs += ex.value
+    for _ in range(3):
+        batch.append(L.obench_batch(seed, 256, cores, seconds / 6, fs, ctypes.byref(ex)))
+        n_batch += ex.value
+    L.ofilter_free(fs)
+    return {"value": statistics.median(pairs), "unit": "keys/s", "cores": cores,
+            "kind": "port",
+            "sample": f"oracle PairEngine batch 8192 (src/pairs.rs restated in C, limb probe), "
+                      f"built {flags} on this host, one block per thread on {cores} threads, "
+                      f"prefix '{prefix}', median of 3 runs of {seconds / 3:.1f} s "
+                      f"({n_pairs} candidates in all)",
+            "runs": pairs, "build": flags, **cpu,
+            "batch_engine": {"value": statistics.median(batch), "runs": batch,
+                             "sample": f"oracle BatchEngine width 256 (src/batch.rs), median of 3 "
+                                       f"runs of {seconds / 6:.1f} s ({n_batch} candidates)"}}
 
 
 def main():
@@ -322,7 +437,7 @@ def main():
                          "hit through the hit sink)")
     ap.add_argument("--tuned", action="store_true",
                     help="run the autotune walk first and search with the shape it picks")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=18.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--filter", default="",
                     help="search this filter instead of the 6-symbol prefix (for profiling the "
@@ -330,8 +445,10 @@ def main():
                          "names a substring dictionary of the forms leg)")
     ap.add_argument("--mix-seconds", type=float, default=5.0,
                     help="wall-time window of each batched-mix run (0 skips the leg)")
-    ap.add_argument("--serve-seconds", type=float, default=20.0,
-                    help="arrival window of each served run (0 skips the leg)")
+    ap.add_argument("--serve-seconds", type=float, default=120.0,
+                    help="arrival window of the continuous served run (0 skips the leg)")
+    ap.add_argument("--fifo-seconds", type=float, default=60.0,
+                    help="limit of the one-at-a-time replay of the completed requests")
     ap.add_argument("--forms-launches", type=int, default=4,
                     help="timed launches per filter form in the forms leg (0 skips it)")
     ap.add_argument("--copy-calibrate", action="store_true",
@@ -363,6 +480,7 @@ def main():
     if args.tuned:
         tuned = gpu.Engine.tune(start, fs)
         args.threads, args.half, args.rounds = tuned["threads"], tuned["half"], tuned["rounds"]
+    p_mul, p_clock = measured_mad_peak()
     eng = gpu.Engine.with_filters(start, fs, threads=args.threads, half=args.half,
                                   rounds=args.rounds, hit_cap=0 if not args.filter else 1 << 22)
     batch = eng.batch()
@@ -378,13 +496,15 @@ def main():
     wall = time.perf_counter() - t0
     examined = steps * batch
     rate = examined / wall
+    progress(f"headline: {steps} launches, {rate:.4g} keys/s")
 
-    verified = 0
-    for s in survivors[:4096]:
-        got = key.finish_hit(seed, s.offset)
-        if got is not None and fs.match_address(key.address_bytes(got[0])) >= 0 and \
-                got[0][:31] == s.packed[:31] and (got[0][31] & 0x7F) == s.packed[31]:
-            verified += 1
+    # every survivor through the hit sink: its secret at the offset, its
+    # public key re-derived on the device from the scalar alone and compared
+    # with the survivor, then the whole address matched on the host
+    recs = key.finish_hits(seed, survivors)
+    verified = sum(1 for r in recs if r.status == key.HIT_OK and
+                   fs.match_address(key.address_bytes(r.public_key)) >= 0)
+    clamping = sum(1 for r in recs if r.status == key.HIT_CLAMPING)
 
     F = field_muls_per_candidate(args.half, with_x=text.startswith("suffix:"))
     kernel_rate = examined / (kernel_ms * 1e-3)
@@ -408,13 +528,18 @@ def main():
                    "threads": args.threads, "half": args.half, "rounds": args.rounds,
                    "candidates_per_step": batch, "tuning_walk": tuned},
         "hits": len(survivors),
-        "hits_rederived": verified,  # of the first 4096
+        "hits_rederived": verified,  # all of them, through the hit sink
+        "hits_clamping_dropped": clamping,
         "roofline": {
             "bound": "valu",
             "achieved": achieved / 1e12,
-            "peak": VALU_PEAK_LANE_OPS / 1e12,
+            "peak": p_mul / 1e12,
             "unit": "T mad32/s",
-            "frac": achieved / VALU_PEAK_LANE_OPS,
+            "frac": achieved / p_mul,
+            "peak_source": (f"measured: v_mad_u64_u32 issued back to back by og_hip_issue_bench "
+                            f"at a shader clock of {p_clock / 1e9:.3f} GHz"),
+            "peak_nominal": VALU_PEAK_LANE_OPS / 1e12,
+            "frac_nominal": achieved / VALU_PEAK_LANE_OPS,
             "traffic": None if args.filter else measured_traffic(args.threads, args.half,
                                                                   args.rounds),
             "traffic_source": "profiles/search_prefix6_traffic.md (rocprofv3 FETCH_SIZE x2.0 "
@@ -426,15 +551,23 @@ def main():
     eng.close()
     if not args.no_dict and not args.filter:
         out["dictionary"] = dictionary_leg(args.threads, args.half, args.rounds)
+        progress("dictionary leg done")
     if args.forms_launches > 0 and not args.filter:
         out["forms"] = forms_leg(args.threads, args.half, args.rounds, args.forms_launches)
+        progress("forms leg done")
     if args.mix_seconds > 0:
         out["batched"] = batched_leg(args.mix_seconds)
     if args.serve_seconds > 0:
-        out["served"] = served_leg(args.serve_seconds)
+        out["served"] = served_leg(args.serve_seconds, fifo_seconds=args.fifo_seconds)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prefix, args.cpu_seconds)
+        progress("cpu baseline done")
+    if verified + clamping != len(survivors):
+        out["error"] = (f"{len(survivors) - verified - clamping} of {len(survivors)} survivors "
+                        "did not re-derive to a matching key")
     print(json.dumps(out), flush=True)
+    if "error" in out:
+        sys.exit(1)
 
 
 if __name__ == "__main__":

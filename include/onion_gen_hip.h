@@ -81,6 +81,9 @@ typedef struct {
                           in 8 needs it), 1: in the search loop, 2: in the
                           address kernel.  Launches of more than one round, or
                           of half below 8, always form it in the loop. */
+  uint64_t max_bytes;  /* threads = 0: the device bytes the chosen shape may
+                          hold (a remembered shape above it is measured
+                          again); 0 = half of the device's free memory */
 } og_engine_opts;
 
 /* The extended constructor: the filter set compiled by og_filters_parse,
@@ -92,12 +95,21 @@ int og_hip_engine_create_ex(int ordinal, const uint8_t start_xyzt[128], const og
                             size_t errcap);
 
 /* Replaces the automatic choice of Engine::new with retune = true
- * (src/gpu/mod.rs:354-412): times one launch of every shape of a walk over
+ * (src/gpu/mod.rs:354-412), bounded by memory as its walk is
+ * (src/gpu/mod.rs:314-342): times one launch of every shape of a walk over
  * chains {2^20 .. 2^16} x half {1024 .. 128} (rounds for about 2^28
- * candidates a launch) with the given filters, records the fastest in the
- * tuning cache under device/<name>/hip/0/<version>, and writes it. */
+ * candidates a launch) that fits `max_bytes` of device memory (0 = half of
+ * the free memory), with the given filters; keeps, among the shapes within 3%
+ * of the fastest, the one with the smallest footprint (og_tuning_choose);
+ * records it in the tuning cache under og_tuning_device_key(<name>, "hip", 0)
+ * and writes it. */
 int og_hip_engine_tune(int ordinal, const uint8_t start_xyzt[128], const og_filterset *filters,
-                       og_tuning_shape *best, char *err, size_t errcap);
+                       uint64_t max_bytes, og_tuning_shape *best, char *err, size_t errcap);
+
+/* Device bytes an engine holds: chain state, scratch planes, tables, ring.
+ * No counterpart in the reference (its comment at src/gpu/mod.rs:314-318
+ * states the figure). */
+uint64_t og_hip_engine_bytes(const og_engine *e);
 
 /* The shape an engine runs: chains, pairs per window, windows per launch. */
 void og_hip_engine_shape(const og_engine *e, uint32_t *threads, uint32_t *half, uint32_t *rounds);
@@ -159,13 +171,25 @@ int og_field_op_host(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, s
 
 /* The radix bench (no counterpart in the reference; it decides the field
  * representation for gfx950): `threads` lanes each run two chains of `iters`
- * dependent multiplications in radix 8 (8 x 32 saturated limbs, row-wise),
- * 9 (9 x 29 limbs, carry-free columns) or 10 (10 x 25.5 limbs, the shape of
+ * dependent multiplications in radix 5 (5 x 51 in 64-bit limbs, the
+ * fiat-crypto form of src/field.rs:368-432, each limb product four 32-bit
+ * multiplies), 8 (8 x 32 saturated limbs, row-wise), 9 (9 x 29 limbs,
+ * carry-free columns) or 10 (10 x 25.5 limbs, the shape of
  * src/field4.rs:568-670).  Writes field
  * multiplications per second and, when `results` is non-null, the 2*threads
  * canonical chain ends (64 bytes per lane). */
 int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
                        double *muls_per_s, uint8_t *results, char *err, size_t errcap);
+
+/* The issue-rate probe (no counterpart in the reference; it measures the
+ * roofline's denominator, the peak rate of the VALU instruction the field
+ * multiplication is made of): `threads` lanes each run eight independent
+ * chains of `iters` x 8 instructions of one kind - op 0 v_mad_u64_u32, 1
+ * v_mul_lo_u32, 2 v_lshrrev_b64, 3 v_add_u32.  Writes lane-instructions per
+ * second and the shader clock the SIMDs ran at (s_memtime against the
+ * constant-rate s_memrealtime, in Hz; 0 when unknown). */
+int og_hip_issue_bench(int ordinal, int op, uint32_t threads, uint32_t iters, double *ops_per_s,
+                       double *clock_hz, char *err, size_t errcap);
 
 /* --------------------------------------------------------------- hit sink */
 
@@ -363,8 +387,10 @@ void og_batch_destroy(og_batch *b);
 typedef struct {
   double arrival_s;    /* seconds after the start, non-decreasing */
   uint8_t seed[32];    /* the search's seed: s0 = clamp(SHA-512(seed)[0..32]) */
-  char filter[64];     /* filter text, NUL-terminated */
-  uint32_t want_hits;  /* n: the request completes at its n-th hit */
+  char filter[64];     /* filter text, NUL-terminated within the 64 bytes
+                          (a request without the NUL is rejected) */
+  uint32_t want_hits;  /* n >= 1: the request completes at its n-th hit
+                          (0 is rejected) */
 } og_request;
 
 enum {
@@ -400,7 +426,10 @@ typedef struct {
 
 /* Runs the loop for `duration_s` seconds of wall time (or until every request
  * is done).  max_queue bounds the waiting queue; device_budget bounds the
- * device bytes of resident searches (0 = 4 GiB). */
+ * device bytes of resident searches (0 = 4 GiB).  A request is rejected
+ * (OG_REQ_REJECTED) when it arrives to a full queue, when its filter does not
+ * parse, when it asks for no hit, or when its footprint alone exceeds the
+ * budget; a rejected request never holds up the queue behind it. */
 int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs, size_t n,
                  double duration_s, uint32_t max_queue, uint64_t device_budget,
                  og_request_result *results, og_serve_stats *stats, char *err, size_t errcap);
@@ -408,6 +437,25 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
 /* ----------------------------------------------------------------- tuning */
 
 
+
+/* Replaces tuning::device_key (src/tuning.rs:65-74): "device/<name>/<api>/
+ * <threads>/<version>" with '/' and newline in the name replaced by '_'.
+ * Copies at most cap-1 bytes and a terminator; returns the key's length. */
+size_t og_tuning_device_key(const char *name, const char *api, uint32_t threads, char *out,
+                            size_t cap);
+/* The version the keys end in: the reference crate's CARGO_PKG_VERSION
+ * (Cargo.toml:3, "1.0.2") unless the host sets its own, as a Rust host that
+ * links this library passes env!("CARGO_PKG_VERSION").  Returns 0, or -1 for
+ * an empty version, one over 64 bytes or one holding '/' or a newline. */
+int og_tuning_set_version(const char *version);
+/* The shape the walk keeps, from the measured rates (0 = not measured) and
+ * footprints of n shapes: the fastest within `budget` bytes (0 = no bound),
+ * then, among those within `tolerance` of its rate and the budget, the
+ * smallest footprint.  Returns its index, or (size_t)-1 when none was
+ * measured.  No counterpart in the reference, whose walk keeps the fastest
+ * (src/gpu/mod.rs:383-411). */
+size_t og_tuning_choose(const double *rate, const uint64_t *bytes, size_t n, uint64_t budget,
+                        double tolerance);
 
 /* Replaces tuning::path (src/tuning.rs:80-91); OG_TUNING_PATH overrides. */
 int og_tuning_path(char *out, size_t cap);

@@ -33,6 +33,9 @@ SIGNATURES = {
     "og_hip_radix_bench": (_c.c_int, [_c.c_int, _c.c_int, _c.c_uint32, _c.c_uint32,
                                       _c.POINTER(_c.c_double), _c.c_char_p, _c.c_char_p,
                                       _c.c_size_t]),
+    "og_hip_issue_bench": (_c.c_int, [_c.c_int, _c.c_int, _c.c_uint32, _c.c_uint32,
+                                      _c.POINTER(_c.c_double), _c.POINTER(_c.c_double),
+                                      _c.c_char_p, _c.c_size_t]),
     "og_field_op_host": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_char_p, _c.c_char_p,
                                     _c.c_size_t]),
     "og_filters_parse": (_c.c_void_p, [_c.c_char_p, _c.c_uint32, _c.c_char_p, _c.c_size_t]),
@@ -76,8 +79,14 @@ SIGNATURES.update({
                                     _c.c_void_p, _c.c_size_t, _c.POINTER(_c.c_size_t),
                                     _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint64),
                                     _c.c_char_p, _c.c_size_t]),
-    "og_hip_engine_tune": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_void_p,
+    "og_hip_engine_tune": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_void_p, _c.c_uint64,
                                       _c.POINTER(TuningShape), _c.c_char_p, _c.c_size_t]),
+    "og_hip_engine_bytes": (_c.c_uint64, [_c.c_void_p]),
+    "og_tuning_device_key": (_c.c_size_t, [_c.c_char_p, _c.c_char_p, _c.c_uint32, _c.c_char_p,
+                                           _c.c_size_t]),
+    "og_tuning_set_version": (_c.c_int, [_c.c_char_p]),
+    "og_tuning_choose": (_c.c_size_t, [_c.POINTER(_c.c_double), _c.POINTER(_c.c_uint64),
+                                       _c.c_size_t, _c.c_uint64, _c.c_double]),
     "og_hip_engine_shape": (None, [_c.c_void_p, _c.POINTER(_c.c_uint32),
                                    _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint32)]),
     "og_hip_engine_sign_stage": (_c.c_int, [_c.c_void_p]),

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/onion_gen_hip.h"
+#include "fe51.hpp"
 #include "fe9.hpp"
 #include "og_devfilter.hpp"
 #include "og_filters.hpp"
@@ -138,8 +139,9 @@ __global__ __launch_bounds__(256) void og_field_kernel(int op, const uint32_t *a
 }
 
 // Radix bench: each lane runs two independent chains of `iters` dependent
-// multiplications x <- x*y in the chosen representation (8: 8 x 32 saturated,
-// 10: 10 x 25.5), then writes the canonical results.  Inputs are a fixed
+// multiplications x <- x*y in the chosen representation (5: 5 x 51 in 64-bit
+// limbs, 8: 8 x 32 saturated, 9: 9 x 29, 10: 10 x 25.5), then writes the
+// canonical results.  Inputs are a fixed
 // function of the lane (radix_input), so the host can recompute the answer.
 __host__ __device__ inline uint32_t radix_input(uint32_t lane, uint32_t which, uint32_t l) {
   uint32_t x = lane * 0x9E3779B9u + which * 0x85EBCA6Bu + l * 0xC2B2AE35u + 0x27220A95u;
@@ -166,6 +168,14 @@ __global__ __launch_bounds__(256) void og_radix_kernel(uint32_t iters, uint32_t 
       a = fe_mul(a, y);
       b = fe_mul(b, y);
     }
+  } else if (RADIX == 5) {
+    Fe51 a5 = fe51_from(a), b5 = fe51_from(b), y5 = fe51_from(y);
+    for (uint32_t i = 0; i < iters; ++i) {
+      a5 = fe51_mul(a5, y5);
+      b5 = fe51_mul(b5, y5);
+    }
+    a = fe51_to(a5);
+    b = fe51_to(b5);
   } else if (RADIX == 9) {
     Fe9 a9 = fe9_from(a), b9 = fe9_from(b), y9 = fe9_from(y);
     for (uint32_t i = 0; i < iters; ++i) {
@@ -191,6 +201,66 @@ __global__ __launch_bounds__(256) void og_radix_kernel(uint32_t iters, uint32_t 
   }
 }
 
+// Issue-rate probe: every lane runs eight independent chains of one VALU
+// instruction, unrolled eight deep, so that the SIMD always has an
+// instruction ready; the rate is lane-instructions per second.  Lane 0 of
+// each workgroup's first wave also reads the shader clock (s_memtime) and the
+// constant-rate clock (s_memrealtime) around the loop, from which the host
+// derives the clock the SIMDs ran at.  OP 0: v_mad_u64_u32 (the product the
+// field multiplication issues 100 of), 1: v_mul_lo_u32, 2: v_lshrrev_b64,
+// 3: v_add_u32.
+template <int OP>
+__global__ __launch_bounds__(256) void og_issue_kernel(uint32_t iters, uint64_t *out,
+                                                       uint64_t *clocks) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t acc[8];
+  uint32_t a[8], b[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    a[c] = g * 0x9E3779B9u + (uint32_t)c * 0x85EBCA6Bu + 1u;
+    b[c] = g ^ (0x27220A95u * (uint32_t)(c + 1));
+    acc[c] = ((uint64_t)b[c] << 32) | a[c];
+  }
+  uint64_t t0 = 0, r0 = 0;
+  if (threadIdx.x == 0) {
+    t0 = __builtin_amdgcn_s_memtime();
+    r0 = __builtin_amdgcn_s_memrealtime();
+  }
+  for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        if (OP == 0) {
+          uint64_t cc;
+          asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0"
+                       : "+v"(acc[c]), "=s"(cc)
+                       : "v"(a[c]), "v"(b[c]));
+        } else if (OP == 1) {
+          uint32_t lo = (uint32_t)acc[c];
+          asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(lo) : "v"(b[c]));
+          acc[c] = (acc[c] & 0xffffffff00000000ull) | lo;
+        } else if (OP == 2) {
+          asm volatile("v_lshrrev_b64 %0, 1, %0" : "+v"(acc[c]));
+        } else {
+          uint32_t lo = (uint32_t)acc[c];
+          asm volatile("v_add_u32 %0, %0, %1" : "+v"(lo) : "v"(b[c]));
+          acc[c] = (acc[c] & 0xffffffff00000000ull) | lo;
+        }
+      }
+    }
+  }
+  if (threadIdx.x == 0) {
+    const uint64_t t1 = __builtin_amdgcn_s_memtime(), r1 = __builtin_amdgcn_s_memrealtime();
+    clocks[2 * blockIdx.x] = t1 - t0;
+    clocks[2 * blockIdx.x + 1] = r1 - r0;
+  }
+  uint64_t x = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) x ^= acc[c];
+  out[g] = x;
+}
+
 }  // namespace
 
 // -------------------------------------------------------------------- host
@@ -210,6 +280,7 @@ struct og_engine {
   HitRec *h_hits = nullptr;
   SearchArgs args;
   float last_ms = 0;
+  uint64_t bytes = 0;  // device memory the engine holds
 };
 
 namespace {
@@ -337,6 +408,26 @@ uint32_t survivor_room(uint64_t batch, const FilterSet *fs, const DevFilterHost 
   return (uint32_t)std::max<double>(floor_room, std::min<double>(want, (double)(1u << 22)));
 }
 
+// Device bytes an engine of this shape holds: chain state (4 coordinates of
+// 10 limbs), the offset table, the scratch planes, the survivor ring, the
+// parked counts of a pattern launch and the filter tables.
+uint64_t shape_bytes(uint32_t threads, uint32_t half, uint32_t cap, bool pattern,
+                     uint64_t filter_bytes) {
+  return (uint64_t)160 * threads + (uint64_t)120 * std::max<uint32_t>(half, 1) +
+         (uint64_t)scratch_planes(half) * 40 * threads + 4 + (uint64_t)cap * sizeof(HitRec) +
+         (pattern ? (uint64_t)4 * threads : 0) + filter_bytes;
+}
+
+// The memory an engine may take when its shape is chosen for it: the
+// caller's cap, else half of what the device has free.
+uint64_t engine_budget(int ordinal, uint64_t cap) {
+  if (cap) return cap;
+  size_t free_b = 0, total_b = 0;
+  if (hipSetDevice(ordinal) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+    return 0;
+  return (uint64_t)free_b / 2;
+}
+
 int create_engine(int ordinal, const uint8_t start_xyzt[128], const og_engine_opts *opts,
                   const FilterSet *fs, const uint64_t *bitmap, size_t bitmap_words,
                   uint32_t bitmap_bits, og_engine **out, char *err, size_t errcap) {
@@ -433,6 +524,8 @@ int create_engine(int ordinal, const uint8_t start_xyzt[128], const og_engine_op
   }
   a.hits = e->d_hits;
   a.cap = e->cap;
+  e->bytes = shape_bytes(o.threads, o.half, e->cap, e->d_parked != nullptr,
+                         e->filt.device_bytes());
 
   // ---- per-chain start points on the device
   SetupArgs sa = hc.setup_args(start_xyzt, e->d_state, o.threads, 0, o.threads, o.half);
@@ -486,10 +579,9 @@ std::string tuning_key(int ordinal) {
   uint64_t mem = 0;
   int cus = 0;
   og_hip_device_info(ordinal, name, sizeof name, &mem, &cus);
-  std::string safe(name);
-  for (char &c : safe)
-    if (c == '/' || c == '\n') c = '_';
-  return "device/" + safe + "/hip/0/" + kTuningVersion;
+  std::string key(og_tuning_device_key(name, "hip", 0, nullptr, 0), '\0');
+  og_tuning_device_key(name, "hip", 0, &key[0], key.size() + 1);
+  return key;
 }
 
 // The shapes the walk times, largest first (src/gpu/mod.rs:315-340 walks its
@@ -510,15 +602,38 @@ std::vector<og_engine_opts> tuning_walk() {
   return out;
 }
 
-// Times one launch of every shape after a warm-up launch (which pays for the
-// tables reaching the device), as src/gpu/mod.rs:415-428 does, keeps the
-// fastest in candidates per second of wall time and records it.
+// The filter tables' device bytes for a shape estimate.
+uint64_t filter_bytes(const FilterSet *fs, size_t bitmap_words) {
+  if (!fs) return (uint64_t)bitmap_words * 8;
+  DevFilterHost h;
+  return h.compile(*fs).empty() ? h.device_bytes() : 0;
+}
+
+// Times one launch of every shape that fits `budget` bytes after a warm-up
+// launch (which pays for the tables reaching the device), as
+// src/gpu/mod.rs:415-428 does, in candidates per second of wall time.  The
+// reference bounds its walk by memory too (src/gpu/mod.rs:314-342): here
+// shapes above the budget are not built, and among the shapes within 3% of
+// the fastest the smallest footprint is kept (og_tuning_choose), so that an
+// engine never holds most of the device for a rate it would nearly reach with
+// a fraction of it.  The choice is recorded.
 int tune(int ordinal, const uint8_t start_xyzt[128], const FilterSet *fs, const uint64_t *bitmap,
-         size_t bitmap_words, uint32_t bitmap_bits, og_tuning_shape *best, char *err,
-         size_t errcap) {
+         size_t bitmap_words, uint32_t bitmap_bits, uint64_t budget, og_tuning_shape *best,
+         char *err, size_t errcap) {
   og_tuning_shape b{};
   std::string last = "no shape was tried";
-  for (const og_engine_opts &o : tuning_walk()) {
+  const std::vector<og_engine_opts> walk = tuning_walk();
+  std::vector<double> rates(walk.size(), 0.0);
+  std::vector<uint64_t> sizes(walk.size(), 0);
+  const uint64_t fbytes = filter_bytes(fs, bitmap_words);
+  const bool pattern = fs && (!fs->general.empty() || !fs->regexes.empty());
+  for (size_t i = 0; i < walk.size(); ++i) {
+    const og_engine_opts &o = walk[i];
+    sizes[i] = shape_bytes(o.threads, o.half, o.hit_cap, pattern && o.rounds == 1, fbytes);
+    if (budget && sizes[i] > budget) {
+      last = "every shape exceeds the memory budget of " + std::to_string(budget) + " bytes";
+      continue;
+    }
     og_engine *e = nullptr;
     char e2[512] = {0};
     if (create_engine(ordinal, start_xyzt, &o, fs, bitmap, bitmap_words, bitmap_bits, &e, e2,
@@ -538,13 +653,16 @@ int tune(int ordinal, const uint8_t start_xyzt[128], const FilterSet *fs, const 
       }
     }
     if (rate == 0 && e2[0]) last = e2;
+    if (rate > 0) sizes[i] = e->bytes;
     og_hip_engine_destroy(e);
-    if (rate > b.rate) {
-      b.first = o.half;
-      b.second = o.threads;
-      b.rounds = o.rounds;
-      b.rate = rate;
-    }
+    rates[i] = rate;
+  }
+  const size_t pick = og_tuning_choose(rates.data(), sizes.data(), walk.size(), budget, 0.03);
+  if (pick < walk.size()) {
+    b.first = walk[pick].half;
+    b.second = walk[pick].threads;
+    b.rounds = walk[pick].rounds;
+    b.rate = rates[pick];
   }
   if (b.rate == 0) {
     put_err(err, errcap, "no search shape this device would take: " + last);
@@ -572,16 +690,24 @@ int create_tuned(int ordinal, const uint8_t start_xyzt[128], const og_engine_opt
     o.hit_cap = opts->hit_cap;
     o.sign_stage = opts->sign_stage;
   }
+  const uint64_t budget = engine_budget(ordinal, opts ? opts->max_bytes : 0);
   og_tuning_shape s{};
   if (og_tuning_get(tuning_key(ordinal).c_str(), &s) == 1 && s.first && s.second) {
     o.threads = s.second;
     o.half = s.first;
     o.rounds = s.rounds ? s.rounds : 1;
-    if (create_engine(ordinal, start_xyzt, &o, fs, bitmap, bitmap_words, bitmap_bits, out, err,
+    // a remembered shape above this engine's budget is measured again, as one
+    // that no longer builds is (src/gpu/mod.rs:371-381)
+    const bool pattern = fs && (!fs->general.empty() || !fs->regexes.empty());
+    const uint64_t est = shape_bytes(o.threads, o.half, o.hit_cap ? o.hit_cap : (1u << 20),
+                                     pattern && o.rounds == 1, filter_bytes(fs, bitmap_words));
+    if ((!budget || est <= budget) &&
+        create_engine(ordinal, start_xyzt, &o, fs, bitmap, bitmap_words, bitmap_bits, out, err,
                       errcap) == 0)
       return 0;
   }
-  if (tune(ordinal, start_xyzt, fs, bitmap, bitmap_words, bitmap_bits, &s, err, errcap) != 0)
+  if (tune(ordinal, start_xyzt, fs, bitmap, bitmap_words, bitmap_bits, budget, &s, err,
+           errcap) != 0)
     return -1;
   o.threads = s.second;
   o.half = s.first;
@@ -615,14 +741,17 @@ int og_hip_engine_create_ex(int ordinal, const uint8_t start_xyzt[128], const og
 }
 
 int og_hip_engine_tune(int ordinal, const uint8_t start_xyzt[128], const og_filterset *filters,
-                       og_tuning_shape *best, char *err, size_t errcap) {
+                       uint64_t max_bytes, og_tuning_shape *best, char *err, size_t errcap) {
   if (!start_xyzt || !filters) {
     put_err(err, errcap, "null argument");
     return -1;
   }
   if (check_device(ordinal, err, errcap)) return -1;
-  return tune(ordinal, start_xyzt, filterset_of(filters), nullptr, 0, 0, best, err, errcap);
+  return tune(ordinal, start_xyzt, filterset_of(filters), nullptr, 0, 0,
+              engine_budget(ordinal, max_bytes), best, err, errcap);
 }
+
+uint64_t og_hip_engine_bytes(const og_engine *e) { return e ? e->bytes : 0; }
 
 void og_hip_engine_shape(const og_engine *e, uint32_t *threads, uint32_t *half,
                          uint32_t *rounds) {
@@ -711,8 +840,9 @@ void og_hip_engine_destroy(og_engine *e) { free_engine(e); }
 
 int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
                        double *muls_per_s, uint8_t *results, char *err, size_t errcap) {
-  if ((radix != 8 && radix != 9 && radix != 10) || threads == 0 || threads % 256 || threads > (1u << 22)) {
-    put_err(err, errcap, "radix must be 8, 9 or 10 and threads a multiple of 256 up to 2^22");
+  if ((radix != 5 && radix != 8 && radix != 9 && radix != 10) || threads == 0 || threads % 256 ||
+      threads > (1u << 22)) {
+    put_err(err, errcap, "radix must be 5, 8, 9 or 10 and threads a multiple of 256 up to 2^22");
     return -1;
   }
   OG_TRY(hipSetDevice(ordinal));
@@ -722,7 +852,9 @@ int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   auto launch = [&](uint32_t it) {
-    if (radix == 8)
+    if (radix == 5)
+      hipLaunchKernelGGL(og_radix_kernel<5>, dim3(threads / 256), dim3(256), 0, 0, it, d, threads);
+    else if (radix == 8)
       hipLaunchKernelGGL(og_radix_kernel<8>, dim3(threads / 256), dim3(256), 0, 0, it, d, threads);
     else if (radix == 9)
       hipLaunchKernelGGL(og_radix_kernel<9>, dim3(threads / 256), dim3(256), 0, 0, it, d, threads);
@@ -745,6 +877,62 @@ int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
     return -1;
   }
   if (muls_per_s) *muls_per_s = 2.0 * threads * (double)iters / (ms * 1e-3);
+  return 0;
+}
+
+int og_hip_issue_bench(int ordinal, int op, uint32_t threads, uint32_t iters, double *ops_per_s,
+                       double *clock_hz, char *err, size_t errcap) {
+  if (op < 0 || op > 3 || threads == 0 || threads % 256 || threads > (1u << 22) || iters == 0) {
+    put_err(err, errcap, "op must be 0..3, threads a multiple of 256 up to 2^22, iters positive");
+    return -1;
+  }
+  OG_TRY(hipSetDevice(ordinal));
+  int wall_khz = 0;
+  OG_TRY(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, ordinal));
+  const uint32_t blocks = threads / 256;
+  uint64_t *d = nullptr, *clk = nullptr;
+  OG_TRY(hipMalloc(&d, (size_t)threads * 8));
+  hipError_t e = hipMalloc(&clk, (size_t)blocks * 16);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  auto launch = [&](uint32_t it) {
+    switch (op) {
+      case 0: hipLaunchKernelGGL(og_issue_kernel<0>, dim3(blocks), dim3(256), 0, 0, it, d, clk); break;
+      case 1: hipLaunchKernelGGL(og_issue_kernel<1>, dim3(blocks), dim3(256), 0, 0, it, d, clk); break;
+      case 2: hipLaunchKernelGGL(og_issue_kernel<2>, dim3(blocks), dim3(256), 0, 0, it, d, clk); break;
+      default: hipLaunchKernelGGL(og_issue_kernel<3>, dim3(blocks), dim3(256), 0, 0, it, d, clk);
+    }
+  };
+  float ms = 0;
+  std::vector<uint64_t> h(2 * (size_t)blocks);
+  if (e == hipSuccess) {
+    launch(iters < 64 ? iters : 64);  // warm-up
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(e0, 0);
+  if (e == hipSuccess) launch(iters);
+  if (e == hipSuccess) e = hipEventRecord(e1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess) e = hipMemcpy(h.data(), clk, h.size() * 8, hipMemcpyDeviceToHost);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(d);
+  if (clk) (void)hipFree(clk);
+  if (e != hipSuccess) {
+    put_err(err, errcap, std::string("issue bench: ") + hipGetErrorString(e));
+    return -1;
+  }
+  if (ops_per_s) *ops_per_s = 64.0 * threads * (double)iters / (ms * 1e-3);
+  if (clock_hz) {
+    double ticks = 0, real = 0;
+    for (uint32_t i = 0; i < blocks; ++i) {
+      ticks += (double)h[2 * i];
+      real += (double)h[2 * i + 1];
+    }
+    *clock_hz = real > 0 && wall_khz > 0 ? ticks / real * wall_khz * 1e3 : 0.0;
+  }
   return 0;
 }
 

@@ -11,10 +11,6 @@
 
 namespace og {
 
-// The version in this library's tuning-cache keys (src/tuning.rs:65-74 puts
-// the crate version there).
-constexpr const char *kTuningVersion = "0.1.0";
-
 struct HostCurve {
   Fe d, two_d;
   Point base;  // B (src/curve.rs:28-31)

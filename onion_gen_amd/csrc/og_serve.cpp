@@ -89,18 +89,33 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
   auto now = [&] {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   };
-  // admission: first come, first served, while the footprint fits the budget
+  // admission: first come, first served, while the footprint fits the budget.
+  // A request that could never fit (its footprint alone exceeds the budget),
+  // whose filter does not parse, or that asks for no hit is rejected and
+  // leaves the queue, so it never blocks the requests behind it.
   auto admit_pending = [&] {
     while (!queue.empty()) {
       const size_t r = queue.front();
       char e2[256];
-      og_filterset *fs = og_filters_parse(reqs[r].filter, 0, e2, sizeof e2);
+      char text[sizeof reqs[r].filter + 1];
+      const size_t len = strnlen(reqs[r].filter, sizeof reqs[r].filter);
+      og_filterset *fs = nullptr;
+      if (len < sizeof reqs[r].filter && reqs[r].want_hits != 0) {
+        std::memcpy(text, reqs[r].filter, len + 1);
+        fs = og_filters_parse(text, 0, e2, sizeof e2);
+      }
       if (!fs) {
         results[r].status = OG_REQ_REJECTED;
         queue.pop_front();
         continue;
       }
       const size_t bytes = og_batch_footprint(b, fs);
+      if (bytes > budget) {
+        og_filters_free(fs);
+        results[r].status = OG_REQ_REJECTED;
+        queue.pop_front();
+        continue;
+      }
       if (used + bytes > budget) {
         og_filters_free(fs);
         break;  // wait for a resident request to free device memory
@@ -116,7 +131,7 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
       }
       queue.pop_front();
       // the request records only the hits it asked for
-      if (reqs[r].want_hits) (void)og_batch_set_want(b, id, reqs[r].want_hits);
+      (void)og_batch_set_want(b, id, reqs[r].want_hits);
       if ((size_t)id >= live.size()) live.resize(id + 1, Live{0, nullptr, 0});
       live[id] = Live{r, fs, bytes};
       used += bytes;
@@ -144,10 +159,11 @@ int og_serve_run(int ordinal, const og_batch_opts *opts, const og_request *reqs,
     admit_pending();
     if (resident == 0) {
       if (next >= n && queue.empty()) break;  // nothing resident, nothing left to arrive
-      if (next < n) {
-        const double wait = std::min(reqs[next].arrival_s - now(), 0.001);
-        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
-      }
+      // nothing to run until the next arrival (admission has emptied the queue
+      // or rejected its head, so the queue cannot wait on an empty device)
+      const double wait =
+          next < n ? std::min(reqs[next].arrival_s - now(), 0.001) : 0.001;
+      if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
       continue;
     }
     uint32_t nh = 0;

@@ -14,6 +14,7 @@
 // representation the bench chose), fields a reader of the reference format
 // ignores.  Every operation is best effort: an unreadable or foreign file is
 // "no entry" (src/tuning.rs:1-18).  OG_TUNING_PATH overrides the location.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -95,9 +96,53 @@ void mkdirs(const std::string &path) {
     if (path[i] == '/') mkdir(path.substr(0, i).c_str(), 0755);
 }
 
+// The version the cache keys end in: the reference crate's (Cargo.toml:3,
+// CARGO_PKG_VERSION), which a Rust host that links this library replaces with
+// its own through og_tuning_set_version.
+std::string g_version = "1.0.2";
+
 }  // namespace
 
 extern "C" {
+
+int og_tuning_set_version(const char *version) {
+  if (!version || !*version || std::strlen(version) > 64 || std::strchr(version, '/') ||
+      std::strchr(version, '\n'))
+    return -1;
+  g_version = version;
+  return 0;
+}
+
+size_t og_tuning_device_key(const char *name, const char *api, uint32_t threads, char *out,
+                            size_t cap) {
+  std::string safe = name ? name : "";
+  for (char &c : safe)
+    if (c == '/' || c == '\n') c = '_';
+  const std::string key = "device/" + safe + "/" + (api ? api : "") + "/" +
+                          std::to_string(threads) + "/" + g_version;
+  if (out && cap) {
+    const size_t n = std::min(cap - 1, key.size());
+    std::memcpy(out, key.data(), n);
+    out[n] = 0;
+  }
+  return key.size();
+}
+
+size_t og_tuning_choose(const double *rate, const uint64_t *bytes, size_t n, uint64_t budget,
+                        double tolerance) {
+  size_t best = (size_t)-1;
+  for (size_t i = 0; i < n; ++i)
+    if (rate[i] > 0 && (!budget || bytes[i] <= budget) && (best == (size_t)-1 || rate[i] > rate[best]))
+      best = i;
+  if (best == (size_t)-1) return best;
+  const double floor = rate[best] * (1.0 - tolerance);
+  size_t pick = best;
+  for (size_t i = 0; i < n; ++i)
+    if (rate[i] > 0 && rate[i] >= floor && (!budget || bytes[i] <= budget) &&
+        (bytes[i] < bytes[pick] || (bytes[i] == bytes[pick] && rate[i] > rate[pick])))
+      pick = i;
+  return pick;
+}
 
 int og_tuning_path(char *out, size_t cap) {
   std::string p = cache_path();

@@ -17,7 +17,7 @@ class _Survivor(ctypes.Structure):
 class _Opts(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_uint32), ("half", ctypes.c_uint32),
                 ("rounds", ctypes.c_uint32), ("hit_cap", ctypes.c_uint32),
-                ("sign_stage", ctypes.c_uint32)]
+                ("sign_stage", ctypes.c_uint32), ("max_bytes", ctypes.c_uint64)]
 
 
 @dataclass(frozen=True)
@@ -85,10 +85,11 @@ class Engine:
 
     @classmethod
     def with_filters(cls, start_xyzt, filters, threads=0, half=0, rounds=0, hit_cap=0, device=0,
-                     sign_stage=0):
+                     sign_stage=0, max_bytes=0):
         """``sign_stage`` (og_engine_opts): 0 chosen from the filters, 1 the
-        search loop, 2 the address kernel."""
-        opts = _Opts(threads, half, rounds, hit_cap, sign_stage)
+        search loop, 2 the address kernel.  ``max_bytes``: with threads = 0,
+        the device memory the chosen shape may hold (0: half the free memory)."""
+        opts = _Opts(threads, half, rounds, hit_cap, sign_stage, max_bytes)
         h = ctypes.c_void_p()
         e = err_buf()
         rc = lib().og_hip_engine_create_ex(device, bytes(start_xyzt), ctypes.byref(opts),
@@ -98,17 +99,22 @@ class Engine:
         return cls(h)
 
     @staticmethod
-    def tune(start_xyzt, filters, device=0):
-        """Measures the walk of shapes (og_hip_engine_tune), records the
-        fastest in the tuning cache and returns it as a dict."""
+    def tune(start_xyzt, filters, device=0, max_bytes=0):
+        """Measures the walk of shapes within ``max_bytes`` of device memory
+        (og_hip_engine_tune; 0: half the free memory), records the shape it
+        keeps in the tuning cache and returns it as a dict."""
         from ._lib import TuningShape
         best = TuningShape()
         e = err_buf()
-        if lib().og_hip_engine_tune(device, bytes(start_xyzt), filters.handle, ctypes.byref(best),
-                                    e, len(e)) != 0:
+        if lib().og_hip_engine_tune(device, bytes(start_xyzt), filters.handle, max_bytes,
+                                    ctypes.byref(best), e, len(e)) != 0:
             raise OgError(e.value.decode())
         return {"threads": best.second, "half": best.first, "rounds": best.rounds,
                 "rate": best.rate, "at": best.at}
+
+    def bytes(self):
+        """Device memory this engine holds (og_hip_engine_bytes)."""
+        return lib().og_hip_engine_bytes(self._h)
 
     def shape(self):
         """(threads, half, rounds) this engine runs."""

@@ -3,7 +3,8 @@
 Runs og_hip_radix_bench for each representation, checks every chain end
 against Python big-integer arithmetic, prints field multiplications per
 second and records the fastest in the tuning cache under
-``device/<name>/hip/radix/<version>`` (field ``radix``).
+``device/<name>/hip-radix/0/<version>`` (src/tuning.rs:65-74's key shape,
+field ``radix``).
 
     python -m onion_gen_amd.radix_bench [--threads N] [--iters N]
 """
@@ -12,12 +13,12 @@ import ctypes
 import json
 
 from ._lib import TuningShape, err_buf, lib
-from . import gpu
+from . import gpu, tuning
 
 P = 2 ** 255 - 19
-VERSION = "0.1.0"
-RADIXES = {8: "8 x 32 saturated, row-wise", 9: "9 x 29, carry-free columns",
-           10: "10 x 25.5 (ref10 shape)"}
+RADIXES = {5: "5 x 51 in 64-bit limbs (fiat-crypto form)", 8: "8 x 32 saturated, row-wise",
+           9: "9 x 29, carry-free columns", 10: "10 x 25.5 (ref10 shape)"}
+ISSUE_OPS = {0: "v_mad_u64_u32", 1: "v_mul_lo_u32", 2: "v_lshrrev_b64", 3: "v_add_u32"}
 
 
 def radix_input(lane, which, limb):
@@ -52,12 +53,32 @@ def run(radix, threads, iters, check_lanes=64):
     return rate.value
 
 
+def issue_rate(op=0, threads=1 << 20, iters=2048, device=0):
+    """og_hip_issue_bench: (lane-instructions per second, shader clock in Hz)
+    of one VALU instruction kind (ISSUE_OPS) issued back to back."""
+    rate, clock = ctypes.c_double(), ctypes.c_double()
+    e = err_buf()
+    if lib().og_hip_issue_bench(device, op, threads, iters, ctypes.byref(rate),
+                                ctypes.byref(clock), e, len(e)) != 0:
+        raise RuntimeError(e.value.decode())
+    return rate.value, clock.value
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=1 << 18)
     ap.add_argument("--iters", type=int, default=256)
     ap.add_argument("--no-record", action="store_true")
+    ap.add_argument("--issue", action="store_true",
+                    help="also the issue rate of each VALU instruction the product is made of")
     args = ap.parse_args()
+    if args.issue:
+        for op, name in ISSUE_OPS.items():
+            best = max((issue_rate(op) for _ in range(3)), key=lambda rc: rc[0])
+            print(json.dumps({"instruction": name, "lane_ops_per_s": best[0],
+                              "clock_hz": best[1],
+                              "per_lane_per_clock": best[0] / (256 * 64 * best[1])
+                              if best[1] else None}), flush=True)
     results = {}
     for radix, name in RADIXES.items():
         rate = run(radix, args.threads, args.iters)
@@ -68,7 +89,7 @@ def main():
     print(json.dumps({"best": best, "form": RADIXES[best], "field_mul_per_s": results[best]}))
     if not args.no_record:
         dev = gpu.look()[0]
-        key = f"device/{dev.name.replace('/', '_')}/hip/radix/{VERSION}".encode()
+        key = tuning.device_key(dev.name, "hip-radix", 0).encode()
         lib().og_tuning_put(key, ctypes.byref(TuningShape(0, 0, results[best], 0, 0, best)))
 
 

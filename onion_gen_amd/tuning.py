@@ -28,3 +28,25 @@ def get(key):
 def put(key, first, second, rate, at=0, rounds=0, radix=0):
     return lib().og_tuning_put(key.encode(), ctypes.byref(
         TuningShape(first, second, rate, at, rounds, radix))) == 0
+
+
+def device_key(name, api="hip", threads=0):
+    """src/tuning.rs:65-74 device_key, as og_tuning_device_key builds it."""
+    n = lib().og_tuning_device_key(name.encode(), api.encode(), threads, None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    lib().og_tuning_device_key(name.encode(), api.encode(), threads, buf, n + 1)
+    return buf.value.decode()
+
+
+def set_version(version):
+    """The version the keys end in (the reference crate's 1.0.2 by default)."""
+    return lib().og_tuning_set_version(version.encode()) == 0
+
+
+def choose(rates, sizes, budget=0, tolerance=0.03):
+    """og_tuning_choose over measured (rate, bytes) pairs: an index or None."""
+    n = len(rates)
+    r = (ctypes.c_double * max(n, 1))(*rates)
+    b = (ctypes.c_uint64 * max(n, 1))(*sizes)
+    i = lib().og_tuning_choose(r, b, n, budget, tolerance)
+    return None if i >= n else i

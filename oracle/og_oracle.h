@@ -141,6 +141,10 @@ typedef struct {
  * Returns the number of hits (may exceed cap; only cap are written). */
 size_t osearch(const uint8_t start_xyzt[128], uint64_t k0, uint64_t n, size_t width,
                const ofilterset *s, ohit *hits, size_t cap);
+/* osearch split over `threads` threads, each on its own contiguous part of the
+ * range; the same hits in the same order. */
+size_t osearch_mt(const uint8_t start_xyzt[128], uint64_t k0, uint64_t n, size_t width,
+                  const ofilterset *s, int threads, ohit *hits, size_t cap);
 /* XOR and 64-bit wrapping sum of the first 8 bytes of every packed y (sign
  * clear) over k0 .. k0+n-1: a digest over all candidates, hits or not. */
 void odigest(const uint8_t start_xyzt[128], uint64_t k0, uint64_t n, size_t width,

@@ -73,6 +73,55 @@ size_t osearch(const uint8_t start_xyzt[128], uint64_t k0, uint64_t n, size_t wi
   return found;
 }
 
+typedef struct {
+  const uint8_t *start;
+  uint64_t k0, n;
+  size_t width;
+  const ofilterset *s;
+  ohit *hits;
+  size_t cap, found;
+} search_part;
+
+static void *search_part_run(void *p) {
+  search_part *a = p;
+  a->found = osearch(a->start, a->k0, a->n, a->width, a->s, a->hits, a->cap);
+  return NULL;
+}
+
+/* osearch over `threads` contiguous parts of [k0, k0+n).  A candidate's key
+ * does not depend on where its batch starts (every canonical y is exact), so
+ * the parts' hits, concatenated in order, are osearch's over the whole range. */
+size_t osearch_mt(const uint8_t start_xyzt[128], uint64_t k0, uint64_t n, size_t width,
+                  const ofilterset *s, int threads, ohit *hits, size_t cap) {
+  if (threads < 1) threads = 1;
+  search_part *parts = calloc((size_t)threads, sizeof *parts);
+  pthread_t *th = calloc((size_t)threads, sizeof *th);
+  const uint64_t chunk = (n / (uint64_t)threads + width - 1) / width * width;
+  for (int t = 0; t < threads; t++) {
+    const uint64_t lo = chunk * (uint64_t)t < n ? chunk * (uint64_t)t : n;
+    const uint64_t hi = lo + chunk < n ? lo + chunk : n;
+    parts[t].start = start_xyzt;
+    parts[t].k0 = k0 + lo;
+    parts[t].n = hi - lo;
+    parts[t].width = width;
+    parts[t].s = s;
+    parts[t].cap = cap;
+    parts[t].hits = malloc((cap ? cap : 1) * sizeof(ohit));
+    pthread_create(&th[t], NULL, search_part_run, &parts[t]);
+  }
+  size_t found = 0;
+  for (int t = 0; t < threads; t++) {
+    pthread_join(th[t], NULL);
+    const size_t got = parts[t].found < cap ? parts[t].found : cap;
+    for (size_t i = 0; i < got && found + i < cap; i++) hits[found + i] = parts[t].hits[i];
+    found += parts[t].found;
+    free(parts[t].hits);
+  }
+  free(parts);
+  free(th);
+  return found;
+}
+
 void odigest(const uint8_t start_xyzt[128], uint64_t k0, uint64_t n, size_t width,
              uint64_t out[2]) {
   opoint acc;

@@ -17,13 +17,21 @@ class OHit(ctypes.Structure):
 _lib = None
 
 
-def load():
+def load(path=None):
+    """The oracle library; `path` loads another build of the same sources (the
+    natively built timing baseline) with the same declarations."""
     global _lib
+    if path is not None:
+        return _declare(ctypes.CDLL(path))
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB):
         subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
-    L = ctypes.CDLL(LIB)
+    _lib = _declare(ctypes.CDLL(LIB))
+    return _lib
+
+
+def _declare(L):
     c = ctypes
     L.ofilter_parse.restype = c.c_void_p
     L.ofilter_parse.argtypes = [c.c_char_p, c.c_char_p, c.c_size_t]
@@ -39,6 +47,9 @@ def load():
     L.osearch.restype = c.c_size_t
     L.osearch.argtypes = [c.c_char_p, c.c_uint64, c.c_uint64, c.c_size_t, c.c_void_p,
                           c.c_void_p, c.c_size_t]
+    L.osearch_mt.restype = c.c_size_t
+    L.osearch_mt.argtypes = [c.c_char_p, c.c_uint64, c.c_uint64, c.c_size_t, c.c_void_p,
+                             c.c_int, c.c_void_p, c.c_size_t]
     L.odigest.argtypes = [c.c_char_p, c.c_uint64, c.c_uint64, c.c_size_t, c.c_void_p]
     L.obench_batch.restype = c.c_double
     L.obench_batch.argtypes = [c.c_char_p, c.c_size_t, c.c_int, c.c_double, c.c_void_p,
@@ -73,7 +84,6 @@ def load():
         getattr(L, name).argtypes = [c.c_char_p, c.c_char_p, c.c_char_p, c.c_size_t]
     for name in ("oracle_fe_invert", "oracle_fe_canon"):
         getattr(L, name).argtypes = [c.c_char_p, c.c_char_p, c.c_size_t]
-    _lib = L
     return L
 
 
@@ -137,6 +147,16 @@ def search(start, k0, n, width, filters, cap=1 << 16):
     """Oracle hits for k0 <= k < k0+n: sorted (k, pk, address)."""
     buf = (OHit * cap)()
     cnt = load().osearch(bytes(start), k0, n, width, filters.h, buf, cap)
+    if cnt > cap:
+        raise ValueError(f"{cnt} oracle hits exceed the cap {cap}")
+    return sorted((buf[i].k, bytes(buf[i].pk), buf[i].address.decode()) for i in range(cnt))
+
+
+def search_mt(start, k0, n, width, filters, threads=None, cap=1 << 16):
+    """search() split over threads (one contiguous part each): the same hits."""
+    threads = threads or min(16, os.cpu_count() or 1)
+    buf = (OHit * cap)()
+    cnt = load().osearch_mt(bytes(start), k0, n, width, filters.h, threads, buf, cap)
     if cnt > cap:
         raise ValueError(f"{cnt} oracle hits exceed the cap {cap}")
     return sorted((buf[i].k, bytes(buf[i].pk), buf[i].address.decode()) for i in range(cnt))

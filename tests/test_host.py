@@ -246,16 +246,66 @@ def test_tuning_cache_round_trip(tmp_path, monkeypatch):
     p = tmp_path / "sub" / "tuning.yaml"
     monkeypatch.setenv("OG_TUNING_PATH", str(p))
     assert tuning.path() == str(p)
-    assert tuning.get("device/x/hip/0/0.1.0") is None
-    assert tuning.put("device/x/hip/0/0.1.0", 512, 262144, 2.9e10, 1700000000, 1, 10)
-    assert tuning.put("device/y/opencl/0/0.1.0", 256, 65536, 1.5e9, 1700000001)
-    got = tuning.get("device/x/hip/0/0.1.0")
+    assert tuning.get("device/x/hip/0/1.0.2") is None
+    assert tuning.put("device/x/hip/0/1.0.2", 512, 262144, 2.9e10, 1700000000, 1, 10)
+    assert tuning.put("device/y/opencl/0/1.0.2", 256, 65536, 1.5e9, 1700000001)
+    got = tuning.get("device/x/hip/0/1.0.2")
     assert got["first"] == 512 and got["second"] == 262144 and got["rounds"] == 1
     assert abs(got["rate"] - 2.9e10) < 1e3 and got["at"] == 1700000000 and got["radix"] == 10
     text = p.read_text()
-    assert text.startswith("format: 1") and "entries:" in text and "device/y/opencl/0/0.1.0" in text
+    assert text.startswith("format: 1") and "entries:" in text and "device/y/opencl/0/1.0.2" in text
     p.write_text("format: 2\nentries: {}\n")
-    assert tuning.get("device/x/hip/0/0.1.0") is None
+    assert tuning.get("device/x/hip/0/1.0.2") is None
+
+
+# The reference crate's version (Cargo.toml:3), which CARGO_PKG_VERSION puts at
+# the end of every tuning key (src/tuning.rs:65-74).
+REFERENCE_VERSION = "1.0.2"
+
+
+def test_tuning_device_key_is_the_references():
+    """og_tuning_device_key builds src/tuning.rs:65-74's key: the name with
+    '/' and newline replaced, the api, the thread count and the version; the
+    reference's own cases (src/tuning.rs:150-176) hold for it."""
+    from onion_gen_amd import tuning
+    assert tuning.device_key("AMD Instinct MI355X (gfx950:sramecc+:xnack-)", "hip", 0) == \
+        "device/AMD Instinct MI355X (gfx950:sramecc+:xnack-)/hip/0/" + REFERENCE_VERSION
+    assert tuning.device_key("a/b\nc", "cuda", 7) == "device/a_b_c/cuda/7/" + REFERENCE_VERSION
+    assert tuning.device_key("a", "cuda", 1) != tuning.device_key("a", "cuda", 2)
+    assert tuning.device_key("x", "metal", 1).endswith(REFERENCE_VERSION)
+    forged, real = tuning.device_key("a/metal/1/9.9.9", "cuda", 1), tuning.device_key("a", "metal", 1)
+    assert forged != real and "a/metal" not in forged
+    # a host passes its own CARGO_PKG_VERSION
+    assert tuning.set_version("2.0.0")
+    try:
+        assert tuning.device_key("x", "hip", 0) == "device/x/hip/0/2.0.0"
+        assert not tuning.set_version("") and not tuning.set_version("1/2")
+        assert tuning.device_key("x", "hip", 0).endswith("2.0.0")
+    finally:
+        assert tuning.set_version(REFERENCE_VERSION)
+
+
+def test_tuning_choice_is_bounded_by_memory():
+    """The walk's choice (og_tuning_choose): the fastest shape within the
+    budget, then the smallest footprint within 3% of it.  The measured walk
+    of a 6-symbol prefix on MI355X is the first case: 2^20 x 1024 chains
+    (129 GB) at 3.37e10 against 2^19 x 1024 (64 GB) at 3.29e10 keys/s."""
+    from onion_gen_amd import tuning
+    GB = 1 << 30
+    rates = [3.37e10, 3.29e10, 3.10e10, 2.50e10]
+    sizes = [129 * GB, 64 * GB, 32 * GB, 8 * GB]
+    assert tuning.choose(rates, sizes) == 1             # within 3%: the smaller
+    assert tuning.choose(rates, sizes, tolerance=0.0) == 0
+    assert tuning.choose(rates, sizes, tolerance=0.10) == 2
+    assert tuning.choose(rates, sizes, budget=100 * GB) == 1
+    assert tuning.choose(rates, sizes, budget=32 * GB) == 2
+    assert tuning.choose(rates, sizes, budget=33 * GB, tolerance=0.25) == 3
+    assert tuning.choose(rates, sizes, budget=4 * GB) is None
+    assert tuning.choose([0.0, 0.0], [1, 2]) is None     # nothing measured
+    # an unmeasured shape is never kept, whatever its size
+    assert tuning.choose([3.0e10, 0.0], [10 * GB, 1 * GB]) == 0
+    # equal footprints: the faster
+    assert tuning.choose([3.0e10, 2.95e10], [GB, GB]) == 0
 
 
 def test_host_checksum_matches_sha3():
