@@ -6,7 +6,9 @@ seeded 0, 2^34 sequential candidates P0 + k*8G.  A step is one engine advance
 (one launch over threads * (2*half+1) * rounds candidates); the default step
 count covers 2^34 candidates.  Every survivor goes through the hit sink (its
 secret at the offset, its public key re-derived from the scalar alone) and its
-address is matched on the host; the run exits non-zero if one fails.
+address is matched on the host; the run exits non-zero if one fails.  (With
+--filter, a profiling run of another form, at most 65,536 evenly spaced
+survivors are checked, and the line says how many.)
 
 Prints one JSON line (rank 0).  `value` is candidates/s with the start state
 already on the device; `roofline` prices the search kernel's field
@@ -501,7 +503,12 @@ def main():
     # every survivor through the hit sink: its secret at the offset, its
     # public key re-derived on the device from the scalar alone and compared
     # with the survivor, then the whole address matched on the host
-    recs = key.finish_hits(seed, survivors)
+    # (a filter of the forms, run for profiling, may return millions: there
+    # an evenly spaced sample of 65,536 of them, which the line states)
+    sample = survivors
+    if args.filter and len(survivors) > 65536:
+        sample = survivors[::-(-len(survivors) // 65536)]
+    recs = key.finish_hits(seed, sample)
     verified = sum(1 for r in recs if r.status == key.HIT_OK and
                    fs.match_address(key.address_bytes(r.public_key)) >= 0)
     clamping = sum(1 for r in recs if r.status == key.HIT_CLAMPING)
@@ -528,7 +535,8 @@ def main():
                    "threads": args.threads, "half": args.half, "rounds": args.rounds,
                    "candidates_per_step": batch, "tuning_walk": tuned},
         "hits": len(survivors),
-        "hits_rederived": verified,  # all of them, through the hit sink
+        "hits_rederived": verified,  # through the hit sink
+        "hits_checked": len(sample),
         "hits_clamping_dropped": clamping,
         "roofline": {
             "bound": "valu",
@@ -562,8 +570,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prefix, args.cpu_seconds)
         progress("cpu baseline done")
-    if verified + clamping != len(survivors):
-        out["error"] = (f"{len(survivors) - verified - clamping} of {len(survivors)} survivors "
+    if verified + clamping != len(sample):
+        out["error"] = (f"{len(sample) - verified - clamping} of {len(sample)} survivors checked "
                         "did not re-derive to a matching key")
     print(json.dumps(out), flush=True)
     if "error" in out:
