@@ -191,6 +191,35 @@ int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
 int og_hip_issue_bench(int ordinal, int op, uint32_t threads, uint32_t iters, double *ops_per_s,
                        double *clock_hz, char *err, size_t errcap);
 
+/* ---------------------------------------------------- allocation-free search */
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that must not allocate,
+ * harnesses that forbid hipMalloc).  One advance of an engine of this shape
+ * (og_hip_engine_create_ex + og_hip_engine_advance, one window per chain) for
+ * a set of exactly one literal prefix: examines candidates k0 .. k0 +
+ * threads * (2 * half + 1) - 1 of the start point, with the offset table,
+ * chain state, scratch planes and survivor ring in a static device workspace
+ * of og_hip_search_noalloc_max() bytes (64 MiB) per device, which each device
+ * holds until the process ends.  Calls no hipMalloc, hipFree or allocating
+ * library routine.  `out` (cap records) and `n_out` are device memory; the
+ * work is ordered on `stream` (a hipStream_t; NULL for the null stream) and
+ * the call returns without waiting.  When the stream reaches its end, *n_out
+ * holds the survivors found and the first min(*n_out, cap, 16384) are in out
+ * (offset = 8k, packed = canonical y); a count above min(cap, 16384) is an
+ * overflow, an error as for og_hip_engine_advance.  A shape whose
+ * og_hip_search_noalloc_bytes exceeds the workspace is refused with a
+ * message, as is any other filter set.  Every call on a device shares its
+ * workspace: calls that overlap on one device race. */
+int og_hip_search_noalloc(int ordinal, const uint8_t start_xyzt[128], uint64_t k0,
+                          uint32_t threads, uint32_t half, const og_filterset *filters,
+                          og_survivor *out, uint32_t cap, uint32_t *n_out, void *stream,
+                          char *err, size_t errcap);
+/* Workspace bytes og_hip_search_noalloc needs for a shape, and the size of the
+ * static workspace. */
+size_t og_hip_search_noalloc_bytes(uint32_t threads, uint32_t half);
+size_t og_hip_search_noalloc_max(void);
+
 /* --------------------------------------------------------------- hit sink */
 
 enum {

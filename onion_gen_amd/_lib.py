@@ -36,6 +36,11 @@ SIGNATURES = {
     "og_hip_issue_bench": (_c.c_int, [_c.c_int, _c.c_int, _c.c_uint32, _c.c_uint32,
                                       _c.POINTER(_c.c_double), _c.POINTER(_c.c_double),
                                       _c.c_char_p, _c.c_size_t]),
+    "og_hip_search_noalloc": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_uint64, _c.c_uint32,
+                                         _c.c_uint32, _c.c_void_p, _c.c_void_p, _c.c_uint32,
+                                         _c.c_void_p, _c.c_void_p, _c.c_char_p, _c.c_size_t]),
+    "og_hip_search_noalloc_bytes": (_c.c_size_t, [_c.c_uint32, _c.c_uint32]),
+    "og_hip_search_noalloc_max": (_c.c_size_t, []),
     "og_field_op_host": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_char_p, _c.c_char_p,
                                     _c.c_size_t]),
     "og_filters_parse": (_c.c_void_p, [_c.c_char_p, _c.c_uint32, _c.c_char_p, _c.c_size_t]),

@@ -91,6 +91,16 @@ inline std::string build_banks(const FilterSet &fs, std::vector<ScanBank> &out,
   return "";
 }
 
+// The masked compare of one literal prefix (kModePrefix): the decoded head
+// bytes and their significant bits as little-endian key words, or'ed in.
+inline void prefix_words(const Filter &f, uint32_t want[8], uint32_t mask[8]) {
+  for (int i = 0; i < f.nbytes; ++i) {
+    const uint8_t m = i == f.nbytes - 1 ? f.mask : 0xff;
+    want[i / 4] |= (uint32_t)f.bytes[i] << (8 * (i % 4));
+    mask[i / 4] |= (uint32_t)m << (8 * (i % 4));
+  }
+}
+
 struct DevFilterHost {
   uint32_t mode = 0;
   uint32_t want[8] = {0}, mask[8] = {0};
@@ -222,13 +232,8 @@ struct DevFilterHost {
       }
     }
     if (fs.fast.size() == 1) {
-      const Filter &f = fs.filters[fs.fast[0]];
       mode |= kModePrefix;
-      for (int i = 0; i < f.nbytes; ++i) {
-        const uint8_t m = i == f.nbytes - 1 ? f.mask : 0xff;
-        want[i / 4] |= (uint32_t)f.bytes[i] << (8 * (i % 4));
-        mask[i / 4] |= (uint32_t)m << (8 * (i % 4));
-      }
+      prefix_words(fs.filters[fs.fast[0]], want, mask);
     } else if (!fs.fast.empty()) {
       mode |= kModeBitmap;
       // The reference's index when it built one; otherwise a 24-bit map at
