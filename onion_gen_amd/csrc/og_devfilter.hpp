@@ -111,7 +111,10 @@ struct DevFilterHost {
   std::vector<uint32_t> pats;     // kPatWords per pattern
   std::vector<ScanBank> banks;    // the automaton over the same patterns
   std::vector<uint32_t> regex;    // per expression: entry offset, first state, live offset
-  std::vector<uint16_t> dfa;      // the expressions' automata
+  std::vector<uint16_t> dfa;      // the expressions' automata: next states only
+  std::vector<uint32_t> dfa_end;  // per state, bit c: the address may end on
+                                  //   symbol c from that state (word off/32 + s
+                                  //   of an automaton at entry offset off)
   std::vector<uint32_t> live;     // their tail-liveness bits
   bool regex_every = false;
   bool proxyless = false;         // an expression found no room for its proxies
@@ -128,11 +131,13 @@ struct DevFilterHost {
   uint32_t *d_bitmap = nullptr, *d_buckets = nullptr, *d_pats = nullptr, *d_banks = nullptr;
   uint32_t *d_regex = nullptr, *d_live = nullptr;
   uint16_t *d_dfa = nullptr;
+  uint32_t *d_dfa_end = nullptr;
   uint64_t *d_keys = nullptr;
 
   size_t device_bytes() const {
     return bitmap.size() * 4 + buckets.size() * 4 + keys.size() * 8 + pats.size() * 4 +
-           banks.size() * sizeof(ScanBank) + regex.size() * 4 + dfa.size() * 2 + live.size() * 4;
+           banks.size() * sizeof(ScanBank) + regex.size() * 4 + dfa.size() * 2 + dfa_end.size() * 4 +
+           live.size() * 4;
   }
 
   uint32_t first_states = 0;  // states of the first automaton
@@ -142,7 +147,16 @@ struct DevFilterHost {
     regex.push_back((uint32_t)dfa.size());
     regex.push_back(re.dfa_start());
     regex.push_back((uint32_t)live.size());
-    dfa.insert(dfa.end(), re.dfa().begin(), re.dfa().end());
+    // the walks read next states alone; the end flags go to their own bits
+    for (size_t i = 0; i < re.dfa().size(); i += 32) {
+      uint32_t ends = 0;
+      for (int c = 0; c < 32; ++c) {
+        const uint16_t e = re.dfa()[i + c];
+        dfa.push_back((uint16_t)(e & ~kRegexEndBit));
+        ends |= (e & kRegexEndBit) ? 1u << c : 0u;
+      }
+      dfa_end.push_back(ends);
+    }
     live.insert(live.end(), re.tail_live().begin(), re.tail_live().end());
   }
 
@@ -307,6 +321,7 @@ struct DevFilterHost {
     if (e == hipSuccess) e = upload(&d_pats, pats, s);
     if (e == hipSuccess) e = upload(&d_regex, regex, s);
     if (e == hipSuccess) e = upload(&d_dfa, dfa, s);
+    if (e == hipSuccess) e = upload(&d_dfa_end, dfa_end, s);
     if (e == hipSuccess) e = upload(&d_live, live, s);
     if (e == hipSuccess && !banks.empty()) {
       std::vector<uint32_t> words(banks.size() * kBankWords);
@@ -326,8 +341,9 @@ struct DevFilterHost {
     (void)hipFree(d_banks);
     (void)hipFree(d_regex);
     (void)hipFree(d_dfa);
+    (void)hipFree(d_dfa_end);
     (void)hipFree(d_live);
-    d_bitmap = d_buckets = d_pats = d_banks = d_regex = d_live = nullptr;
+    d_bitmap = d_buckets = d_pats = d_banks = d_regex = d_live = d_dfa_end = nullptr;
     d_dfa = nullptr;
     d_keys = nullptr;
   }
@@ -354,6 +370,7 @@ struct DevFilterHost {
     a.key_proxy = key_proxy ? 1u : 0u;
     a.regex = d_regex;
     a.dfa = d_dfa;
+    a.dfa_end = d_dfa_end;
     a.defer_sign = 0;
     return a;
   }

@@ -406,7 +406,7 @@ int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int si
     for (uint32_t g : s->set.general)
       if (s->set.filters[g].matches_address(addr)) r = 1;
   }
-  if (r != 1 && nre && regex_address(w, h.regex.data(), h.dfa.data(), nre)) r = 1;
+  if (r != 1 && nre && regex_address(w, h.regex.data(), h.dfa.data(), h.dfa_end.data(), nre)) r = 1;
   *key_hit = kh;
   *need = nd;
   *hit = r == 1;

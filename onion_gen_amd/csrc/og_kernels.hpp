@@ -84,6 +84,7 @@ struct DevFilterArgs {
                                   //   only sends the lane to the automata
   const uint32_t *regex;          //   records, entries and tail-liveness
   const uint16_t *dfa;            //   bits: og_scan.hpp
+  const uint32_t *dfa_end;        //   (the end flags of the automata's entries)
   const uint32_t *regex_live;
   uint32_t lds_states;            //   the first automaton's states, when it is
                                   //   copied to LDS whole (0: it is not)

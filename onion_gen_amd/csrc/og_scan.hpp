@@ -192,9 +192,9 @@ template <class T16, class T32>
 OG_HD bool regex_open(T16 t, T32 lv, uint32_t s, const uint32_t w[8]) {
   const uint32_t t49 = sym_at(w, 49), t50 = sym_at(w, 50), y248 = w[7] & 1u;
   for (uint32_t sign = 0; sign < 4; sign += 2) {
-    const uint32_t e = t[s * 32u + (t49 | sign)] & 0x7fffu;
+    const uint32_t e = t[s * 32u + (t49 | sign)];
     if (e == 1) return true;
-    const uint32_t e2 = t[e * 32u + t50] & 0x7fffu;
+    const uint32_t e2 = t[e * 32u + t50];
     const uint32_t bit = 2 * e2 + y248;
     if (e2 == 1 || ((lv[bit >> 5] >> (bit & 31)) & 1u)) return true;
   }
@@ -203,56 +203,97 @@ OG_HD bool regex_open(T16 t, T32 lv, uint32_t s, const uint32_t w[8]) {
 
 // One walk of an automaton over symbols 0..48 from state s, reading the
 // table at `t`.  TP is the pointer type: the device walks either the global
-// table or, when the whole automaton fits, its LDS copy.
-// The symbols of big-endian words in order, five bits at a time: a 64-bit
-// window and the words still to come.  The automaton walks read them in a
-// rolled loop, so that a walk holds a dozen registers rather than one per
-// symbol: the walks are called from the search loop, and every register a
-// callee uses is one the loop must give up around the call.
-struct SymStream {
-  uint64_t win;   // the next bits, left-aligned
-  int have;       // how many of them are valid (the same in every lane)
-  uint32_t q[7];  // the words after the window, then zeros
-
-  OG_HD SymStream(const uint32_t *w, int nwords) {
-    win = ((uint64_t)w[0] << 32) | w[1];
-    have = 64;
-#pragma unroll
-    for (int i = 0; i < 7; ++i) q[i] = i + 2 < nwords ? w[i + 2] : 0u;
+// table or, when the whole automaton fits, its LDS copy.  Table entries hold
+// the next state alone (the end flags are a bit array of their own), so a
+// step is the next row's offset or'ed with the symbol and one load.  The walk
+// is unrolled, and each symbol is taken from the key words at a constant bit
+// offset - a shift and a mask, or a funnel shift where it straddles two words
+// - so it costs no register per symbol and no queue of words to shift along.
+template <int J>
+OG_HD uint32_t sym_c(const uint32_t *w) {
+  constexpr int bit = 5 * J, i = bit >> 5, sh = bit & 31;
+  if constexpr (sh <= 27) {
+    return (w[i] >> (27 - sh)) & 31u;
+  } else {
+    return (uint32_t)((((uint64_t)w[i] << 32) | w[i + 1]) >> (59 - sh)) & 31u;
   }
+}
 
-  OG_HD uint32_t next() {
-    const uint32_t c = (uint32_t)(win >> 59);
-    win <<= 5;
-    have -= 5;
-    if (have < 32) {
-      win |= (uint64_t)q[0] << (32 - have);
-      have += 32;
-#pragma unroll
-      for (int i = 0; i < 6; ++i) q[i] = q[i + 1];
-      q[6] = 0;
-    }
-    return c;
+// Left to itself the scheduler computes every symbol up front, which holds
+// one register per symbol for the whole walk; a scheduling barrier after
+// each step keeps each symbol next to the step that reads it.
+OG_HD void step_barrier() {
+#ifdef __HIP_DEVICE_COMPILE__
+  __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
+template <int J, int N, class TP>
+OG_HD uint32_t walk_from(TP t, uint32_t s, const uint32_t *w) {
+  if constexpr (J == N) {
+    return s;
+  } else {
+    const uint32_t next = t[(s << 5) | sym_c<J>(w)];
+    step_barrier();
+    return walk_from<J + 1, N>(t, next, w);
   }
-};
+}
+
+// Two walks step by step, so that each hides the other's table latency.
+template <int J, int N, class TP>
+OG_HD void walk2_from(TP t, uint32_t &a, uint32_t &b, const uint32_t *wa, const uint32_t *wb) {
+  if constexpr (J < N) {
+    a = t[(a << 5) | sym_c<J>(wa)];
+    b = t[(b << 5) | sym_c<J>(wb)];
+    step_barrier();
+    walk2_from<J + 1, N>(t, a, b, wa, wb);
+  }
+}
+
+// A table pointer every lane of the wave holds alike (each wave searches one
+// filter set), moved to scalar registers, so that a step's load takes the
+// table as its scalar base and the 32-bit row-and-symbol offset alone.
+template <class TP>
+OG_HD TP wave_uniform(TP p) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const uint64_t x = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+  return (TP)(((uint64_t)hi << 32) | lo);
+#else
+  return p;
+#endif
+}
+
+// The key words as values of this walk: a walk inside a loop over automata
+// would otherwise have its symbols taken once, before the loop, and held in
+// registers through it.
+template <int N>
+OG_HD void fresh_words(const uint32_t *w, uint32_t out[N]) {
+#pragma unroll
+  for (int l = 0; l < N; ++l) {
+    uint32_t x = w[l];
+#ifdef __HIP_DEVICE_COMPILE__
+    asm volatile("" : "+v"(x));
+#endif
+    out[l] = x;
+  }
+}
 
 template <class TP>
 OG_HD uint32_t regex_walk49(TP t, uint32_t s, const uint32_t w[8]) {
-  SymStream st(w, 8);
-#pragma unroll 1
-  for (int j = 0; j < 49; ++j) s = t[s * 32u + st.next()] & 0x7fffu;
-  return s;
+  uint32_t f[8];
+  fresh_words<8>(w, f);
+  return walk_from<0, 49>(t, s, f);
 }
 
 template <class TP>
 OG_HD void regex_walk49x2(TP t, uint32_t &sa, uint32_t &sb, const uint32_t wa[8],
                           const uint32_t wb[8]) {
-  SymStream a(wa, 8), b(wb, 8);
-#pragma unroll 1
-  for (int j = 0; j < 49; ++j) {
-    sa = t[sa * 32u + a.next()] & 0x7fffu;
-    sb = t[sb * 32u + b.next()] & 0x7fffu;
-  }
+  uint32_t fa[8], fb[8];
+  fresh_words<8>(wa, fa);
+  fresh_words<8>(wb, fb);
+  walk2_from<0, 49>(t, sa, sb, fa, fb);
 }
 
 // Symbols 0..48 (what the key settles) through every expression: 2 when one
@@ -265,7 +306,7 @@ OG_HD uint32_t regex_key(const uint32_t w[8], const uint32_t *recs, T16 dfa, T32
   for (uint32_t r = 0; r < n; ++r) {
     const T16 t = dfa + recs[3 * r];
     uint32_t s = recs[3 * r + 1];
-    s = (r == 0 && in_lds) ? regex_walk49(lds, s, w) : regex_walk49(t, s, w);
+    s = (r == 0 && in_lds) ? regex_walk49(lds, s, w) : regex_walk49(wave_uniform(t), s, w);
     if (s == 1) return 2;
     if (s != 0 && !out && regex_open(t, live + recs[3 * r + 2], s, w)) out = 1;
   }
@@ -284,7 +325,7 @@ OG_HD uint32_t regex_key2(const uint32_t wa[8], const uint32_t wb[8], const uint
     const T32 lv = live + recs[3 * r + 2];
     uint32_t sa = recs[3 * r + 1], sb = sa;
     if (r == 0 && in_lds) regex_walk49x2(lds, sa, sb, wa, wb);
-    else regex_walk49x2(t, sa, sb, wa, wb);
+    else regex_walk49x2(wave_uniform(t), sa, sb, wa, wb);
     if (sa == 1) ra = 2;
     else if (sa != 0 && !ra && regex_open(t, lv, sa, wa)) ra = 1;
     if (sb == 1) rb = 2;
@@ -293,16 +334,17 @@ OG_HD uint32_t regex_key2(const uint32_t wa[8], const uint32_t wb[8], const uint
   return ra | (rb << 2);
 }
 
-// All 56 symbols of the address words: whether an expression matches.
-template <class T16>
-OG_HD bool regex_address(const uint32_t w[9], const uint32_t *recs, T16 dfa, uint32_t n) {
+// All 56 symbols of the address words: whether an expression matches.  The
+// last symbol's entry is read from the end flags: whether the address may
+// end there.
+template <class T16, class T32>
+OG_HD bool regex_address(const uint32_t w[9], const uint32_t *recs, T16 dfa, T32 ends,
+                         uint32_t n) {
   for (uint32_t r = 0; r < n; ++r) {
-    const T16 t = dfa + recs[3 * r];
-    uint32_t e = recs[3 * r + 1];
-    SymStream st(w, 9);
-#pragma unroll 1
-    for (int j = 0; j < 56; ++j) e = t[(e & 0x7fffu) * 32u + st.next()];
-    if (e >> 15) return true;
+    uint32_t f[9];
+    fresh_words<9>(w, f);
+    const uint32_t s = walk_from<0, 55>(wave_uniform(dfa + recs[3 * r]), recs[3 * r + 1], f);
+    if ((ends[(recs[3 * r] >> 5) + s] >> sym_c<55>(w)) & 1u) return true;
   }
   return false;
 }

@@ -184,9 +184,9 @@ __device__ __noinline__ bool regex_address_slow(uint32_t w0, uint32_t w1, uint32
                                                 uint32_t w3, uint32_t w4, uint32_t w5,
                                                 uint32_t w6, uint32_t w7, uint32_t w8,
                                                 const uint32_t *recs, const uint16_t *dfa,
-                                                uint32_t n) {
+                                                const uint32_t *ends, uint32_t n) {
   const uint32_t w[9] = {w0, w1, w2, w3, w4, w5, w6, w7, w8};
-  return regex_address(w, recs, (gptr16)dfa, n);
+  return regex_address(w, recs, (gptr16)dfa, (gptr32)ends, n);
 }
 
 // Appends a wave's hits with one atomic per wave.  Records past `cap` are
@@ -379,7 +379,7 @@ __device__ __forceinline__ void drain_one(const Fe10 &e, const DevFilterArgs &f,
       (r == 2 && pattern_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8], f.pats,
                               f.npat)) ||
       (f.nregex && regex_address_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8],
-                                      f.regex, f.dfa, f.nregex));
+                                      f.regex, f.dfa, f.dfa_end, f.nregex));
   Fe c;
 #pragma unroll
   for (int l = 0; l < 8; ++l) c.v[l] = pk[l];
