@@ -127,24 +127,29 @@ def run_mix(searches, seconds, slots, chains, half, policy, ring_cap=1 << 20):
 def batched_leg(seconds):
     """The 64-search mix in one launch per step, against one search per
     launch: at the same per-search shape, and at the full-GPU shape."""
-    from onion_gen_amd import batch, workload
+    from onion_gen_amd import batch, filters, workload
     mix = workload.search_mix(64, 4)
     kinds = {}
     for _, text, _ in mix:
         k = text.split(":")[0] if ":" in text else "prefix"
         kinds[k] = kinds.get(k, 0) + 1
-    # the unit shape: 64 slots of 16384 chains, 256 pairs per window (2^20
-    # chains in all; a sweep of 2048..32768 chains and half 256..1024 ranked it
-    # first for this mix, at a median time to first hit under 0.1 s)
-    batched = run_mix(mix, seconds, 64, 16384, 256, batch.LEAST_SERVED)
+    # the unit shape the slot engine's timed walk (og_batch_tune) picks for
+    # this mix: slots x chains per slot x pairs per window
+    t0 = time.perf_counter()
+    tuned = batch.tune([filters.FilterSet.parse(text) for _, text, _ in mix])
+    tuned["walk_s"] = time.perf_counter() - t0
+    progress(f"slot engine walk: {tuned}")
+    batched = run_mix(mix, seconds, tuned["slots"], tuned["chains"], tuned["half"],
+                      batch.LEAST_SERVED)
     progress("batched mix done")
-    plain = run_mix(mix, seconds, 1, 16384, 256, batch.ROUND_ROBIN)
+    plain = run_mix(mix, seconds, 1, tuned["chains"], tuned["half"], batch.ROUND_ROBIN)
     full = run_mix(mix, seconds, 1, 1 << 18, 256, batch.ROUND_ROBIN, ring_cap=1 << 21)
     progress("one search per launch done")
     return {
         "workload": "64 searches (search_mix seed 4: " + ", ".join(f"{v} {k}" for k, v in
                                                                    sorted(kinds.items())) +
                     "; wanted hits log-uniform 1..1000), fixed wall-time window",
+        "unit_shape": tuned,
         "batched": batched,
         "plain_per_search_shape": plain,
         "plain_full_gpu": full,
@@ -157,7 +162,7 @@ def _family(text):
     return text.split(":")[0] if ":" in text else "prefix"
 
 
-def served_leg(seconds, rate=200.0, fifo_seconds=60.0):
+def served_leg(seconds, rate=200.0, fifo_seconds=60.0, unit=(64, 16384, 256)):
     """Poisson arrivals at `rate`/s for `seconds`, each request a filter of
     the batched mix wanting 1, 10 or 100 hits, through the continuous-batching
     loop.  The comparator is like for like: the requests the loop completed,
@@ -171,7 +176,7 @@ def served_leg(seconds, rate=200.0, fifo_seconds=60.0):
     keep = ("wall_s", "keys_per_s", "completed", "running", "queued", "rejected",
             "max_resident", "p50_time_to_n", "p99_time_to_n", "p50_time_to_first",
             "p99_time_to_first")
-    st, res = serve.run(reqs, duration=seconds, slots=64, chains=16384, half=256,
+    st, res = serve.run(reqs, duration=seconds, slots=unit[0], chains=unit[1], half=unit[2],
                         max_queue=1 << 16, policy=batch.LEAST_SERVED)
     progress(f"served, continuous: {st['completed']} of {len(reqs)} completed")
     cont = {k: st[k] for k in keep}
@@ -219,6 +224,7 @@ def served_leg(seconds, rate=200.0, fifo_seconds=60.0):
     return {
         "workload": f"Poisson {rate:.0f} req/s for {seconds:.0f} s ({len(reqs)} requests), "
                     "filters of the batched mix, n in {1, 10, 100}",
+        "unit_shape": {"slots": unit[0], "chains": unit[1], "half": unit[2]},
         "continuous": cont,
         "fifo_same_completed": fifo,
         "fifo_workload": f"the {len(same)} requests the continuous loop completed, all at t = 0, "
@@ -566,7 +572,11 @@ def main():
     if args.mix_seconds > 0:
         out["batched"] = batched_leg(args.mix_seconds)
     if args.serve_seconds > 0:
-        out["served"] = served_leg(args.serve_seconds, fifo_seconds=args.fifo_seconds)
+        unit = (64, 16384, 256)
+        if "batched" in out:
+            u = out["batched"]["unit_shape"]
+            unit = (u["slots"], u["chains"], u["half"])
+        out["served"] = served_leg(args.serve_seconds, fifo_seconds=args.fifo_seconds, unit=unit)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prefix, args.cpu_seconds)
         progress("cpu baseline done")

@@ -42,8 +42,9 @@ typedef struct {
   uint8_t packed[32]; /* canonical y, little-endian, sign bit clear */
 } og_survivor;
 
-/* Replaces tuning::Shape (src/tuning.rs:27-39), extended with `rounds` and
- * `radix`; first = pairs per window (half), second = chains (threads). */
+/* Replaces tuning::Shape (src/tuning.rs:27-39), extended with `rounds`,
+ * `radix` and `slots`; first = pairs per window (half), second = chains
+ * (threads; per unit slot for the slot engine's entry). */
 typedef struct {
   uint32_t first;
   uint32_t second;
@@ -51,6 +52,7 @@ typedef struct {
   int64_t at;
   uint32_t rounds;
   uint32_t radix;
+  uint32_t slots;  /* the slot engine's unit slots per step; 0 elsewhere */
 } og_tuning_shape;
 
 typedef struct og_engine og_engine;
@@ -368,8 +370,23 @@ typedef struct {
   uint8_t packed[32]; /* canonical y, sign bit clear */
 } og_batch_hit;
 
+/* With slots, chains and half all 0 the unit shape is the one og_batch_tune
+ * recorded for this device (key og_tuning_device_key(<name>, "hip-batch", 0)),
+ * else 64 x 16384 x 256. */
 int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char *err,
                     size_t errcap);
+/* The slot engine's unit shape chosen by a timed walk, as og_hip_engine_tune
+ * chooses the single engine's (src/tuning.rs:27-50, src/gpu/mod.rs:346-428): for
+ * each of slots {32, 64, 128} x chains per slot {4096 .. 32768} x half {256, 512}
+ * that fits `max_bytes` (0 = half the free memory), as many searches as slots -
+ * the given filter sets in turn, each on its own chain - run one warm-up step and
+ * then steps for at least 0.25 s; among the shapes within 3% of the fastest
+ * aggregate rate the smallest footprint is kept (og_tuning_choose), recorded
+ * under og_tuning_device_key(<name>, "hip-batch", 0), and written to `best` with
+ * its rate.  No counterpart in the reference, which has no slot engine. */
+int og_batch_tune(int ordinal, const og_filterset *const *filters, size_t nfilters,
+                  uint64_t max_bytes, og_batch_opts *best, double *rate, char *err,
+                  size_t errcap);
 /* Device bytes a resident search takes: its chain state and filter tables
  * (the admission footprint).  b may be null for the default chain count. */
 size_t og_batch_footprint(const og_batch *b, const og_filterset *filters);

@@ -73,7 +73,8 @@ SIGNATURES = {
 
 class TuningShape(_c.Structure):
     _fields_ = [("first", _c.c_uint32), ("second", _c.c_uint32), ("rate", _c.c_double),
-                ("at", _c.c_int64), ("rounds", _c.c_uint32), ("radix", _c.c_uint32)]
+                ("at", _c.c_int64), ("rounds", _c.c_uint32), ("radix", _c.c_uint32),
+                ("slots", _c.c_uint32)]
 
 
 SIGNATURES.update({
@@ -103,6 +104,8 @@ SIGNATURES.update({
 SIGNATURES.update({
     "og_batch_create": (_c.c_int, [_c.c_int, _c.c_void_p, _c.POINTER(_c.c_void_p), _c.c_char_p,
                                    _c.c_size_t]),
+    "og_batch_tune": (_c.c_int, [_c.c_int, _c.c_void_p, _c.c_size_t, _c.c_uint64, _c.c_void_p,
+                                 _c.POINTER(_c.c_double), _c.c_char_p, _c.c_size_t]),
     "og_batch_footprint": (_c.c_size_t, [_c.c_void_p, _c.c_void_p]),
     "og_batch_admit": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_void_p, _c.c_uint64,
                                   _c.POINTER(_c.c_int), _c.c_char_p, _c.c_size_t]),

@@ -31,6 +31,22 @@ class BatchHit:
     packed: bytes
 
 
+def tune(filter_sets, max_bytes=0, device=0):
+    """og_batch_tune: the unit shape a timed walk picks for this mix of filter
+    sets (FilterSet objects), recorded in the tuning cache, as a dict with its
+    aggregate rate."""
+    n = len(filter_sets)
+    arr = (ctypes.c_void_p * max(n, 1))(*[f.handle.value for f in filter_sets])
+    best = _Opts()
+    rate = ctypes.c_double()
+    e = err_buf()
+    if lib().og_batch_tune(device, arr, n, max_bytes, ctypes.byref(best), ctypes.byref(rate),
+                           e, len(e)) != 0:
+        raise OgError(e.value.decode())
+    return {"slots": best.slots, "chains": best.chains, "half": best.half,
+            "rate": rate.value}
+
+
 class Batch:
     def __init__(self, slots=64, chains=16384, half=256, rounds=1, ring_cap=0,
                  policy=LEAST_SERVED, device=0):

@@ -22,7 +22,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
+#include <ctime>
 #include <deque>
 #include <memory>
 #include <string>
@@ -294,6 +296,26 @@ uint32_t target_width(const og_batch *b, size_t live) {
   return (uint32_t)std::max<size_t>(1, std::min<size_t>(b->o.slots, b->o.slots / live));
 }
 
+// The cache key of the slot engine's unit shape on this device:
+// device/<name>/hip-batch/0/<version> (src/tuning.rs:65-74's key shape).
+std::string batch_tuning_key(int ordinal) {
+  char name[256] = {0};
+  uint64_t mem = 0;
+  int cus = 0;
+  og_hip_device_info(ordinal, name, sizeof name, &mem, &cus);
+  std::string key(og_tuning_device_key(name, "hip-batch", 0, nullptr, 0), '\0');
+  og_tuning_device_key(name, "hip-batch", 0, &key[0], key.size() + 1);
+  return key;
+}
+
+// Device bytes of a slot engine of this shape with every unit slot taken:
+// the scratch planes, the offset table, the ring, and the chain state.
+uint64_t batch_bytes(uint32_t slots, uint32_t chains, uint32_t half, uint32_t ring) {
+  const uint64_t T = (uint64_t)slots * chains;
+  return scratch_planes(half) * 40 * T + (uint64_t)half * 120 + (uint64_t)ring * sizeof(HitRec) +
+         160 * T;
+}
+
 }  // namespace
 
 extern "C" {
@@ -315,6 +337,16 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
     return -1;
   }
   og_batch_opts o = opts ? *opts : og_batch_opts{};
+  if (!o.slots && !o.chains && !o.half) {
+    // the unit shape the tuning walk recorded for this device, if any
+    og_tuning_shape t{};
+    if (og_tuning_get(batch_tuning_key(ordinal).c_str(), &t) == 1 && t.slots && t.second &&
+        t.first) {
+      o.slots = t.slots;
+      o.chains = t.second;
+      o.half = t.first;
+    }
+  }
   if (!o.slots) o.slots = 64;
   if (!o.chains) o.chains = 16384;
   if (!o.half) o.half = 256;
@@ -401,6 +433,112 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
   OG_TRY(hipStreamEndCapture(b->stream, &b->graph));
   OG_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   *out = b.release();
+  return 0;
+}
+
+int og_batch_tune(int ordinal, const og_filterset *const *filters, size_t nfilters,
+                  uint64_t max_bytes, og_batch_opts *best, double *best_rate, char *err,
+                  size_t errcap) {
+  if (!filters || nfilters == 0) {
+    put_err(err, errcap, "the walk needs at least one filter set");
+    return -1;
+  }
+  for (size_t i = 0; i < nfilters; ++i)
+    if (!filterset_of(filters[i])) {
+      put_err(err, errcap, "null filter set");
+      return -1;
+    }
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || ordinal < 0 || ordinal >= count) {
+    put_err(err, errcap, "no such HIP device");
+    return -1;
+  }
+  uint64_t budget = max_bytes;
+  if (!budget) {
+    size_t free_b = 0, total_b = 0;
+    OG_TRY(hipSetDevice(ordinal));
+    OG_TRY(hipMemGetInfo(&free_b, &total_b));
+    budget = free_b / 2;
+  }
+  // unit slots x chains per slot x pairs per window, every search on one slot
+  std::vector<og_batch_opts> walk;
+  for (uint32_t slots : {32u, 64u, 128u})
+    for (uint32_t chains : {4096u, 8192u, 16384u, 32768u})
+      for (uint32_t half : {256u, 512u}) {
+        if ((uint64_t)slots * chains > (1u << 22)) continue;
+        og_batch_opts o{};
+        o.slots = slots;
+        o.chains = chains;
+        o.half = half;
+        o.rounds = 1;
+        o.ring_cap = 1u << 16;
+        o.policy = OG_POLICY_LEAST_SERVED;
+        walk.push_back(o);
+      }
+  std::vector<double> rates(walk.size(), 0.0);
+  std::vector<uint64_t> sizes(walk.size(), 0);
+  std::string last = "no shape was tried";
+  for (size_t w = 0; w < walk.size(); ++w) {
+    const og_batch_opts &o = walk[w];
+    sizes[w] = batch_bytes(o.slots, o.chains, o.half, o.ring_cap);
+    if (sizes[w] > budget) {
+      last = "every shape exceeds the memory budget of " + std::to_string(budget) + " bytes";
+      continue;
+    }
+    og_batch *b = nullptr;
+    char e2[512] = {0};
+    if (og_batch_create(ordinal, &o, &b, e2, sizeof e2) != 0) {
+      last = e2;
+      continue;
+    }
+    // as many searches as slots, the given filter sets in turn, each on its
+    // own block's chain; each records one hit, so no set floods the ring
+    bool ok = true;
+    for (uint32_t i = 0; i < o.slots && ok; ++i) {
+      uint8_t root[32] = {0}, seed[32], start[128];
+      std::memcpy(root, "slot engine tuning walk", 23);
+      og_run_block_seed(root, i, seed);
+      og_key_start_point(seed, start);
+      int id = -1;
+      ok = og_batch_admit(b, start, filters[i % nfilters], i, &id, e2, sizeof e2) == 0 &&
+           og_batch_set_want(b, id, 1) == 0;
+    }
+    std::vector<og_batch_hit> hits(o.ring_cap);
+    uint32_t n = 0;
+    double rate = 0;
+    // one step warms up; then steps for at least 0.25 s (and three)
+    if (ok && og_batch_step(b, hits.data(), o.ring_cap, &n, e2, sizeof e2) == 0) {
+      const uint64_t ex0 = og_batch_examined(b);
+      const auto t0 = std::chrono::steady_clock::now();
+      double sec = 0;
+      int steps = 0;
+      while (ok && (steps < 3 || sec < 0.25)) {
+        ok = og_batch_step(b, hits.data(), o.ring_cap, &n, e2, sizeof e2) == 0;
+        ++steps;
+        sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      }
+      if (ok && sec > 0) rate = (double)(og_batch_examined(b) - ex0) / sec;
+    }
+    if (rate == 0 && e2[0]) last = e2;
+    og_batch_destroy(b);
+    rates[w] = rate;
+  }
+  const size_t pick = og_tuning_choose(rates.data(), sizes.data(), walk.size(), budget, 0.03);
+  if (pick >= walk.size()) {
+    put_err(err, errcap, "no unit shape this device would take: " + last);
+    return -1;
+  }
+  og_tuning_shape t{};
+  t.first = walk[pick].half;
+  t.second = walk[pick].chains;
+  t.slots = walk[pick].slots;
+  t.rounds = 1;
+  t.rate = rates[pick];
+  t.at = (int64_t)std::time(nullptr);
+  t.radix = 10;
+  og_tuning_put(batch_tuning_key(ordinal).c_str(), &t);
+  if (best) *best = walk[pick];
+  if (best_rate) *best_rate = rates[pick];
   return 0;
 }
 

@@ -10,9 +10,9 @@
 //       rate: <f64>       candidates per second measured
 //       at: <i64>         unix seconds
 //
-// extended with `rounds` (windows per chain per launch) and `radix` (the field
-// representation the bench chose), fields a reader of the reference format
-// ignores.  Every operation is best effort: an unreadable or foreign file is
+// extended with `rounds` (windows per chain per launch), `radix` (the field
+// representation the bench chose) and `slots` (the slot engine's unit slots),
+// fields a reader of the reference format ignores.  Every operation is best effort: an unreadable or foreign file is
 // "no entry" (src/tuning.rs:1-18).  OG_TUNING_PATH overrides the location.
 #include <algorithm>
 #include <cstdio>
@@ -86,6 +86,7 @@ bool read_file(std::map<std::string, og_tuning_shape> &out) {
       else if (k == "at") s.at = std::strtoll(v.c_str(), nullptr, 10);
       else if (k == "rounds") s.rounds = (uint32_t)std::strtoul(v.c_str(), nullptr, 10);
       else if (k == "radix") s.radix = (uint32_t)std::strtoul(v.c_str(), nullptr, 10);
+      else if (k == "slots") s.slots = (uint32_t)std::strtoul(v.c_str(), nullptr, 10);
     }
   }
   return format_ok;
@@ -182,6 +183,7 @@ int og_tuning_put(const char *key, const og_tuning_shape *shape) {
     o << "    at: " << kv.second.at << "\n";
     if (kv.second.rounds) o << "    rounds: " << kv.second.rounds << "\n";
     if (kv.second.radix) o << "    radix: " << kv.second.radix << "\n";
+    if (kv.second.slots) o << "    slots: " << kv.second.slots << "\n";
   }
   mkdirs(path);
   // written beside and renamed, so a reader never sees half a file

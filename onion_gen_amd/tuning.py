@@ -22,12 +22,12 @@ def get(key):
     if lib().og_tuning_get(key.encode(), ctypes.byref(s)) != 1:
         return None
     return {"first": s.first, "second": s.second, "rate": s.rate, "at": s.at,
-            "rounds": s.rounds, "radix": s.radix}
+            "rounds": s.rounds, "radix": s.radix, "slots": s.slots}
 
 
-def put(key, first, second, rate, at=0, rounds=0, radix=0):
+def put(key, first, second, rate, at=0, rounds=0, radix=0, slots=0):
     return lib().og_tuning_put(key.encode(), ctypes.byref(
-        TuningShape(first, second, rate, at, rounds, radix))) == 0
+        TuningShape(first, second, rate, at, rounds, radix, slots))) == 0
 
 
 def device_key(name, api="hip", threads=0):

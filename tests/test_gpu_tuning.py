@@ -62,3 +62,31 @@ def test_tuned_engine_fits_the_memory_budget(tmp_path, monkeypatch):
     with gpu.Engine.with_filters(start, fs, max_bytes=8 << 30) as eng:
         say(f"engine under 8 GiB: {eng.shape()} holds {eng.bytes()} bytes")
         assert 0 < eng.bytes() <= 8 << 30
+
+
+def test_batch_walk_records_and_the_zero_shape_uses_it(tmp_path, monkeypatch):
+    """og_batch_tune times the slot engine's unit shapes with the 64-search
+    mix's filters under a 48 GiB budget, records the pick under the device's
+    hip-batch key, and og_batch_create with slots, chains and half all 0 runs
+    that shape (src/tuning.rs:27-50 for the cache, src/gpu/mod.rs:346-428 for
+    the walk)."""
+    from onion_gen_amd import batch
+    monkeypatch.setenv("OG_TUNING_PATH", str(tmp_path / "tuning.yaml"))
+    mix = workload.search_mix(64, 4)
+    sets = [filters.FilterSet.parse(text) for _, text, _ in mix]
+    say("case: slot engine tuning walk, the 64-search mix, 48 GiB")
+    best = batch.tune(sets, max_bytes=48 << 30)
+    say(f"best: {best}")
+    assert best["slots"] in (32, 64, 128) and best["chains"] in (4096, 8192, 16384, 32768)
+    assert best["half"] in (256, 512) and best["rate"] > 1e9
+    dev = gpu.look()[0]
+    cached = tuning.get(tuning.device_key(dev.name, "hip-batch", 0))
+    assert cached and cached["slots"] == best["slots"] and cached["second"] == best["chains"]
+    assert cached["first"] == best["half"]
+    with batch.Batch(slots=0, chains=0, half=0) as b:
+        assert b.per_step() == best["chains"] * (2 * best["half"] + 1)
+        seed, text, _ = mix[0]
+        sid = b.admit(key.start_point(seed), filters.FilterSet.parse(text), 0)
+        b.set_want(sid, 10)  # alone it widens to every slot: its hits would fill the ring
+        b.step()
+        assert b.search_state(sid)["examined"] >= b.per_step()
