@@ -42,7 +42,7 @@ def tune(filter_sets, max_bytes=0, device=0):
     e = err_buf()
     if lib().og_batch_tune(device, arr, n, max_bytes, ctypes.byref(best), ctypes.byref(rate),
                            e, len(e)) != 0:
-        raise OgError(e.value.decode())
+        raise OgError(e.value.decode(errors="replace"))
     return {"slots": best.slots, "chains": best.chains, "half": best.half,
             "rate": rate.value}
 
@@ -56,7 +56,7 @@ class Batch:
                                                                ring_cap, policy)),
                                    ctypes.byref(h), e, len(e))
         if rc != 0:
-            raise OgError(e.value.decode())
+            raise OgError(e.value.decode(errors="replace"))
         self._h = h
         self._out = (_Hit * (ring_cap or (1 << 16)))()
 
@@ -79,7 +79,7 @@ class Batch:
         rc = lib().og_batch_admit(self._h, bytes(start_xyzt), filters.handle, tag,
                                   ctypes.byref(sid), e, len(e))
         if rc != 0:
-            raise OgError(e.value.decode())
+            raise OgError(e.value.decode(errors="replace"))
         return sid.value
 
     def set_want(self, sid, want):
@@ -108,7 +108,7 @@ class Batch:
         e = err_buf()
         rc = lib().og_batch_step(self._h, self._out, len(self._out), ctypes.byref(n), e, len(e))
         if rc != 0:
-            raise OgError(e.value.decode())
+            raise OgError(e.value.decode(errors="replace"))
         return [BatchHit(self._out[i].tag, self._out[i].search, self._out[i].offset,
                          bytes(self._out[i].packed)) for i in range(n.value)]
 
@@ -120,7 +120,7 @@ class Batch:
         e = err_buf()
         rc = lib().og_batch_step(self._h, self._out, len(self._out), ctypes.byref(n), e, len(e))
         if rc != 0:
-            raise OgError(e.value.decode())
+            raise OgError(e.value.decode(errors="replace"))
         raw = np.frombuffer(self._out, dtype=np.uint64, count=n.value * (ctypes.sizeof(_Hit) // 8))
         return raw.reshape(n.value, ctypes.sizeof(_Hit) // 8)[:, 0].copy()
 

@@ -23,7 +23,7 @@ class FilterSet:
         e = err_buf()
         h = lib().og_filters_parse(text, index_bits, e, len(e))
         if not h:
-            raise OgError(e.value.decode())
+            raise OgError(e.value.decode(errors="replace"))
         return cls(ctypes.c_void_p(h))
 
     def __len__(self):
@@ -51,7 +51,7 @@ class FilterSet:
         n = lib().og_filters_describe(self.handle, None, 0)
         buf = ctypes.create_string_buffer(n + 1)
         lib().og_filters_describe(self.handle, buf, n + 1)
-        return buf.value.decode().splitlines()
+        return buf.value.decode(errors="replace").splitlines()
 
     def scan(self, packed, sign):
         """The device matcher of the general forms and the expressions, run on

@@ -51,7 +51,7 @@ def look():
         mem = ctypes.c_uint64()
         cus = ctypes.c_int()
         if lib().og_hip_device_info(i, name, 256, ctypes.byref(mem), ctypes.byref(cus)) == 0:
-            out.append(Device(i, name.value.decode(), mem.value, cus.value))
+            out.append(Device(i, name.value.decode(errors="replace"), mem.value, cus.value))
     return out
 
 
@@ -80,7 +80,7 @@ class Engine:
         rc = lib().og_hip_engine_create(device, bytes(start_xyzt), threads, words, len(bitmap),
                                         bitmap_bits, ctypes.byref(h), e, len(e))
         if rc != 0:
-            raise OgError(e.value.decode())
+            raise OgError(e.value.decode(errors="replace"))
         return cls(h)
 
     @classmethod
@@ -95,7 +95,7 @@ class Engine:
         rc = lib().og_hip_engine_create_ex(device, bytes(start_xyzt), ctypes.byref(opts),
                                            filters.handle, ctypes.byref(h), e, len(e))
         if rc != 0:
-            raise OgError(e.value.decode())
+            raise OgError(e.value.decode(errors="replace"))
         return cls(h)
 
     @staticmethod
@@ -108,7 +108,7 @@ class Engine:
         e = err_buf()
         if lib().og_hip_engine_tune(device, bytes(start_xyzt), filters.handle, max_bytes,
                                     ctypes.byref(best), e, len(e)) != 0:
-            raise OgError(e.value.decode())
+            raise OgError(e.value.decode(errors="replace"))
         return {"threads": best.second, "half": best.first, "rounds": best.rounds,
                 "rate": best.rate, "at": best.at}
 
@@ -131,7 +131,7 @@ class Engine:
         """Starts over from another start point (og_hip_engine_restart)."""
         e = err_buf()
         if lib().og_hip_engine_restart(self._h, bytes(start_xyzt), e, len(e)) != 0:
-            raise OgError(e.value.decode())
+            raise OgError(e.value.decode(errors="replace"))
 
     def room(self):
         """Survivors one advance may return (og_hip_engine_room)."""
@@ -156,7 +156,7 @@ class Engine:
         e = err_buf()
         rc = lib().og_hip_engine_advance(self._h, self._out, cap, ctypes.byref(n), e, len(e))
         if rc != 0:
-            raise OgError(e.value.decode())
+            raise OgError(e.value.decode(errors="replace"))
         return [Survivor(self._out[i].offset, bytes(self._out[i].packed)) for i in range(n.value)]
 
     def close(self):

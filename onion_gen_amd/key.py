@@ -65,7 +65,7 @@ def address_bytes(pk):
 def address(pk):
     out = ctypes.create_string_buffer(57)
     lib().og_key_address(bytes(pk), out)
-    return out.value.decode()
+    return out.value.decode(errors="replace")
 
 
 def block_seed(root, index):
@@ -101,7 +101,7 @@ def finish_hits(seed, survivors, device=0):
     out = (_KeyRecord * n)()
     e = err_buf()
     if lib().og_hip_finish_hits(device, bytes(seed), src, n, out, e, len(e)) != 0:
-        raise OgError(e.value.decode())
+        raise OgError(e.value.decode(errors="replace"))
     return [KeyRecord(r.offset, r.status, bytes(r.public_key), bytes(r.expanded_secret),
                       r.hostname.decode()) for r in out]
 
@@ -113,5 +113,5 @@ def write_key(directory, public_key, expanded_secret):
     rc = lib().og_write_key(str(directory).encode(), bytes(public_key), bytes(expanded_secret),
                             path, len(path))
     if rc < 0:
-        raise OSError(path.value.decode())
-    return rc == 0, path.value.decode()
+        raise OSError(path.value.decode(errors="replace"))
+    return rc == 0, path.value.decode(errors="replace")

@@ -45,7 +45,7 @@ def run(lease, filters, engine, launches=0, cap=1 << 20):
                                 ctypes.byref(n), ctypes.byref(cand), ctypes.byref(done), e, len(e))
     elapsed = time.perf_counter() - t0
     if rc != 0:
-        raise OgError(e.value.decode())
+        raise OgError(e.value.decode(errors="replace"))
     found = [{"lease": out[i].lease, "block": out[i].block, "offset": out[i].offset}
              for i in range(n.value)]
     return {"found": found,

@@ -42,7 +42,7 @@ def run(radix, threads, iters, check_lanes=64):
     e = err_buf()
     rc = lib().og_hip_radix_bench(0, radix, threads, iters, ctypes.byref(rate), out, e, len(e))
     if rc != 0:
-        raise RuntimeError(e.value.decode())
+        raise RuntimeError(e.value.decode(errors="replace"))
     for lane in range(0, threads, max(1, threads // check_lanes)):
         y = pow(lane_value(lane, 2), iters, P)
         for which in (0, 1):
@@ -60,7 +60,7 @@ def issue_rate(op=0, threads=1 << 20, iters=2048, device=0):
     e = err_buf()
     if lib().og_hip_issue_bench(device, op, threads, iters, ctypes.byref(rate),
                                 ctypes.byref(clock), e, len(e)) != 0:
-        raise RuntimeError(e.value.decode())
+        raise RuntimeError(e.value.decode(errors="replace"))
     return rate.value, clock.value
 
 

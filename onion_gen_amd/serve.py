@@ -53,7 +53,7 @@ def run(requests, duration, slots=64, chains=16384, half=256, rounds=1, max_queu
     rc = lib().og_serve_run(device, ctypes.byref(opts), reqs, n, duration, max_queue,
                             device_budget, results, ctypes.byref(stats), e, len(e))
     if rc != 0:
-        raise OgError(e.value.decode())
+        raise OgError(e.value.decode(errors="replace"))
     st = {f: getattr(stats, f) for f, _ in Stats._fields_}
     out = []
     for i in range(n):

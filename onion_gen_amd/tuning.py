@@ -13,7 +13,7 @@ def path():
     buf = ctypes.create_string_buffer(4096)
     if lib().og_tuning_path(buf, len(buf)) != 0:
         return None
-    return buf.value.decode()
+    return buf.value.decode(errors="replace")
 
 
 def get(key):
@@ -35,7 +35,7 @@ def device_key(name, api="hip", threads=0):
     n = lib().og_tuning_device_key(name.encode(), api.encode(), threads, None, 0)
     buf = ctypes.create_string_buffer(n + 1)
     lib().og_tuning_device_key(name.encode(), api.encode(), threads, buf, n + 1)
-    return buf.value.decode()
+    return buf.value.decode(errors="replace")
 
 
 def set_version(version):
