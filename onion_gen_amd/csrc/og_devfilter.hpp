@@ -106,6 +106,8 @@ struct DevFilterHost {
   uint32_t want[8] = {0}, mask[8] = {0};
   uint32_t bits = 0;
   std::vector<uint32_t> bitmap;   // 32-bit halves of the 64-bit words
+  std::vector<uint32_t> coarse;   // the bitmap or-folded to coarse_bits bits
+  uint32_t coarse_bits = 0;
   std::vector<uint32_t> buckets;  // [offset, count, mask_lo, mask_hi]
   std::vector<uint64_t> keys;
   std::vector<uint32_t> pats;     // kPatWords per pattern
@@ -129,13 +131,14 @@ struct DevFilterHost {
   FilterSet proxy_set;
   // device copies
   uint32_t *d_bitmap = nullptr, *d_buckets = nullptr, *d_pats = nullptr, *d_banks = nullptr;
+  uint32_t *d_coarse = nullptr;
   uint32_t *d_regex = nullptr, *d_live = nullptr;
   uint16_t *d_dfa = nullptr;
   uint32_t *d_dfa_end = nullptr;
   uint64_t *d_keys = nullptr;
 
   size_t device_bytes() const {
-    return bitmap.size() * 4 + buckets.size() * 4 + keys.size() * 8 + pats.size() * 4 +
+    return bitmap.size() * 4 + coarse.size() * 4 + buckets.size() * 4 + keys.size() * 8 + pats.size() * 4 +
            banks.size() * sizeof(ScanBank) + regex.size() * 4 + dfa.size() * 2 + dfa_end.size() * 4 +
            live.size() * 4;
   }
@@ -298,12 +301,25 @@ struct DevFilterHost {
     return "";
   }
 
+  // `bits` must be set first.  Slot s of the map is bit s & 31 of word s >> 5
+  // (the low half of the reference's 64-bit word first); its coarse slot is
+  // s >> (bits - coarse_bits), so a coarse bit is set when any full slot
+  // under it is, and a clear coarse bit rules the key out.
   void set_bitmap(const std::vector<uint64_t> &w) {
     bitmap.clear();
     for (uint64_t v : w) {
       bitmap.push_back((uint32_t)v);
       bitmap.push_back((uint32_t)(v >> 32));
     }
+    coarse_bits = std::min<uint32_t>(bits, kCoarseBits);
+    coarse.assign(((size_t)1 << coarse_bits) / 32 + 1, 0u);
+    const uint32_t shift = bits - coarse_bits;
+    for (size_t i = 0; i < bitmap.size(); ++i)
+      for (uint32_t m = bitmap[i]; m; m &= m - 1) {
+        const uint64_t slot = ((uint64_t)i << 5) | (uint32_t)__builtin_ctz(m);
+        const uint64_t c = slot >> shift;
+        coarse[c >> 5] |= 1u << (c & 31);
+      }
   }
 
   template <class T>
@@ -316,6 +332,7 @@ struct DevFilterHost {
 
   hipError_t to_device(hipStream_t s) {
     hipError_t e = upload(&d_bitmap, bitmap, s);
+    if (e == hipSuccess) e = upload(&d_coarse, coarse, s);
     if (e == hipSuccess) e = upload(&d_buckets, buckets, s);
     if (e == hipSuccess) e = upload(&d_keys, keys, s);
     if (e == hipSuccess) e = upload(&d_pats, pats, s);
@@ -335,6 +352,8 @@ struct DevFilterHost {
 
   void free_device() {
     (void)hipFree(d_bitmap);
+    (void)hipFree(d_coarse);
+    d_coarse = nullptr;
     (void)hipFree(d_buckets);
     (void)hipFree(d_keys);
     (void)hipFree(d_pats);
@@ -356,6 +375,8 @@ struct DevFilterHost {
     std::memcpy(a.mask, mask, sizeof mask);
     a.bits = bits;
     a.bitmap = d_bitmap;
+    a.coarse = d_coarse;
+    a.coarse_bits = d_coarse ? coarse_bits : 0u;
     a.nbuckets = (uint32_t)(buckets.size() / 4);
     a.buckets = d_buckets;
     a.keys = d_keys;

@@ -44,9 +44,20 @@ __global__ __launch_bounds__(256) OG_SEARCH_ATTR(MODE) void og_search_kernel(Sea
   __shared__ ScanBank banks[(MODE & kModePattern) ? kMaxBanks : 1];
   __shared__ uint16_t dfa_lds[(MODE & kModePattern) ? kLdsStates * 32 : 2];
   if (MODE & kModePattern) load_filter(banks, dfa_lds, a.f);
+  // the key forms' bitmap probe reads this workgroup's LDS copy of the map,
+  // or of its coarse fold, before (or instead of) the map in L2
+  constexpr bool kCoarse = (MODE & kModeBitmap) && !(MODE & kModePattern);
+  __shared__ uint32_t coarse_lds[kCoarse ? kCoarseWords : 1];
+  const uint32_t *coarse = nullptr;
+  if (kCoarse && a.f.coarse_bits) {
+    const uint32_t words = (1u << a.f.coarse_bits) >> 5;
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) coarse_lds[i] = a.f.coarse[i];
+    __syncthreads();
+    coarse = coarse_lds;
+  }
   HitSink sink{a.counter, a.hits, a.cap, 0, nullptr, 0};
   search_chain<MODE>(P, st, (cptr)a.table, a.scratch, T, g, a.half, a.rounds, T, g, a.k_base, a.f,
-                     banks, dfa_lds, a.parked, sink);
+                     banks, dfa_lds, a.parked, sink, coarse);
   st_point(a.state, T, g, P);
 }
 

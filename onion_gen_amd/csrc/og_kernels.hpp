@@ -63,6 +63,10 @@ constexpr int kMaxRegexes = 16;
 // this many states (32 KiB).  A larger one is walked in global memory: a wave
 // waits whenever any lane leaves the LDS part, so a partial copy does not pay.
 constexpr uint32_t kLdsStates = 512;
+// The bitmap probe's LDS copy: at most 2^18 bits (32 KiB), so the key forms'
+// three workgroups per CU hold three copies.
+constexpr uint32_t kCoarseBits = 18;
+constexpr uint32_t kCoarseWords = 1u << (kCoarseBits - 5);
 
 struct DevFilterArgs {
   uint32_t mode;
@@ -91,6 +95,10 @@ struct DevFilterArgs {
   uint32_t defer_sign;            // (kModePattern, one-round launches) the key
                                   //   stage parks candidates without the sign
                                   //   of x, and the address kernel derives it
+  const uint32_t *coarse;         // (kModeBitmap) the bitmap or-folded to
+  uint32_t coarse_bits;           //   coarse_bits <= kCoarseBits bits: each
+                                  //   workgroup's LDS copy is probed first, and
+                                  //   equals the map when coarse_bits == bits
 };
 
 struct SearchArgs {

@@ -91,12 +91,31 @@ __device__ __forceinline__ bool key_is_bitmap(const DevFilterArgs &f) {
   return kBoth ? (f.mode & kModeBitmap) != 0 : (MODE & kModeBitmap) != 0;
 }
 
+// The bitmap probe through the workgroup's LDS copy (`coarse`, null when the
+// kernel keeps none): the coarse map first, then the full map in L2 for the
+// lanes whose coarse bit is set, unless the copy is the full map itself.
+__device__ __forceinline__ bool coarse_bit(const uint32_t *coarse, uint32_t cbits, uint32_t head) {
+  const uint32_t slot = head >> (32 - cbits);
+  return (coarse[slot >> 5] >> (slot & 31)) & 1u;
+}
+
 template <uint32_t MODE>
-__device__ __forceinline__ bool maybe_key(const Fe10 &y, const DevFilterArgs &f) {
+__device__ __forceinline__ bool maybe_key(const Fe10 &y, const DevFilterArgs &f,
+                                          const uint32_t *coarse = nullptr) {
   const uint32_t lo = fe10_low32(y), lo19 = lo + 19u;
   if (key_is_prefix<MODE>(f))
     return (((lo ^ f.want[0]) & f.mask[0]) == 0) | (((lo19 ^ f.want[0]) & f.mask[0]) == 0);
-  if (key_is_bitmap<MODE>(f)) return bitmap_bit(f, bswap32(lo)) | bitmap_bit(f, bswap32(lo19));
+  if (key_is_bitmap<MODE>(f)) {
+    const uint32_t h1 = bswap32(lo), h2 = bswap32(lo19);
+    if (!coarse) return bitmap_bit(f, h1) | bitmap_bit(f, h2);
+    const bool c1 = coarse_bit(coarse, f.coarse_bits, h1);
+    const bool c2 = coarse_bit(coarse, f.coarse_bits, h2);
+    if (f.coarse_bits == f.bits) return c1 | c2;
+    bool r = false;
+    if (c1) r = bitmap_bit(f, h1);
+    if (c2) r |= bitmap_bit(f, h2);
+    return r;
+  }
   return false;
 }
 
@@ -243,8 +262,9 @@ __device__ __forceinline__ void report(bool hit, uint64_t k, const Fe &c, const 
 // wave ballot names.
 template <uint32_t MODE>
 __device__ __forceinline__ void test_key_only(const Fe10 &y, uint64_t k, const DevFilterArgs &f,
-                                              const HitSink &sink) {
-  const bool maybe = maybe_key<MODE>(y, f);
+                                              const HitSink &sink,
+                                              const uint32_t *coarse = nullptr) {
+  const bool maybe = maybe_key<MODE>(y, f, coarse);
   if (__ballot(maybe) == 0) return;  // the common case: no lane needs more
   bool hit = false;
   Fe c;
@@ -483,7 +503,8 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
                                              uint32_t local, uint64_t k_base,
                                              const DevFilterArgs &f, const ScanBank *banks,
                                              const uint16_t *dfa_lds, uint32_t *parked_out,
-                                             const HitSink &sink) {
+                                             const HitSink &sink,
+                                             const uint32_t *coarse = nullptr) {
   constexpr bool kPat = (MODE & kModePattern) != 0;
   // the sign of x is left to the address kernel (set only for one-round
   // launches with an address kernel: parked_out non-null)
@@ -526,7 +547,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         }
         report(hit, kc, c, sink);
       } else {
-        test_key_only<MODE>(y, kc, f, sink);
+        test_key_only<MODE>(y, kc, f, sink, coarse);
       }
     }
     inv = fe10_mul(inv, P.Z);
@@ -582,8 +603,8 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         report(hm, km, cm, sink);
         report(hp, kp, cp, sink);
       } else {
-        test_key_only<MODE>(ym, km, f, sink);
-        test_key_only<MODE>(yp, kp, f, sink);
+        test_key_only<MODE>(ym, km, f, sink, coarse);
+        test_key_only<MODE>(yp, kp, f, sink, coarse);
       }
     }
     if (kPat) {

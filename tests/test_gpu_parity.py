@@ -260,3 +260,30 @@ def test_sign_stage_chosen_from_the_filters():
     with gpu.Engine.with_filters(start, filters.FilterSet.parse("contains:q2x7a"),
                                  threads=256, half=64, rounds=2, sign_stage=2) as eng:
         assert eng.sign_stage() == 1
+
+
+@pytest.mark.parametrize("bits", [16, 20, 24])
+def test_bitmap_survivors_are_exactly_the_slots_set(bits):
+    """The bitmap probe through each workgroup's LDS copy: the whole map at
+    16 bits, its 18-bit or-fold in front of the map in L2 at 20 and 24 bits.
+    A bitmap-only engine (Engine::new, src/gpu/mod.rs:346-353) must report
+    exactly the candidates whose canonical key's leading `bits` bits name a set
+    slot - every key of the range enumerated by the oracle's BatchEngine."""
+    from tests.oracle_lib import batch_chain
+    lines = workload.dictionary(10000, seed=2)
+    words, got_bits = filters.FilterSet.parse(lines, index_bits=bits).index_words()
+    assert got_bits == bits
+    start = key.start_point(workload.chacha_stream(4, 32))
+    n = 1 << 16
+    say(f"case: bitmap-only engine, 10,000 prefixes at {bits} bits, n=2^16")
+    eng = gpu.Engine.new(start, 1024, words, bits)
+    assert eng.batch() >= n
+    got = sorted((s.offset // 8, s.packed) for s in eng.advance() if s.offset // 8 < n)
+    eng.close()
+    want = []
+    for k, pk in enumerate(batch_chain(start, 256, n // 256)):
+        packed = pk[:31] + bytes([pk[31] & 0x7F])
+        slot = int.from_bytes(packed[:8], "big") >> (64 - bits)
+        if (words[slot >> 6] >> (slot & 63)) & 1:
+            want.append((k, packed))
+    assert got == want and want, (len(got), len(want))
