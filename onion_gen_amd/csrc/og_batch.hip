@@ -678,6 +678,8 @@ int og_batch_step(og_batch *b, og_batch_hit *out, uint32_t cap, uint32_t *n, cha
       SlotDesc &d = b->h_slots[unit];
       b->assigned[unit] = id;
       d.f = sr.filt.args();
+      d.f.hash = nullptr;  // the slot kernels keep the automaton path
+      d.f.hash_lmax = 0;
       d.k_base = sr.k_next;
       d.state = sr.d_state;
       d.stride = stride;

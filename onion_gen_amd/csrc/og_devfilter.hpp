@@ -106,6 +106,9 @@ struct DevFilterHost {
   uint32_t want[8] = {0}, mask[8] = {0};
   uint32_t bits = 0;
   std::vector<uint32_t> bitmap;   // 32-bit halves of the 64-bit words
+  std::vector<uint32_t> hash;     // the window prefilter's two maps, or empty
+  uint32_t hash_lmax = 0;         // its longest word
+  uint32_t hash_stride = 1;       // 2: both windows of every word are mapped
   std::vector<uint32_t> coarse;   // the bitmap or-folded to coarse_bits bits
   uint32_t coarse_bits = 0;
   std::vector<uint32_t> buckets;  // [offset, count, mask_lo, mask_hi]
@@ -131,14 +134,14 @@ struct DevFilterHost {
   FilterSet proxy_set;
   // device copies
   uint32_t *d_bitmap = nullptr, *d_buckets = nullptr, *d_pats = nullptr, *d_banks = nullptr;
-  uint32_t *d_coarse = nullptr;
+  uint32_t *d_coarse = nullptr, *d_hash = nullptr;
   uint32_t *d_regex = nullptr, *d_live = nullptr;
   uint16_t *d_dfa = nullptr;
   uint32_t *d_dfa_end = nullptr;
   uint64_t *d_keys = nullptr;
 
   size_t device_bytes() const {
-    return bitmap.size() * 4 + coarse.size() * 4 + buckets.size() * 4 + keys.size() * 8 + pats.size() * 4 +
+    return bitmap.size() * 4 + coarse.size() * 4 + hash.size() * 4 + buckets.size() * 4 + keys.size() * 8 + pats.size() * 4 +
            banks.size() * sizeof(ScanBank) + regex.size() * 4 + dfa.size() * 2 + dfa_end.size() * 4 +
            live.size() * 4;
   }
@@ -284,6 +287,7 @@ struct DevFilterHost {
         const std::string eb = build_banks(fs, banks, &proxyless, false);
         if (!eb.empty()) return eb;
         for (const auto &d : dictionaries) add_automaton(*d);
+        build_window_prefilter(fs);
       } else {
         for (uint32_t i : fs.general) {
           DevPattern p = dev_pattern(fs.filters[i]);
@@ -299,6 +303,46 @@ struct DevFilterHost {
     }
     if (!mode) return "the filter set is empty";
     return "";
+  }
+
+  // The window prefilter, for a dictionary that is only plain substrings -
+  // unanchored, one symbol per position, kHashWindow to 49 symbols - of at
+  // most 2048 words (beyond that its maps fill and every wave would walk),
+  // walked in global memory (an automaton in LDS walks faster than this saves),
+  // with no expression and no bank beside it.
+  void build_window_prefilter(const FilterSet &fs) {
+    hash.clear();
+    hash_lmax = 0;
+    if (fs.general.size() > 2048 || !fs.regexes.empty() || !banks.empty() || regex.empty() ||
+        first_states <= kLdsStates)
+      return;
+    uint32_t lmax = 0, lmin = 49;
+    for (uint32_t i : fs.general) {
+      const Filter &f = fs.filters[i];
+      if (f.anchor != kAnywhere || f.sets.size() < (size_t)kHashWindow || f.sets.size() > 49)
+        return;
+      for (uint32_t set : f.sets)
+        if (__builtin_popcount(set) != 1) return;
+      lmax = std::max<uint32_t>(lmax, (uint32_t)f.sets.size());
+      lmin = std::min<uint32_t>(lmin, (uint32_t)f.sets.size());
+    }
+    // every second placement suffices when each word's windows at its symbols
+    // 0 and 1 are both mapped: a word at an odd placement shows at the next
+    const uint32_t stride = lmin >= (uint32_t)kHashWindow + 1 ? 2u : 1u;
+    std::vector<uint32_t> maps(2 * kHashSliceWords, 0u);
+    for (uint32_t i : fs.general)
+      for (uint32_t off = 0; off < stride; ++off) {
+        uint64_t v = 0;  // kHashWindow symbols from `off`, 35 bits, big-endian
+        for (int j = 0; j < kHashWindow; ++j)
+          v = (v << 5) | (uint32_t)__builtin_ctz(fs.filters[i].sets[off + j]);
+        const uint32_t a = (uint32_t)(v >> (5 * kHashWindow - kHashSliceBits));
+        const uint32_t b = (uint32_t)(v & ((1u << kHashSliceBits) - 1));
+        maps[a >> 5] |= 1u << (a & 31);
+        maps[kHashSliceWords + (b >> 5)] |= 1u << (b & 31);
+      }
+    hash.swap(maps);
+    hash_lmax = lmax;
+    hash_stride = stride;
   }
 
   // `bits` must be set first.  Slot s of the map is bit s & 31 of word s >> 5
@@ -333,6 +377,7 @@ struct DevFilterHost {
   hipError_t to_device(hipStream_t s) {
     hipError_t e = upload(&d_bitmap, bitmap, s);
     if (e == hipSuccess) e = upload(&d_coarse, coarse, s);
+    if (e == hipSuccess) e = upload(&d_hash, hash, s);
     if (e == hipSuccess) e = upload(&d_buckets, buckets, s);
     if (e == hipSuccess) e = upload(&d_keys, keys, s);
     if (e == hipSuccess) e = upload(&d_pats, pats, s);
@@ -353,7 +398,8 @@ struct DevFilterHost {
   void free_device() {
     (void)hipFree(d_bitmap);
     (void)hipFree(d_coarse);
-    d_coarse = nullptr;
+    (void)hipFree(d_hash);
+    d_coarse = d_hash = nullptr;
     (void)hipFree(d_buckets);
     (void)hipFree(d_keys);
     (void)hipFree(d_pats);
@@ -376,6 +422,9 @@ struct DevFilterHost {
     a.bits = bits;
     a.bitmap = d_bitmap;
     a.coarse = d_coarse;
+    a.hash = d_hash;
+    a.hash_lmax = d_hash ? hash_lmax : 0u;
+    a.hash_stride = hash_stride;
     a.coarse_bits = d_coarse ? coarse_bits : 0u;
     a.nbuckets = (uint32_t)(buckets.size() / 4);
     a.buckets = d_buckets;

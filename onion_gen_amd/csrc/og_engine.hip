@@ -42,8 +42,10 @@ __global__ __launch_bounds__(256) OG_SEARCH_ATTR(MODE) void og_search_kernel(Sea
     st.t2d.v[l] = a.stride[20 + l];
   }
   __shared__ ScanBank banks[(MODE & kModePattern) ? kMaxBanks : 1];
-  __shared__ uint16_t dfa_lds[(MODE & kModePattern) ? kLdsStates * 32 : 2];
-  if (MODE & kModePattern) load_filter(banks, dfa_lds, a.f);
+  // the first automaton, or a dictionary's window prefilter maps
+  __shared__ uint32_t pattern_lds[(MODE & kModePattern) ? kPatternLdsWords : 1];
+  uint16_t *dfa_lds = reinterpret_cast<uint16_t *>(pattern_lds);
+  if (MODE & kModePattern) load_filter(banks, dfa_lds, a.f, true);
   // the key forms' bitmap probe reads this workgroup's LDS copy of the map,
   // or of its coarse fold, before (or instead of) the map in L2
   constexpr bool kCoarse = (MODE & kModeBitmap) && !(MODE & kModePattern);

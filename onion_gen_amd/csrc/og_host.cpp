@@ -293,8 +293,12 @@ int og_filters_scan_pair_host(const og_filterset *s, const uint8_t a[32], const 
     for (int i = 0; i < 2; ++i) maybe[i] = (maybe[i] || h.regex_every) && !kh[i];
     if (maybe[0] || maybe[1]) {
       const uint32_t K = h.first_states <= kLdsStates ? h.first_states : 0u;
-      const uint32_t r = regex_key2(kw[0], kw[1], h.regex.data(), h.dfa.data(), h.live.data(),
-                                    nre, h.dfa.data() + h.regex[0], K);
+      const uint32_t r =
+          h.hash_lmax ? regex_key2_prefiltered(kw[0], kw[1], h.regex.data(), h.dfa.data(),
+                                               h.live.data(), nre, h.hash.data(), h.hash_lmax,
+                                               h.hash_stride)
+                      : regex_key2(kw[0], kw[1], h.regex.data(), h.dfa.data(), h.live.data(),
+                                   nre, h.dfa.data() + h.regex[0], K);
       const uint32_t ri[2] = {maybe[0] ? (r & 3u) : 0u, maybe[1] ? (r >> 2) : 0u};
       for (int i = 0; i < 2; ++i) {
         kh[i] = kh[i] || ri[i] == 2;
@@ -327,6 +331,10 @@ size_t og_filters_describe(const og_filterset *s, char *out, size_t cap) {
          std::to_string(states) + " states over " + std::to_string(h.dictionary_forms) +
          " general form(s), run on every candidate; the checksum only where a match can still "
          "end in it\n";
+    if (h.hash_lmax)
+      t += "window prefilter: two 2^18-bit maps over every word's first 7 symbols, in LDS; "
+           "the automaton walks symbols " + std::to_string(50 - h.hash_lmax) +
+           "..48 for every candidate and 0..48 only where a window hits\n";
   }
   const bool dict = !h.dictionaries.empty();
   if ((!fs.general.empty() && !dict) || (!fs.regexes.empty() && !h.key_proxy)) {

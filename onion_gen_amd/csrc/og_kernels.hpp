@@ -63,6 +63,9 @@ constexpr int kMaxRegexes = 16;
 // this many states (32 KiB).  A larger one is walked in global memory: a wave
 // waits whenever any lane leaves the LDS part, so a partial copy does not pay.
 constexpr uint32_t kLdsStates = 512;
+// The pattern kernels' LDS for a filter: the first automaton (kLdsStates rows
+// of 32 16-bit entries) or the window prefilter's two 2^18-bit maps.
+constexpr uint32_t kPatternLdsWords = 2u << (18 - 5);
 // The bitmap probe's LDS copy: at most 2^18 bits (32 KiB), so the key forms'
 // three workgroups per CU hold three copies.
 constexpr uint32_t kCoarseBits = 18;
@@ -95,6 +98,11 @@ struct DevFilterArgs {
   uint32_t defer_sign;            // (kModePattern, one-round launches) the key
                                   //   stage parks candidates without the sign
                                   //   of x, and the address kernel derives it
+  const uint32_t *hash;           // (kModePattern) a dictionary of plain
+  uint32_t hash_lmax;             //   substrings: the window prefilter's two
+                                  //   maps, and its longest word (0: none)
+  uint32_t hash_stride;           //   placements probed: every one, or every
+                                  //   second (both windows of a word mapped)
   const uint32_t *coarse;         // (kModeBitmap) the bitmap or-folded to
   uint32_t coarse_bits;           //   coarse_bits <= kCoarseBits bits: each
                                   //   workgroup's LDS copy is probed first, and

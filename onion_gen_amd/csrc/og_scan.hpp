@@ -296,6 +296,69 @@ OG_HD void regex_walk49x2(TP t, uint32_t &sa, uint32_t &sb, const uint32_t wa[8]
   walk2_from<0, 49>(t, sa, sb, fa, fb);
 }
 
+// The window prefilter of a dictionary of plain substrings (every word at
+// least kHashWindow symbols, unanchored, one symbol per position): two
+// bitmaps of 2^18 bits, one over bits [0, 18) and one over bits [17, 35) of
+// each word's first kHashWindow symbols.  A key whose every window of that
+// length at symbols 0..49-kHashWindow misses one of the two maps holds no
+// word inside symbols 0..48; a window that hits both is checked exactly.
+constexpr int kHashWindow = 7;
+constexpr uint32_t kHashSliceBits = 18;
+constexpr uint32_t kHashSliceWords = 1u << (kHashSliceBits - 5);  // per map
+
+// The 32 key bits from bit `pos` on (big-endian words; past the key, zeros).
+template <int POS>
+OG_HD uint32_t key_bits32(const uint32_t *w) {
+  constexpr int i = POS >> 5, sh = POS & 31;
+  if constexpr (sh == 0) {
+    return w[i];
+  } else {
+    const uint32_t next = i + 1 < 8 ? w[i + 1 < 8 ? i + 1 : 7] : 0u;
+    return (w[i] << sh) | (next >> (32 - sh));
+  }
+}
+
+// Windows at placements 0, STRIDE, 2 STRIDE, ... up to 49 - kHashWindow.  With
+// stride 2 the maps hold both windows of each word (at its symbols 0 and 1,
+// every word having at least kHashWindow + 1 symbols), so a word at any
+// placement shows in a probed window.
+template <int J, int STRIDE, class TL>
+OG_HD uint32_t window_hits_from(const uint32_t *w, TL maps) {
+  if constexpr (J > 49 - kHashWindow) {
+    return 0u;
+  } else {
+    const uint32_t a = key_bits32<5 * J>(w) >> (32 - kHashSliceBits);
+    const uint32_t b = key_bits32<5 * J + 17>(w) >> (32 - kHashSliceBits);
+    const uint32_t hit = (maps[a >> 5] >> (a & 31)) & (maps[kHashSliceWords + (b >> 5)] >> (b & 31));
+    if constexpr (J + STRIDE > 49 - kHashWindow && J != 49 - kHashWindow) {
+      // the last window placement, 49 - kHashWindow, is always probed
+      const uint32_t a2 = key_bits32<5 * (49 - kHashWindow)>(w) >> (32 - kHashSliceBits);
+      const uint32_t b2 = key_bits32<5 * (49 - kHashWindow) + 17>(w) >> (32 - kHashSliceBits);
+      const uint32_t hit2 =
+          (maps[a2 >> 5] >> (a2 & 31)) & (maps[kHashSliceWords + (b2 >> 5)] >> (b2 & 31));
+      return (hit | hit2) & 1u;
+    } else {
+      return (hit & 1u) | window_hits_from<J + STRIDE, STRIDE>(w, maps);
+    }
+  }
+}
+
+// The state after symbols j0..48 from state s: the walk with a run-time first
+// symbol (the short walk of the prefiltered dictionaries).
+template <class TP>
+OG_HD uint32_t walk_tail(TP t, uint32_t s, const uint32_t w[8], uint32_t j0) {
+  switch (j0) {  // the common word lengths 8..12 take the unrolled steps
+    case 42: return walk_from<42, 49>(t, s, w);
+    case 41: return walk_from<41, 49>(t, s, w);
+    case 40: return walk_from<40, 49>(t, s, w);
+    case 39: return walk_from<39, 49>(t, s, w);
+    case 38: return walk_from<38, 49>(t, s, w);
+    default:
+      for (uint32_t j = j0; j < 49; ++j) s = t[(s << 5) | sym_at(w, (int)j)];
+      return s;
+  }
+}
+
 // Symbols 0..48 (what the key settles) through every expression: 2 when one
 // has matched, 1 when one may still match on the address, 0 otherwise.
 // `in_lds`: the first expression's whole table is also at `lds`.
@@ -326,6 +389,44 @@ OG_HD uint32_t regex_key2(const uint32_t wa[8], const uint32_t wb[8], const uint
     uint32_t sa = recs[3 * r + 1], sb = sa;
     if (r == 0 && in_lds) regex_walk49x2(lds, sa, sb, wa, wb);
     else regex_walk49x2(wave_uniform(t), sa, sb, wa, wb);
+    if (sa == 1) ra = 2;
+    else if (sa != 0 && !ra && regex_open(t, lv, sa, wa)) ra = 1;
+    if (sb == 1) rb = 2;
+    else if (sb != 0 && !rb && regex_open(t, lv, sb, wb)) rb = 1;
+  }
+  return ra | (rb << 2);
+}
+
+// regex_key2 for a dictionary of plain substrings behind the window
+// prefilter (`maps`, kHashWindow, the longest word `lmax`).  A word that
+// ends at symbol 49 or later starts at 50 - lmax or later, and the automaton
+// of unanchored words forgets everything older than its longest word, so a
+// walk from the start state over symbols 50-lmax..48 reaches the state the
+// whole walk reaches unless a word ended earlier - and such a word, wholly
+// inside symbols 0..48, shows in the prefilter, whose hits run the whole walk.
+// Same result as regex_key2 for such a set; the long walks run only where a
+// window hits.
+template <class T16, class T32, class TL>
+OG_HD uint32_t regex_key2_prefiltered(const uint32_t wa[8], const uint32_t wb[8],
+                                      const uint32_t *recs, T16 dfa, T32 live, uint32_t n,
+                                      TL maps, uint32_t lmax, uint32_t stride) {
+  const uint32_t j0 = lmax < 49 ? 50 - lmax : 1;
+  bool pa, pb;
+  if (stride == 2) {
+    pa = window_hits_from<0, 2>(wa, maps) != 0;
+    pb = window_hits_from<0, 2>(wb, maps) != 0;
+  } else {
+    pa = window_hits_from<0, 1>(wa, maps) != 0;
+    pb = window_hits_from<0, 1>(wb, maps) != 0;
+  }
+  uint32_t ra = 0, rb = 0;
+  for (uint32_t r = 0; r < n && (ra != 2 || rb != 2); ++r) {
+    const T16 t = wave_uniform(dfa + recs[3 * r]);
+    const T32 lv = live + recs[3 * r + 2];
+    const uint32_t start = recs[3 * r + 1];
+    uint32_t sa = walk_tail(t, start, wa, j0), sb = walk_tail(t, start, wb, j0);
+    if (pa && sa != 1) sa = regex_walk49(t, start, wa) == 1 ? 1u : sa;
+    if (pb && sb != 1) sb = regex_walk49(t, start, wb) == 1 ? 1u : sb;
     if (sa == 1) ra = 2;
     else if (sa != 0 && !ra && regex_open(t, lv, sa, wa)) ra = 1;
     if (sb == 1) rb = 2;

@@ -177,6 +177,7 @@ __device__ __noinline__ bool pattern_slow(uint32_t w0, uint32_t w1, uint32_t w2,
 typedef const __attribute__((address_space(1))) uint16_t *gptr16;
 typedef const __attribute__((address_space(1))) uint32_t *gptr32;
 typedef const __attribute__((address_space(3))) uint16_t *lptr16;
+typedef const __attribute__((address_space(3))) uint32_t *lptr32;
 __device__ __noinline__ uint32_t regex_key_slow(uint32_t w0, uint32_t w1, uint32_t w2,
                                                 uint32_t w3, uint32_t w4, uint32_t w5,
                                                 uint32_t w6, uint32_t w7, const uint32_t *recs,
@@ -197,6 +198,22 @@ __device__ __noinline__ uint32_t regex_key2_slow(uint32_t a0, uint32_t a1, uint3
   const uint32_t wa[8] = {a0, a1, a2, a3, a4, a5, a6, a7};
   const uint32_t wb[8] = {b0, b1, b2, b3, b4, b5, b6, b7};
   return regex_key2(wa, wb, recs, (gptr16)dfa, (gptr32)live, n, lds, K);
+}
+
+// The dictionary of plain substrings behind the window prefilter, whose maps
+// are in this workgroup's LDS.
+__device__ __noinline__ uint32_t regex_key2_pref_slow(uint32_t a0, uint32_t a1, uint32_t a2,
+                                                      uint32_t a3, uint32_t a4, uint32_t a5,
+                                                      uint32_t a6, uint32_t a7, uint32_t b0,
+                                                      uint32_t b1, uint32_t b2, uint32_t b3,
+                                                      uint32_t b4, uint32_t b5, uint32_t b6,
+                                                      uint32_t b7, const uint32_t *recs,
+                                                      const uint16_t *dfa, const uint32_t *live,
+                                                      uint32_t n, lptr32 maps, uint32_t lmax,
+                                                      uint32_t stride) {
+  const uint32_t wa[8] = {a0, a1, a2, a3, a4, a5, a6, a7};
+  const uint32_t wb[8] = {b0, b1, b2, b3, b4, b5, b6, b7};
+  return regex_key2_prefiltered(wa, wb, recs, (gptr16)dfa, (gptr32)live, n, maps, lmax, stride);
 }
 
 __device__ __noinline__ bool regex_address_slow(uint32_t w0, uint32_t w1, uint32_t w2,
@@ -325,10 +342,15 @@ __device__ __forceinline__ void key_stage_pair(const Fe &ca, const uint32_t wa[8
     ma = (ma || f.regex_every != 0) && !ha;
     mb = (mb || f.regex_every != 0) && !hb;
     if (__ballot(ma || mb) && (ma || mb)) {
-      const uint32_t r = regex_key2_slow(wa[0], wa[1], wa[2], wa[3], wa[4], wa[5], wa[6], wa[7],
-                                         wb[0], wb[1], wb[2], wb[3], wb[4], wb[5], wb[6], wb[7],
-                                         f.regex, f.dfa, f.regex_live, f.nregex,
-                                         (lptr16)dfa_lds, f.lds_states);
+      const uint32_t r =
+          f.hash_lmax
+              ? regex_key2_pref_slow(wa[0], wa[1], wa[2], wa[3], wa[4], wa[5], wa[6], wa[7],
+                                     wb[0], wb[1], wb[2], wb[3], wb[4], wb[5], wb[6], wb[7],
+                                     f.regex, f.dfa, f.regex_live, f.nregex,
+                                     (lptr32)dfa_lds, f.hash_lmax, f.hash_stride)
+              : regex_key2_slow(wa[0], wa[1], wa[2], wa[3], wa[4], wa[5], wa[6], wa[7], wb[0],
+                                wb[1], wb[2], wb[3], wb[4], wb[5], wb[6], wb[7], f.regex, f.dfa,
+                                f.regex_live, f.nregex, (lptr16)dfa_lds, f.lds_states);
       const uint32_t ra = ma ? (r & 3u) : 0u, rb = mb ? (r >> 2) : 0u;
       ha |= ra == 2;
       na |= ra == 1;
@@ -632,11 +654,16 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
 // that fits, into this workgroup's LDS copies.  The tables are whole rows of
 // 32 16-bit entries from a 256-byte aligned allocation, so they copy as
 // 32-bit words.
+// With `with_maps` (the search kernel's LDS holds kPatternLdsWords words) the
+// window prefilter's maps are copied instead when the set has them.
 __device__ __forceinline__ void load_filter(ScanBank *lds, uint16_t *dfa_lds,
-                                            const DevFilterArgs &f) {
+                                            const DevFilterArgs &f, bool with_maps = false) {
   uint32_t *dst = reinterpret_cast<uint32_t *>(lds);
   for (uint32_t i = threadIdx.x; i < f.nbanks * kBankWords; i += blockDim.x) dst[i] = f.banks[i];
-  if (f.lds_states) {
+  if (with_maps && f.hash_lmax) {
+    uint32_t *d = reinterpret_cast<uint32_t *>(dfa_lds);
+    for (uint32_t i = threadIdx.x; i < 2 * kHashSliceWords; i += blockDim.x) d[i] = f.hash[i];
+  } else if (f.lds_states) {
     const uint32_t *src = reinterpret_cast<const uint32_t *>(f.dfa + f.regex[0]);
     uint32_t *d = reinterpret_cast<uint32_t *>(dfa_lds);
     for (uint32_t i = threadIdx.x; i < f.lds_states * 16; i += blockDim.x) d[i] = src[i];

@@ -177,6 +177,34 @@ def test_dictionary_split_across_automata_matches_every_address():
     assert got == want, (len(got), len(want))
 
 
+def test_dictionary_behind_the_window_prefilter_matches_every_address():
+    """1,200 eight-symbol substrings (one automaton in global memory) take the
+    window prefilter: every key walks symbols 42..48, and the whole walk runs
+    only where a 7-symbol window hits both maps.  A twentieth of the words are
+    taken from the range's addresses at every placement, those that run into
+    the checksum included.  The engine reports exactly the candidates whose
+    address holds a word, as a set of the words over every 8-symbol window of
+    the oracle's addresses decides."""
+    import random
+    say("case: 1,200 substrings behind the window prefilter, n=2^16")
+    n = 1 << 16
+    start = key.start_point(hashlib.sha256(b"prefiltered dictionary").digest())
+    every = search(start, 0, n, 256, OracleFilters(EVERY), cap=n)
+    rng = random.Random(12)
+    A = workload.ALPHABET
+    words = {"".join(rng.choice(A) for _ in range(8)) for _ in range(1140)}
+    for i, (_, _, addr) in enumerate(rng.sample(every, 60)):
+        s = i % 49
+        words.add(addr[s:s + 8])
+    fs = filters.FilterSet.parse(["contains:" + w for w in sorted(words)])
+    assert any(l.startswith("window prefilter") for l in fs.describe())
+    got = run_engine(start, fs, n, threads=1024, half=64)
+    want = sorted((k, clear_sign(pk)) for k, pk, addr in every
+                  if any(addr[s:s + 8] in words for s in range(49)))
+    assert len(want) >= 60
+    assert got == want, (len(got), len(want))
+
+
 def test_dictionary_that_reaches_the_checksum_matches_every_address():
     """1,000 five-symbol substrings: a word can still end in the checksum for
     most candidates, so most park, and each lane drains a share of many

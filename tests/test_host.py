@@ -449,3 +449,54 @@ def test_dictionary_beyond_one_automaton_splits():
     keys = planted + [bytes(rng.getrandbits(8) for _ in range(32)) for _ in range(100)]
     hits, needed, key_hits = _check_dictionary(lines, keys)
     assert hits >= 20 and key_hits >= 20
+
+
+def test_window_prefilter_gives_the_whole_walks_verdict():
+    """A dictionary of plain substrings (1,000 eight-symbol words, one
+    automaton in global memory) runs the key stage behind the window
+    prefilter: a short walk over symbols 42..48 for every key, the whole walk
+    only where a window hits both maps.  Its verdicts (the pair scan) must equal
+    the whole walk's (the single scan) on random keys and on keys with a word
+    planted at every placement, across the key/checksum boundary included."""
+    import random
+    from onion_gen_amd import filters
+    rng = random.Random(31)
+    alphabet = "abcdefghijklmnopqrstuvwxyz234567"
+    words = ["".join(rng.choice(alphabet) for _ in range(8)) for _ in range(1000)]
+    fs = filters.FilterSet.parse(["contains:" + w for w in words])
+    assert any("window prefilter" in line for line in fs.describe())
+    # a set that cannot take it (a class in a word) keeps the plain walk
+    fs2 = filters.FilterSet.parse(["contains:" + w for w in words[:999]] + ["contains:ab[cd]efghi"])
+    assert not any("window prefilter" in line for line in fs2.describe())
+
+    def key_with(word, at):
+        bits = [rng.getrandbits(1) for _ in range(255)]
+        for i, c in enumerate(word):
+            s = at + i
+            if s >= 51:  # symbols 49 and 50 are key bits too (49 holds the sign)
+                break
+            v = alphabet.index(c)
+            for b in range(5):
+                if 5 * s + b < 255:
+                    bits[5 * s + b] = (v >> (4 - b)) & 1
+        # the address reads the packed key's bytes in order, each from its top
+        # bit; bit 248 of that string (the top bit of byte 31) is the sign
+        pk = bytearray((int("".join(map(str, bits)), 2) << 1).to_bytes(32, "big"))
+        pk[31] &= 0x7F
+        return bytes(pk)
+
+    keys = [bytes(rng.getrandbits(8) for _ in range(31)) + bytes([rng.getrandbits(7)])
+            for _ in range(600)]
+    for at in range(0, 51):
+        for _ in range(8):
+            keys.append(key_with(rng.choice(words), at))
+    checked = hits = needs = 0
+    for a, b in zip(keys[0::2], keys[1::2]):
+        pair = fs.scan_pair(a, b)
+        for i, k in enumerate((a, b)):
+            kh1, nd1, _ = fs.scan(k, 0)
+            assert pair[i] == (kh1, nd1), (k.hex(), pair, kh1, nd1)
+            checked += 1
+            hits += kh1
+            needs += nd1
+    assert checked == len(keys) - len(keys) % 2 and hits > 150 and needs > 10
