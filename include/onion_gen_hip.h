@@ -380,13 +380,15 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
  * each of slots {32, 64, 128} x chains per slot {4096 .. 32768} x half {256, 512}
  * that fits `max_bytes` (0 = half the free memory), as many searches as slots -
  * the given filter sets in turn, each on its own chain - run one warm-up step and
- * then steps for at least 0.25 s; among the shapes within 3% of the fastest
+ * then steps for at least 0.25 s.  A step bounds how soon an admitted search runs
+ * and a found hit is reported, so only shapes whose steps average at most
+ * `max_step_ms` (0 = 25 ms) take part; among those within 3% of the fastest
  * aggregate rate the smallest footprint is kept (og_tuning_choose), recorded
  * under og_tuning_device_key(<name>, "hip-batch", 0), and written to `best` with
  * its rate.  No counterpart in the reference, which has no slot engine. */
 int og_batch_tune(int ordinal, const og_filterset *const *filters, size_t nfilters,
-                  uint64_t max_bytes, og_batch_opts *best, double *rate, char *err,
-                  size_t errcap);
+                  uint64_t max_bytes, double max_step_ms, og_batch_opts *best, double *rate,
+                  char *err, size_t errcap);
 /* Device bytes a resident search takes: its chain state and filter tables
  * (the admission footprint).  b may be null for the default chain count. */
 size_t og_batch_footprint(const og_batch *b, const og_filterset *filters);

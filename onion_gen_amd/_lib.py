@@ -104,8 +104,9 @@ SIGNATURES.update({
 SIGNATURES.update({
     "og_batch_create": (_c.c_int, [_c.c_int, _c.c_void_p, _c.POINTER(_c.c_void_p), _c.c_char_p,
                                    _c.c_size_t]),
-    "og_batch_tune": (_c.c_int, [_c.c_int, _c.c_void_p, _c.c_size_t, _c.c_uint64, _c.c_void_p,
-                                 _c.POINTER(_c.c_double), _c.c_char_p, _c.c_size_t]),
+    "og_batch_tune": (_c.c_int, [_c.c_int, _c.c_void_p, _c.c_size_t, _c.c_uint64, _c.c_double,
+                                 _c.c_void_p, _c.POINTER(_c.c_double), _c.c_char_p,
+                                 _c.c_size_t]),
     "og_batch_footprint": (_c.c_size_t, [_c.c_void_p, _c.c_void_p]),
     "og_batch_admit": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_void_p, _c.c_uint64,
                                   _c.POINTER(_c.c_int), _c.c_char_p, _c.c_size_t]),

@@ -31,20 +31,23 @@ class BatchHit:
     packed: bytes
 
 
-def tune(filter_sets, max_bytes=0, device=0):
+def tune(filter_sets, max_bytes=0, max_step_ms=0.0, device=0):
     """og_batch_tune: the unit shape a timed walk picks for this mix of filter
-    sets (FilterSet objects), recorded in the tuning cache, as a dict with its
+    sets (FilterSet objects) among those whose steps take at most
+    `max_step_ms` (0: 25 ms), recorded in the tuning cache, as a dict with its
     aggregate rate."""
     n = len(filter_sets)
     arr = (ctypes.c_void_p * max(n, 1))(*[f.handle.value for f in filter_sets])
     best = _Opts()
     rate = ctypes.c_double()
     e = err_buf()
-    if lib().og_batch_tune(device, arr, n, max_bytes, ctypes.byref(best), ctypes.byref(rate),
-                           e, len(e)) != 0:
+    if lib().og_batch_tune(device, arr, n, max_bytes, max_step_ms, ctypes.byref(best),
+                           ctypes.byref(rate), e, len(e)) != 0:
         raise OgError(e.value.decode(errors="replace"))
     return {"slots": best.slots, "chains": best.chains, "half": best.half,
-            "rate": rate.value}
+            "rate": rate.value,
+            "step_ms": best.slots * best.chains * (2 * best.half + 1) / rate.value * 1e3
+            if rate.value else None}
 
 
 class Batch:

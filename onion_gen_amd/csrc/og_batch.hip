@@ -437,8 +437,11 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
 }
 
 int og_batch_tune(int ordinal, const og_filterset *const *filters, size_t nfilters,
-                  uint64_t max_bytes, og_batch_opts *best, double *best_rate, char *err,
-                  size_t errcap) {
+                  uint64_t max_bytes, double max_step_ms, og_batch_opts *best, double *best_rate,
+                  char *err, size_t errcap) {
+  // a step bounds how soon a new search runs and how soon a found hit is
+  // reported, so only shapes whose steps stay within the bound are eligible
+  const double step_cap = max_step_ms > 0 ? max_step_ms : 25.0;
   if (!filters || nfilters == 0) {
     put_err(err, errcap, "the walk needs at least one filter set");
     return -1;
@@ -518,6 +521,11 @@ int og_batch_tune(int ordinal, const og_filterset *const *filters, size_t nfilte
         sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       }
       if (ok && sec > 0) rate = (double)(og_batch_examined(b) - ex0) / sec;
+      if (ok && steps > 0 && sec * 1e3 / steps > step_cap) {
+        rate = 0;
+        last = "a step takes " + std::to_string(sec * 1e3 / steps) + " ms, over the bound of " +
+               std::to_string(step_cap) + " ms";
+      }
     }
     if (rate == 0 && e2[0]) last = e2;
     og_batch_destroy(b);

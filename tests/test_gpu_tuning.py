@@ -66,10 +66,10 @@ def test_tuned_engine_fits_the_memory_budget(tmp_path, monkeypatch):
 
 def test_batch_walk_records_and_the_zero_shape_uses_it(tmp_path, monkeypatch):
     """og_batch_tune times the slot engine's unit shapes with the 64-search
-    mix's filters under a 48 GiB budget, records the pick under the device's
-    hip-batch key, and og_batch_create with slots, chains and half all 0 runs
-    that shape (src/tuning.rs:27-50 for the cache, src/gpu/mod.rs:346-428 for
-    the walk)."""
+    mix's filters under a 48 GiB budget and the default 25 ms step bound,
+    records the pick under the device's hip-batch key, and og_batch_create
+    with slots, chains and half all 0 runs that shape (src/tuning.rs:27-50 for
+    the cache, src/gpu/mod.rs:346-428 for the walk)."""
     from onion_gen_amd import batch
     monkeypatch.setenv("OG_TUNING_PATH", str(tmp_path / "tuning.yaml"))
     mix = workload.search_mix(64, 4)
@@ -79,6 +79,9 @@ def test_batch_walk_records_and_the_zero_shape_uses_it(tmp_path, monkeypatch):
     say(f"best: {best}")
     assert best["slots"] in (32, 64, 128) and best["chains"] in (4096, 8192, 16384, 32768)
     assert best["half"] in (256, 512) and best["rate"] > 1e9
+    # only shapes whose steps average at most 25 ms take part (every slot
+    # busy, so a step examines the whole unit)
+    assert best["step_ms"] <= 25.0
     dev = gpu.look()[0]
     cached = tuning.get(tuning.device_key(dev.name, "hip-batch", 0))
     assert cached and cached["slots"] == best["slots"] and cached["second"] == best["chains"]
