@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -123,6 +124,15 @@ struct DevFilterHost {
   std::vector<uint32_t> live;     // their tail-liveness bits
   bool regex_every = false;
   bool proxyless = false;         // an expression found no room for its proxies
+  // The fraction of sample keys whose key stage leaves a checksum placement
+  // possible (need_rate over kNeedSamples), and whether it is high enough
+  // that the key stage is skipped: every candidate the key forms do not
+  // settle goes to the address stage, whose exact pass over the whole address
+  // decides the general forms and expressions alone.
+  double need = 0.0;
+  bool park_all = false;
+  double window_park = 0.0;       // the window prefilter's expected share of
+                                  //   keys sent to the address stage
   // General filters beyond the banks, compiled into automata (a dictionary
   // of substrings and suffixes): one, or several when one would exceed the
   // state limit; run on every candidate.
@@ -302,19 +312,22 @@ struct DevFilterHost {
       regex_every = proxyless || !dictionaries.empty();
     }
     if (!mode) return "the filter set is empty";
+    need = need_rate(kNeedSamples);
+    park_all = (mode & kModePattern) && need >= kParkAllRate;
     return "";
   }
 
   // The window prefilter, for a dictionary that is only plain substrings -
-  // unanchored, one symbol per position, kHashWindow to 49 symbols - of at
-  // most 2048 words (beyond that its maps fill and every wave would walk),
-  // walked in global memory (an automaton in LDS walks faster than this saves),
-  // with no expression and no bank beside it.
+  // unanchored, one symbol per position, kHashWindow to 49 symbols - walked in
+  // global memory (an automaton in LDS walks faster than this saves), with no
+  // expression and no bank beside it.  A key with a window that hits both maps
+  // goes to the address stage, so the maps are kept only while the expected
+  // share of such keys, 1 - (1 - a b)^probes for map occupancies a and b, is
+  // at most kWindowParkMax.
   void build_window_prefilter(const FilterSet &fs) {
     hash.clear();
     hash_lmax = 0;
-    if (fs.general.size() > 2048 || !fs.regexes.empty() || !banks.empty() || regex.empty() ||
-        first_states <= kLdsStates)
+    if (!fs.regexes.empty() || !banks.empty() || regex.empty() || first_states <= kLdsStates)
       return;
     uint32_t lmax = 0, lmin = 49;
     for (uint32_t i : fs.general) {
@@ -340,6 +353,16 @@ struct DevFilterHost {
         maps[a >> 5] |= 1u << (a & 31);
         maps[kHashSliceWords + (b >> 5)] |= 1u << (b & 31);
       }
+    double fill[2] = {0, 0};
+    for (uint32_t m = 0; m < 2; ++m) {
+      for (uint32_t i = 0; i < kHashSliceWords; ++i)
+        fill[m] += __builtin_popcount(maps[m * kHashSliceWords + i]);
+      fill[m] /= (double)(1u << kHashSliceBits);
+    }
+    const uint32_t placements = 49 - kHashWindow + 1;
+    const uint32_t probes = (placements + stride - 1) / stride;
+    window_park = 1.0 - std::pow(1.0 - fill[0] * fill[1], (double)probes);
+    if (window_park > kWindowParkMax) return;
     hash.swap(maps);
     hash_lmax = lmax;
     hash_stride = stride;
@@ -442,6 +465,7 @@ struct DevFilterHost {
     a.dfa = d_dfa;
     a.dfa_end = d_dfa_end;
     a.defer_sign = 0;
+    a.park_all = park_all ? 1u : 0u;
     return a;
   }
 

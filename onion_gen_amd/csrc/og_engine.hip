@@ -530,8 +530,10 @@ int create_engine(int ordinal, const uint8_t start_xyzt[128], const og_engine_op
   if ((e->filt.mode & kModePattern) && o.rounds == 1) {
     OG_TRY(hipMalloc(&e->d_parked, (size_t)o.threads * 4));
     a.parked = e->d_parked;
+    // the share that parks: the key stage's, plus the window prefilter's hits
+    const double parks = e->filt.need + (e->filt.hash_lmax ? e->filt.window_park : 0.0);
     if (o.half >= 8 && o.sign_stage != 1)
-      a.f.defer_sign = o.sign_stage == 2 || e->filt.need_rate(4096) < 0.125 ? 1u : 0u;
+      a.f.defer_sign = o.sign_stage == 2 || parks < 0.125 ? 1u : 0u;
   } else {
     a.parked = nullptr;
   }

@@ -331,10 +331,21 @@ size_t og_filters_describe(const og_filterset *s, char *out, size_t cap) {
          std::to_string(states) + " states over " + std::to_string(h.dictionary_forms) +
          " general form(s), run on every candidate; the checksum only where a match can still "
          "end in it\n";
-    if (h.hash_lmax)
+    if (h.hash_lmax) {
+      char pct[16];
+      std::snprintf(pct, sizeof pct, "%.2f", h.window_park * 100.0);
       t += "window prefilter: two 2^18-bit maps over every word's first 7 symbols, in LDS; "
            "the automaton walks symbols " + std::to_string(50 - h.hash_lmax) +
-           "..48 for every candidate and 0..48 only where a window hits\n";
+           "..48 for every candidate, and a key whose window hits (about " + pct +
+           "%) goes to the address stage\n";
+    }
+  }
+  if (h.park_all) {
+    char pct[16];
+    std::snprintf(pct, sizeof pct, "%.1f", h.need * 100.0);
+    t += std::string("key stage skipped: ") + pct +
+         "% of sample keys can still match in the checksum, so every candidate goes to the "
+         "address stage\n";
   }
   const bool dict = !h.dictionaries.empty();
   if ((!fs.general.empty() && !dict) || (!fs.regexes.empty() && !h.key_proxy)) {

@@ -70,6 +70,18 @@ constexpr uint32_t kPatternLdsWords = 2u << (18 - 5);
 // three workgroups per CU hold three copies.
 constexpr uint32_t kCoarseBits = 18;
 constexpr uint32_t kCoarseWords = 1u << (kCoarseBits - 5);
+// The key stage is skipped when at least this fraction of kNeedSamples
+// pseudo-random keys would go to the address stage anyway.  On MI355X a key
+// stage's automaton walk costs about 64 ps a candidate and the address stage
+// about 107 ps (1,000 five-symbol words, 2^31-candidate launches), so skipping
+// pays from about 40%; the bound leaves room for the sign of x, which every
+// parked lane then forms.
+constexpr uint32_t kNeedSamples = 4096;
+constexpr double kParkAllRate = 0.9;
+// The window prefilter sends a key whose window hits both maps to the address
+// stage; it is used while the expected share of such keys stays at most this
+// (about 27 ps a candidate against 64 ps for one automaton's whole walk).
+constexpr double kWindowParkMax = 0.25;
 
 struct DevFilterArgs {
   uint32_t mode;
@@ -98,6 +110,10 @@ struct DevFilterArgs {
   uint32_t defer_sign;            // (kModePattern, one-round launches) the key
                                   //   stage parks candidates without the sign
                                   //   of x, and the address kernel derives it
+  uint32_t park_all;              // (kModePattern) the key stage would send
+                                  //   nearly every candidate to the address
+                                  //   stage: it is skipped, and every candidate
+                                  //   the key forms leave goes there
   const uint32_t *hash;           // (kModePattern) a dictionary of plain
   uint32_t hash_lmax;             //   substrings: the window prefilter's two
                                   //   maps, and its longest word (0: none)

@@ -397,15 +397,20 @@ OG_HD uint32_t regex_key2(const uint32_t wa[8], const uint32_t wb[8], const uint
   return ra | (rb << 2);
 }
 
-// regex_key2 for a dictionary of plain substrings behind the window
-// prefilter (`maps`, kHashWindow, the longest word `lmax`).  A word that
-// ends at symbol 49 or later starts at 50 - lmax or later, and the automaton
-// of unanchored words forgets everything older than its longest word, so a
-// walk from the start state over symbols 50-lmax..48 reaches the state the
-// whole walk reaches unless a word ended earlier - and such a word, wholly
-// inside symbols 0..48, shows in the prefilter, whose hits run the whole walk.
-// Same result as regex_key2 for such a set; the long walks run only where a
-// window hits.
+// The key stage of a dictionary of plain substrings behind the window
+// prefilter (`maps`, kHashWindow, the longest word `lmax`): 2 when a word has
+// matched, 1 when the address stage must decide, 0 when no word can occur.
+// A word that ends at symbol 49 or later starts at 50 - lmax or later, and the
+// automaton of unanchored words forgets everything older than its longest
+// word, so a walk from the start state over symbols 50-lmax..48 reaches the
+// state the whole walk reaches unless a word ended earlier.  Such a word,
+// wholly inside symbols 0..48, shows in a window that hits both maps, and a
+// key with such a window goes to the address stage (1), whose exact pass
+// over the whole address finds it: parked candidates are compacted per wave,
+// so the rare window hits cost one address stage each instead of a whole
+// walk for every lane of their waves.  The verdict differs from regex_key2's
+// only by 1 in place of 0 or 2 where a window hits, which the address stage
+// settles.
 template <class T16, class T32, class TL>
 OG_HD uint32_t regex_key2_prefiltered(const uint32_t wa[8], const uint32_t wb[8],
                                       const uint32_t *recs, T16 dfa, T32 live, uint32_t n,
@@ -419,14 +424,12 @@ OG_HD uint32_t regex_key2_prefiltered(const uint32_t wa[8], const uint32_t wb[8]
     pa = window_hits_from<0, 1>(wa, maps) != 0;
     pb = window_hits_from<0, 1>(wb, maps) != 0;
   }
-  uint32_t ra = 0, rb = 0;
+  uint32_t ra = pa ? 1u : 0u, rb = pb ? 1u : 0u;
   for (uint32_t r = 0; r < n && (ra != 2 || rb != 2); ++r) {
     const T16 t = wave_uniform(dfa + recs[3 * r]);
     const T32 lv = live + recs[3 * r + 2];
     const uint32_t start = recs[3 * r + 1];
-    uint32_t sa = walk_tail(t, start, wa, j0), sb = walk_tail(t, start, wb, j0);
-    if (pa && sa != 1) sa = regex_walk49(t, start, wa) == 1 ? 1u : sa;
-    if (pb && sb != 1) sb = regex_walk49(t, start, wb) == 1 ? 1u : sb;
+    const uint32_t sa = walk_tail(t, start, wa, j0), sb = walk_tail(t, start, wb, j0);
     if (sa == 1) ra = 2;
     else if (sa != 0 && !ra && regex_open(t, lv, sa, wa)) ra = 1;
     if (sb == 1) rb = 2;

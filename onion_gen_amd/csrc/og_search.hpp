@@ -304,10 +304,19 @@ __device__ __forceinline__ void test_key_only(const Fe10 &y, uint64_t k, const D
 // Lanes where an expression's proxy occurred (all lanes, for an expression
 // without one) run the automata over symbols 0..48: matched is a hit, still
 // open asks for the address.
+//
+// A set whose key stage would send nearly every candidate on (park_all)
+// skips it: the key forms decide their hits, and every other candidate goes
+// to the address stage, whose exact pass covers all 56 symbols.
 template <uint32_t MODE>
 __device__ __forceinline__ void key_stage(const Fe &c, const uint32_t w[8], const DevFilterArgs &f,
                                           const ScanBank *banks, const uint16_t *dfa_lds,
                                           bool &hit, bool &need) {
+  if (f.park_all) {
+    hit = (MODE & (kModePrefix | kModeBitmap)) ? exact_key<MODE>(c, f) : false;
+    need = !hit;
+    return;
+  }
   bool maybe;
   scan_key(w, banks, f.nbanks, hit, need, maybe);
   if (MODE & (kModePrefix | kModeBitmap)) hit |= exact_key<MODE>(c, f);
@@ -331,6 +340,16 @@ __device__ __forceinline__ void key_stage_pair(const Fe &ca, const uint32_t wa[8
                                                const uint32_t wb[8], const DevFilterArgs &f,
                                                const ScanBank *banks, const uint16_t *dfa_lds,
                                                bool &ha, bool &na, bool &hb, bool &nb) {
+  if (f.park_all) {
+    ha = hb = false;
+    if (MODE & (kModePrefix | kModeBitmap)) {
+      ha = exact_key<MODE>(ca, f);
+      hb = exact_key<MODE>(cb, f);
+    }
+    na = !ha;
+    nb = !hb;
+    return;
+  }
   bool ma, mb;
   scan_key(wa, banks, f.nbanks, ha, na, ma);
   scan_key(wb, banks, f.nbanks, hb, nb, mb);

@@ -154,9 +154,10 @@ def test_dictionary_of_general_forms_matches_oracle():
 
 def test_dictionary_split_across_automata_matches_every_address():
     """10,000 eight-symbol substrings exceed one automaton's states and are
-    split across two; the engine reports exactly the candidates whose address
-    holds one of the words (decided here by a set of the words over every
-    8-symbol window of the oracle's addresses)."""
+    split across two, behind the window prefilter (about 11% of keys go to the
+    address stage through a window hit); the engine reports exactly the
+    candidates whose address holds one of the words (decided here by a set of
+    the words over every 8-symbol window of the oracle's addresses)."""
     import random
     say("case: 10,000 substrings in two automata, n=2^16")
     n = 1 << 16
@@ -170,6 +171,7 @@ def test_dictionary_split_across_automata_matches_every_address():
         words.add(addr[s:s + 8])
     fs = filters.FilterSet.parse(["contains:" + w for w in sorted(words)])
     assert any(l.startswith("dictionary automata: 2 of") for l in fs.describe())
+    assert any(l.startswith("window prefilter") for l in fs.describe())
     got = run_engine(start, fs, n, threads=1024, half=64)
     want = sorted((k, clear_sign(pk)) for k, pk, addr in every
                   if any(addr[s:s + 8] in words for s in range(49)))
@@ -179,12 +181,13 @@ def test_dictionary_split_across_automata_matches_every_address():
 
 def test_dictionary_behind_the_window_prefilter_matches_every_address():
     """1,200 eight-symbol substrings (one automaton in global memory) take the
-    window prefilter: every key walks symbols 42..48, and the whole walk runs
-    only where a 7-symbol window hits both maps.  A twentieth of the words are
-    taken from the range's addresses at every placement, those that run into
-    the checksum included.  The engine reports exactly the candidates whose
-    address holds a word, as a set of the words over every 8-symbol window of
-    the oracle's addresses decides."""
+    window prefilter: every key walks symbols 42..48, and a key whose 7-symbol
+    window hits both maps goes to the address stage.  A twentieth of the words
+    are taken from the range's addresses at every placement, those that run
+    into the checksum included.  The engine reports exactly the candidates
+    whose address holds a word, as a set of the words over every 8-symbol
+    window of the oracle's addresses decides, in one-round launches (the
+    address kernel) and in two-round launches (drained in place)."""
     import random
     say("case: 1,200 substrings behind the window prefilter, n=2^16")
     n = 1 << 16
@@ -198,19 +201,22 @@ def test_dictionary_behind_the_window_prefilter_matches_every_address():
         words.add(addr[s:s + 8])
     fs = filters.FilterSet.parse(["contains:" + w for w in sorted(words)])
     assert any(l.startswith("window prefilter") for l in fs.describe())
-    got = run_engine(start, fs, n, threads=1024, half=64)
     want = sorted((k, clear_sign(pk)) for k, pk, addr in every
                   if any(addr[s:s + 8] in words for s in range(49)))
     assert len(want) >= 60
-    assert got == want, (len(got), len(want))
+    for rounds in (1, 2):
+        got = run_engine(start, fs, n, threads=1024, half=64, rounds=rounds)
+        assert got == want, (rounds, len(got), len(want))
 
 
 def test_dictionary_that_reaches_the_checksum_matches_every_address():
     """1,000 five-symbol substrings: a word can still end in the checksum for
-    most candidates, so most park, and each lane drains a share of many
-    entries.  The engine reports exactly the candidates whose address holds a
-    word anywhere in its 56 symbols, in one-round launches (the address
-    kernel) and in two-round launches (drained in place)."""
+    nearly every candidate, so the key stage is skipped and every candidate
+    parks; each lane drains a share of many entries.  The engine reports
+    exactly the candidates whose address holds a word anywhere in its 56
+    symbols, in one-round launches (the address kernel) and in two-round
+    launches (drained in place), alone and beside a literal prefix, whose
+    hits the key stage still decides."""
     import random
     say("case: 1,000 five-symbol substrings, n=2^17")
     n = 1 << 17
@@ -219,13 +225,23 @@ def test_dictionary_that_reaches_the_checksum_matches_every_address():
     rng = random.Random(11)
     A = workload.ALPHABET
     words = {"".join(rng.choice(A) for _ in range(5)) for _ in range(1000)}
-    fs = filters.FilterSet.parse(["contains:" + w for w in sorted(words)])
+    lines = ["contains:" + w for w in sorted(words)]
+    fs = filters.FilterSet.parse(lines)
+    assert any(l.startswith("key stage skipped") for l in fs.describe())
     want = sorted((k, clear_sign(pk)) for k, pk, addr in every
                   if any(addr[s:s + 5] in words for s in range(52)))
     assert len(want) > 100
     for rounds in (1, 2):
         got = run_engine(start, fs, n, threads=256, half=64, rounds=rounds)
         assert got == want, (rounds, len(got), len(want))
+    say("case: the same beside the prefix zz, n=2^17")
+    fs = filters.FilterSet.parse(lines + ["zz"])
+    assert any(l.startswith("key stage skipped") for l in fs.describe())
+    want = sorted((k, clear_sign(pk)) for k, pk, addr in every
+                  if addr.startswith("zz") or any(addr[s:s + 5] in words for s in range(52)))
+    assert sum(1 for k, pk, addr in every if addr.startswith("zz")) > 50
+    got = run_engine(start, fs, n, threads=256, half=64)
+    assert got == want, (len(got), len(want))
 
 
 def test_address_stage_in_place_and_deferred_agree():
