@@ -187,7 +187,8 @@ int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
  * roofline's denominator, the peak rate of the VALU instruction the field
  * multiplication is made of): `threads` lanes each run eight independent
  * chains of `iters` x 8 instructions of one kind - op 0 v_mad_u64_u32, 1
- * v_mul_lo_u32, 2 v_lshrrev_b64, 3 v_add_u32.  Writes lane-instructions per
+ * v_mul_lo_u32, 2 v_lshrrev_b64, 3 v_add_u32, 4 v_bitop3_b32, 5
+ * v_alignbit_b32 (the last two make up the Keccak rounds).  Writes lane-instructions per
  * second and the shader clock the SIMDs ran at (s_memtime against the
  * constant-rate s_memrealtime, in Hz; 0 when unknown). */
 int og_hip_issue_bench(int ordinal, int op, uint32_t threads, uint32_t iters, double *ops_per_s,
@@ -333,6 +334,14 @@ int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int si
  * general filters or expressions in the automaton banks). */
 int og_filters_scan_pair_host(const og_filterset *s, const uint8_t a[32], const uint8_t b[32],
                               int key_hit[2], int need[2]);
+/* The address stage's automata on a 35-byte address (packed key, checksum,
+ * version) as the address kernels run them: *match = whether one accepts it,
+ * *window = whether the set's address-stage window prefilter lets the walk
+ * run (1 for a set without one).  No counterpart in the reference; it exists
+ * so that the prefilter is checked never to stop a walk that would match.
+ * Returns 0, or -1 for a null argument or a set without automata. */
+int og_filters_address_window_host(const og_filterset *s, const uint8_t address[35],
+                                   int *window, int *match);
 
 /* ------------------------------------------------------------------ batch */
 

@@ -687,7 +687,7 @@ int og_batch_step(og_batch *b, og_batch_hit *out, uint32_t cap, uint32_t *n, cha
       b->assigned[unit] = id;
       d.f = sr.filt.args();
       d.f.hash = nullptr;  // the slot kernels keep the automaton path
-      d.f.hash_lmax = 0;
+      d.f.hash_key = d.f.hash_addr = 0;
       d.k_base = sr.k_next;
       d.state = sr.d_state;
       d.stride = stride;

@@ -109,7 +109,9 @@ struct DevFilterHost {
   std::vector<uint32_t> bitmap;   // 32-bit halves of the 64-bit words
   std::vector<uint32_t> hash;     // the window prefilter's two maps, or empty
   uint32_t hash_lmax = 0;         // its longest word
-  uint32_t hash_stride = 1;       // 2: both windows of every word are mapped
+  uint32_t hash_ws = 0;           // its form: 8 x window length + stride
+  bool hash_key = false;          // the key stage uses it
+  bool hash_addr = false;         // the address stage uses it
   std::vector<uint32_t> coarse;   // the bitmap or-folded to coarse_bits bits
   uint32_t coarse_bits = 0;
   std::vector<uint32_t> buckets;  // [offset, count, mask_lo, mask_hi]
@@ -133,6 +135,7 @@ struct DevFilterHost {
   bool park_all = false;
   double window_park = 0.0;       // the window prefilter's expected share of
                                   //   keys sent to the address stage
+  double window_share = 0.0;      // the same, or (hash_addr) of waves that walk
   // General filters beyond the banks, compiled into automata (a dictionary
   // of substrings and suffixes): one, or several when one would exceed the
   // state limit; run on every candidate.
@@ -297,7 +300,6 @@ struct DevFilterHost {
         const std::string eb = build_banks(fs, banks, &proxyless, false);
         if (!eb.empty()) return eb;
         for (const auto &d : dictionaries) add_automaton(*d);
-        build_window_prefilter(fs);
       } else {
         for (uint32_t i : fs.general) {
           DevPattern p = dev_pattern(fs.filters[i]);
@@ -314,58 +316,107 @@ struct DevFilterHost {
     if (!mode) return "the filter set is empty";
     need = need_rate(kNeedSamples);
     park_all = (mode & kModePattern) && need >= kParkAllRate;
+    build_window_prefilter(fs);
     return "";
   }
 
   // The window prefilter, for a dictionary that is only plain substrings -
-  // unanchored, one symbol per position, kHashWindow to 49 symbols - walked in
-  // global memory (an automaton in LDS walks faster than this saves), with no
-  // expression and no bank beside it.  A key with a window that hits both maps
-  // goes to the address stage, so the maps are kept only while the expected
-  // share of such keys, 1 - (1 - a b)^probes for map occupancies a and b, is
-  // at most kWindowParkMax.
+  // unanchored, one symbol per position, 5 to 49 symbols - walked in global
+  // memory (an automaton in LDS walks faster than this saves), with no
+  // expression and no bank beside it.  Its form - window length W and
+  // placement stride S, W + S - 1 at most the shortest word - is the one that
+  // costs least: probes per key (a few instructions and two LDS reads each)
+  // against the keys its false hits send on.
+  // - The key stage (hash_key) sends a key whose window hits to the address
+  //   stage.  A form is worth it while the expected share of such keys is at
+  //   most kWindowParkMax; each costs about what 400 probes do.
+  // - When nearly every candidate goes to the address stage anyway
+  //   (park_all), the address stage uses the maps instead (hash_addr): its
+  //   whole walk runs only in waves where a lane's window hits, worth it while
+  //   at most half the waves do; a wave's walk costs about what 250 probes do.
   void build_window_prefilter(const FilterSet &fs) {
     hash.clear();
-    hash_lmax = 0;
-    if (!fs.regexes.empty() || !banks.empty() || regex.empty() || first_states <= kLdsStates)
+    hash_key = hash_addr = false;
+    hash_lmax = hash_ws = 0;
+    window_park = window_share = 0.0;
+    if (dictionaries.empty() || !fs.regexes.empty() || !banks.empty() || regex.empty() ||
+        first_states <= kLdsStates)
       return;
     uint32_t lmax = 0, lmin = 49;
     for (uint32_t i : fs.general) {
       const Filter &f = fs.filters[i];
-      if (f.anchor != kAnywhere || f.sets.size() < (size_t)kHashWindow || f.sets.size() > 49)
-        return;
+      if (f.anchor != kAnywhere || f.sets.size() < 5 || f.sets.size() > 49) return;
       for (uint32_t set : f.sets)
         if (__builtin_popcount(set) != 1) return;
       lmax = std::max<uint32_t>(lmax, (uint32_t)f.sets.size());
       lmin = std::min<uint32_t>(lmin, (uint32_t)f.sets.size());
     }
-    // every second placement suffices when each word's windows at its symbols
-    // 0 and 1 are both mapped: a word at an odd placement shows at the next
-    const uint32_t stride = lmin >= (uint32_t)kHashWindow + 1 ? 2u : 1u;
+    const uint32_t end = park_all ? 56 : 49;  // the symbols probed
+    double best = 1e300, best_share = 1.0;
+    uint32_t best_ws = 0;
+    std::vector<uint32_t> best_maps;
+    for (uint32_t W = 5; W <= 7 && W <= lmin; ++W)
+      for (uint32_t S = 1; S <= 4 && W + S - 1 <= lmin; ++S) {
+        std::vector<uint32_t> maps = window_maps(fs, W, S);
+        const double p = window_probe_rate(maps, W);
+        const uint32_t probes = (end - W) / S + 1;
+        const double keys = 1.0 - std::pow(1.0 - p, (double)probes);
+        const double share = park_all ? 1.0 - std::pow(1.0 - keys, 64.0) : keys;
+        const double cost = probes + share * (park_all ? 250.0 : 400.0);
+        if (cost < best) {
+          best = cost;
+          best_share = share;
+          best_ws = 8 * W + S;
+          best_maps.swap(maps);
+        }
+      }
+    if (!best_ws || best_share > (park_all ? 0.5 : kWindowParkMax)) return;
+    hash.swap(best_maps);
+    hash_lmax = lmax;
+    hash_ws = best_ws;
+    window_park = park_all ? 0.0 : best_share;
+    window_share = best_share;
+    hash_key = !park_all;
+    hash_addr = park_all;
+  }
+
+  // The two maps of the form (W, S): each word's windows at its symbols
+  // 0..S-1, the first 18 of their 5W bits in one map and the last 18 in the
+  // other.
+  static std::vector<uint32_t> window_maps(const FilterSet &fs, uint32_t W, uint32_t S) {
     std::vector<uint32_t> maps(2 * kHashSliceWords, 0u);
     for (uint32_t i : fs.general)
-      for (uint32_t off = 0; off < stride; ++off) {
-        uint64_t v = 0;  // kHashWindow symbols from `off`, 35 bits, big-endian
-        for (int j = 0; j < kHashWindow; ++j)
+      for (uint32_t off = 0; off < S; ++off) {
+        uint64_t v = 0;  // W symbols from `off`, big-endian
+        for (uint32_t j = 0; j < W; ++j)
           v = (v << 5) | (uint32_t)__builtin_ctz(fs.filters[i].sets[off + j]);
-        const uint32_t a = (uint32_t)(v >> (5 * kHashWindow - kHashSliceBits));
+        const uint32_t a = (uint32_t)(v >> (5 * W - kHashSliceBits));
         const uint32_t b = (uint32_t)(v & ((1u << kHashSliceBits) - 1));
         maps[a >> 5] |= 1u << (a & 31);
         maps[kHashSliceWords + (b >> 5)] |= 1u << (b & 31);
       }
-    double fill[2] = {0, 0};
-    for (uint32_t m = 0; m < 2; ++m) {
-      for (uint32_t i = 0; i < kHashSliceWords; ++i)
-        fill[m] += __builtin_popcount(maps[m * kHashSliceWords + i]);
-      fill[m] /= (double)(1u << kHashSliceBits);
-    }
-    const uint32_t placements = 49 - kHashWindow + 1;
-    const uint32_t probes = (placements + stride - 1) / stride;
-    window_park = 1.0 - std::pow(1.0 - fill[0] * fill[1], (double)probes);
-    if (window_park > kWindowParkMax) return;
-    hash.swap(maps);
-    hash_lmax = lmax;
-    hash_stride = stride;
+    return maps;
+  }
+
+  // The chance that a uniformly random W-symbol window hits both maps.  The
+  // two slices share their middle 36 - 5W bits (1 for W = 7, 11 for W = 5),
+  // so it is the sum over the first map's entries of the share of the second
+  // map's entries that agree with it there.
+  static double window_probe_rate(const std::vector<uint32_t> &maps, uint32_t W) {
+    const uint32_t ov = 36 - 5 * W, rest = kHashSliceBits - ov;
+    std::vector<uint32_t> agree((size_t)1 << ov, 0u);  // second-map entries per shared value
+    for (uint32_t i = 0; i < kHashSliceWords; ++i)
+      for (uint32_t m = maps[kHashSliceWords + i]; m; m &= m - 1) {
+        const uint32_t b = (i << 5) | (uint32_t)__builtin_ctz(m);
+        agree[b >> rest]++;
+      }
+    double sum = 0;
+    for (uint32_t i = 0; i < kHashSliceWords; ++i)
+      for (uint32_t m = maps[i]; m; m &= m - 1) {
+        const uint32_t a = (i << 5) | (uint32_t)__builtin_ctz(m);
+        sum += agree[a & ((1u << ov) - 1)];
+      }
+    return sum / (double)(1u << kHashSliceBits) / (double)(1u << rest);
   }
 
   // `bits` must be set first.  Slot s of the map is bit s & 31 of word s >> 5
@@ -446,8 +497,10 @@ struct DevFilterHost {
     a.bitmap = d_bitmap;
     a.coarse = d_coarse;
     a.hash = d_hash;
-    a.hash_lmax = d_hash ? hash_lmax : 0u;
-    a.hash_stride = hash_stride;
+    a.hash_lmax = hash_lmax;
+    a.hash_ws = hash_ws;
+    a.hash_key = d_hash && hash_key ? 1u : 0u;
+    a.hash_addr = d_hash && hash_addr ? 1u : 0u;
     a.coarse_bits = d_coarse ? coarse_bits : 0u;
     a.nbuckets = (uint32_t)(buckets.size() / 4);
     a.buckets = d_buckets;

@@ -69,20 +69,25 @@ __global__ __launch_bounds__(256) OG_SEARCH_ATTR(MODE) void og_search_kernel(Sea
 // runs at four waves per SIMD instead of two; the form that derives the signs
 // (products, an inversion per chunk) is its own instantiation, held to two
 // waves so that it does not spill, and the plain one keeps its registers.
-template <bool kSign>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSign ? 2 : 4, 8))) void
-og_address_kernel(SearchArgs a) {
+// kMaps: the set's window prefilter serves the address stage; its two maps
+// (64 KiB) take the LDS, so a CU holds two workgroups, of 1024 threads when
+// the launch allows it.
+template <bool kSign, bool kMaps>
+__global__ __launch_bounds__(kMaps ? 1024 : 256)
+    __attribute__((amdgpu_waves_per_eu(kSign ? 2 : 4, 8))) void og_address_kernel(SearchArgs a) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ ScanBank banks[kMaxBanks];
-  __shared__ uint16_t dfa_lds[kLdsStates * 32];
-  load_filter(banks, dfa_lds, a.f);
+  __shared__ uint32_t lds_words[kMaps ? kPatternLdsWords : kLdsStates * 16];
+  uint16_t *dfa_lds = reinterpret_cast<uint16_t *>(lds_words);
+  load_filter(banks, dfa_lds, a.f, kMaps);
   HitSink sink{a.counter, a.hits, a.cap, 0, nullptr, 0};
   const uint32_t top = kPlanesPerPair * a.half;
+  const lptr32 maps = kMaps ? (lptr32)lds_words : (lptr32) nullptr;
   if (kSign)
     drain_deferred(a.scratch, top, a.half, a.threads, g, a.parked[g], a.k_base, (cptr)a.table,
-                   a.f, banks, sink);
+                   a.f, banks, maps, sink);
   else
-    drain<kModePattern>(a.scratch, top, a.threads, g, a.parked[g], a.f, banks, sink);
+    drain<kModePattern>(a.scratch, top, a.threads, g, a.parked[g], a.f, banks, maps, sink);
 }
 
 __global__ __launch_bounds__(256) void og_setup_kernel(SetupArgs a) {
@@ -221,7 +226,9 @@ __global__ __launch_bounds__(256) void og_radix_kernel(uint32_t iters, uint32_t 
 // constant-rate clock (s_memrealtime) around the loop, from which the host
 // derives the clock the SIMDs ran at.  OP 0: v_mad_u64_u32 (the product the
 // field multiplication issues 100 of), 1: v_mul_lo_u32, 2: v_lshrrev_b64,
-// 3: v_add_u32.
+// 3: v_add_u32, 4: v_bitop3_b32 (a three-input XOR) and 5: v_alignbit_b32 (a
+// 32-bit half of a 64-bit rotation), the two instructions the Keccak rounds
+// are made of.
 template <int OP>
 __global__ __launch_bounds__(256) void og_issue_kernel(uint32_t iters, uint64_t *out,
                                                        uint64_t *clocks) {
@@ -255,6 +262,14 @@ __global__ __launch_bounds__(256) void og_issue_kernel(uint32_t iters, uint64_t 
           acc[c] = (acc[c] & 0xffffffff00000000ull) | lo;
         } else if (OP == 2) {
           asm volatile("v_lshrrev_b64 %0, 1, %0" : "+v"(acc[c]));
+        } else if (OP == 4) {
+          uint32_t lo = (uint32_t)acc[c];
+          asm volatile("v_bitop3_b32 %0, %0, %1, %2 bitop3:0x96" : "+v"(lo) : "v"(a[c]), "v"(b[c]));
+          acc[c] = (acc[c] & 0xffffffff00000000ull) | lo;
+        } else if (OP == 5) {
+          uint32_t lo = (uint32_t)acc[c];
+          asm volatile("v_alignbit_b32 %0, %0, %1, 7" : "+v"(lo) : "v"(b[c]));
+          acc[c] = (acc[c] & 0xffffffff00000000ull) | lo;
         } else {
           uint32_t lo = (uint32_t)acc[c];
           asm volatile("v_add_u32 %0, %0, %1" : "+v"(lo) : "v"(b[c]));
@@ -339,12 +354,21 @@ hipError_t launch_search(og_engine *e) {
   const uint32_t grid = e->threads / e->block;
   hipLaunchKernelGGL(og_search_kernel<MODE>, dim3(grid), dim3(e->block), 0, e->stream, e->args);
   if ((MODE & kModePattern) && e->args.parked) {
-    if (e->args.f.defer_sign)
-      hipLaunchKernelGGL(og_address_kernel<true>, dim3(grid), dim3(e->block), 0, e->stream,
-                         e->args);
-    else
-      hipLaunchKernelGGL(og_address_kernel<false>, dim3(grid), dim3(e->block), 0, e->stream,
-                         e->args);
+    const bool sign = e->args.f.defer_sign != 0;
+    if (e->args.f.hash_addr) {
+      const uint32_t block = e->threads % 1024 == 0 ? 1024 : e->block;
+      const dim3 g(e->threads / block), b(block);
+      if (sign)
+        hipLaunchKernelGGL((og_address_kernel<true, true>), g, b, 0, e->stream, e->args);
+      else
+        hipLaunchKernelGGL((og_address_kernel<false, true>), g, b, 0, e->stream, e->args);
+    } else if (sign) {
+      hipLaunchKernelGGL((og_address_kernel<true, false>), dim3(grid), dim3(e->block), 0,
+                         e->stream, e->args);
+    } else {
+      hipLaunchKernelGGL((og_address_kernel<false, false>), dim3(grid), dim3(e->block), 0,
+                         e->stream, e->args);
+    }
   }
   return hipGetLastError();
 }
@@ -531,7 +555,7 @@ int create_engine(int ordinal, const uint8_t start_xyzt[128], const og_engine_op
     OG_TRY(hipMalloc(&e->d_parked, (size_t)o.threads * 4));
     a.parked = e->d_parked;
     // the share that parks: the key stage's, plus the window prefilter's hits
-    const double parks = e->filt.need + (e->filt.hash_lmax ? e->filt.window_park : 0.0);
+    const double parks = e->filt.need + (e->filt.hash_key ? e->filt.window_park : 0.0);
     if (o.half >= 8 && o.sign_stage != 1)
       a.f.defer_sign = o.sign_stage == 2 || parks < 0.125 ? 1u : 0u;
   } else {
@@ -897,8 +921,8 @@ int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
 
 int og_hip_issue_bench(int ordinal, int op, uint32_t threads, uint32_t iters, double *ops_per_s,
                        double *clock_hz, char *err, size_t errcap) {
-  if (op < 0 || op > 3 || threads == 0 || threads % 256 || threads > (1u << 22) || iters == 0) {
-    put_err(err, errcap, "op must be 0..3, threads a multiple of 256 up to 2^22, iters positive");
+  if (op < 0 || op > 5 || threads == 0 || threads % 256 || threads > (1u << 22) || iters == 0) {
+    put_err(err, errcap, "op must be 0..5, threads a multiple of 256 up to 2^22, iters positive");
     return -1;
   }
   OG_TRY(hipSetDevice(ordinal));
@@ -916,6 +940,8 @@ int og_hip_issue_bench(int ordinal, int op, uint32_t threads, uint32_t iters, do
       case 0: hipLaunchKernelGGL(og_issue_kernel<0>, dim3(blocks), dim3(256), 0, 0, it, d, clk); break;
       case 1: hipLaunchKernelGGL(og_issue_kernel<1>, dim3(blocks), dim3(256), 0, 0, it, d, clk); break;
       case 2: hipLaunchKernelGGL(og_issue_kernel<2>, dim3(blocks), dim3(256), 0, 0, it, d, clk); break;
+      case 4: hipLaunchKernelGGL(og_issue_kernel<4>, dim3(blocks), dim3(256), 0, 0, it, d, clk); break;
+      case 5: hipLaunchKernelGGL(og_issue_kernel<5>, dim3(blocks), dim3(256), 0, 0, it, d, clk); break;
       default: hipLaunchKernelGGL(og_issue_kernel<3>, dim3(blocks), dim3(256), 0, 0, it, d, clk);
     }
   };

@@ -294,9 +294,9 @@ int og_filters_scan_pair_host(const og_filterset *s, const uint8_t a[32], const 
     if (maybe[0] || maybe[1]) {
       const uint32_t K = h.first_states <= kLdsStates ? h.first_states : 0u;
       const uint32_t r =
-          h.hash_lmax ? regex_key2_prefiltered(kw[0], kw[1], h.regex.data(), h.dfa.data(),
-                                               h.live.data(), nre, h.hash.data(), h.hash_lmax,
-                                               h.hash_stride)
+          h.hash_key ? regex_key2_prefiltered(kw[0], kw[1], h.regex.data(), h.dfa.data(),
+                                              h.live.data(), nre, h.hash.data(), h.hash_lmax,
+                                              h.hash_ws)
                       : regex_key2(kw[0], kw[1], h.regex.data(), h.dfa.data(), h.live.data(),
                                    nre, h.dfa.data() + h.regex[0], K);
       const uint32_t ri[2] = {maybe[0] ? (r & 3u) : 0u, maybe[1] ? (r >> 2) : 0u};
@@ -310,6 +310,23 @@ int og_filters_scan_pair_host(const og_filterset *s, const uint8_t a[32], const 
     key_hit[i] = kh[i];
     need[i] = nd[i] && !kh[i];
   }
+  return 0;
+}
+
+int og_filters_address_window_host(const og_filterset *s, const uint8_t address[35], int *window,
+                                   int *match) {
+  if (!s || !address || !window || !match) return -1;
+  const DevFilterHost &h = const_cast<og_filterset *>(s)->host_tables();
+  if (!s->tables_err.empty() || h.regex.empty()) return -1;
+  uint32_t w[9];  // big-endian words, as the address kernels form them
+  for (int i = 0; i < 8; ++i)
+    w[i] = ((uint32_t)address[4 * i] << 24) | ((uint32_t)address[4 * i + 1] << 16) |
+           ((uint32_t)address[4 * i + 2] << 8) | address[4 * i + 3];
+  w[8] = ((uint32_t)address[32] << 24) | ((uint32_t)address[33] << 16) |
+         ((uint32_t)address[34] << 8);
+  *window = !h.hash_addr || window_hit<56, 9>(w, h.hash.data(), h.hash_ws);
+  *match = regex_address(w, h.regex.data(), h.dfa.data(), h.dfa_end.data(),
+                         (uint32_t)(h.regex.size() / 3));
   return 0;
 }
 
@@ -331,13 +348,23 @@ size_t og_filters_describe(const og_filterset *s, char *out, size_t cap) {
          std::to_string(states) + " states over " + std::to_string(h.dictionary_forms) +
          " general form(s), run on every candidate; the checksum only where a match can still "
          "end in it\n";
-    if (h.hash_lmax) {
+    if (!h.hash.empty()) {
       char pct[16];
-      std::snprintf(pct, sizeof pct, "%.2f", h.window_park * 100.0);
-      t += "window prefilter: two 2^18-bit maps over every word's first 7 symbols, in LDS; "
-           "the automaton walks symbols " + std::to_string(50 - h.hash_lmax) +
-           "..48 for every candidate, and a key whose window hits (about " + pct +
-           "%) goes to the address stage\n";
+      std::snprintf(pct, sizeof pct, "%.2f", h.window_share * 100.0);
+      const std::string form = "two 2^18-bit maps over " + std::to_string(h.hash_ws / 8) +
+                               "-symbol windows at every " +
+                               (h.hash_ws % 8 == 1 ? std::string("")
+                                                   : std::to_string(h.hash_ws % 8) + "th ") +
+                               "placement, in LDS; ";
+      if (h.hash_key)
+        t += "window prefilter: " + form + "the automaton walks symbols " +
+             std::to_string(50 - h.hash_lmax) +
+             "..48 for every candidate, and a key whose window hits (about " + pct +
+             "%) goes to the address stage\n";
+      else
+        t += "window prefilter in the address stage: " + form +
+             "the whole walk runs only in waves where a window of the 56 symbols hits "
+             "(about " + pct + "% of waves)\n";
     }
   }
   if (h.park_all) {

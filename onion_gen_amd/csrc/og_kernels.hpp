@@ -78,9 +78,10 @@ constexpr uint32_t kCoarseWords = 1u << (kCoarseBits - 5);
 // parked lane then forms.
 constexpr uint32_t kNeedSamples = 4096;
 constexpr double kParkAllRate = 0.9;
-// The window prefilter sends a key whose window hits both maps to the address
-// stage; it is used while the expected share of such keys stays at most this
-// (about 27 ps a candidate against 64 ps for one automaton's whole walk).
+// The window prefilter in the key stage sends a key whose window hits both
+// maps to the address stage; it is used while the expected share of such keys
+// stays at most this (about 27 ps a candidate against 64 ps for one
+// automaton's whole walk).
 constexpr double kWindowParkMax = 0.25;
 
 struct DevFilterArgs {
@@ -115,10 +116,12 @@ struct DevFilterArgs {
                                   //   stage: it is skipped, and every candidate
                                   //   the key forms leave goes there
   const uint32_t *hash;           // (kModePattern) a dictionary of plain
-  uint32_t hash_lmax;             //   substrings: the window prefilter's two
-                                  //   maps, and its longest word (0: none)
-  uint32_t hash_stride;           //   placements probed: every one, or every
-                                  //   second (both windows of a word mapped)
+                                  //   substrings: the window prefilter's two
+                                  //   maps (null: none)
+  uint32_t hash_lmax;             //   its longest word
+  uint32_t hash_ws;               //   its form, 8 x window length + stride
+  uint32_t hash_key;              //   the key stage probes it
+  uint32_t hash_addr;             //   the address stage probes it
   const uint32_t *coarse;         // (kModeBitmap) the bitmap or-folded to
   uint32_t coarse_bits;           //   coarse_bits <= kCoarseBits bits: each
                                   //   workgroup's LDS copy is probed first, and

@@ -297,49 +297,58 @@ OG_HD void regex_walk49x2(TP t, uint32_t &sa, uint32_t &sb, const uint32_t wa[8]
 }
 
 // The window prefilter of a dictionary of plain substrings (every word at
-// least kHashWindow symbols, unanchored, one symbol per position): two
-// bitmaps of 2^18 bits, one over bits [0, 18) and one over bits [17, 35) of
-// each word's first kHashWindow symbols.  A key whose every window of that
-// length at symbols 0..49-kHashWindow misses one of the two maps holds no
-// word inside symbols 0..48; a window that hits both is checked exactly.
-constexpr int kHashWindow = 7;
+// least W symbols, unanchored, one symbol per position): two bitmaps of 2^18
+// bits, one over the first 18 and one over the last 18 of the 5W bits of a
+// W-symbol window.  Each word maps its windows at its symbols 0..S-1 (every
+// word has at least W + S - 1 symbols), so a word at placement p shows in the
+// window at the next multiple of S, which ends no later than the word does:
+// probing placements 0, S, 2S, ... finds every word, and a string whose
+// probed windows all miss one of the maps holds none.  W is 5, 6 or 7 and S
+// 1 to 4, packed as 8 W + S (DevFilterArgs::hash_ws).
 constexpr uint32_t kHashSliceBits = 18;
 constexpr uint32_t kHashSliceWords = 1u << (kHashSliceBits - 5);  // per map
 
-// The 32 key bits from bit `pos` on (big-endian words; past the key, zeros).
-template <int POS>
+// The 32 bits from bit POS on of a string held in N big-endian words (zeros
+// past them).
+template <int POS, int N>
 OG_HD uint32_t key_bits32(const uint32_t *w) {
   constexpr int i = POS >> 5, sh = POS & 31;
+  static_assert(i < N, "window past the string");
   if constexpr (sh == 0) {
     return w[i];
   } else {
-    const uint32_t next = i + 1 < 8 ? w[i + 1 < 8 ? i + 1 : 7] : 0u;
+    const uint32_t next = i + 1 < N ? w[i + 1 < N ? i + 1 : N - 1] : 0u;
     return (w[i] << sh) | (next >> (32 - sh));
   }
 }
 
-// Windows at placements 0, STRIDE, 2 STRIDE, ... up to 49 - kHashWindow.  With
-// stride 2 the maps hold both windows of each word (at its symbols 0 and 1,
-// every word having at least kHashWindow + 1 symbols), so a word at any
-// placement shows in a probed window.
-template <int J, int STRIDE, class TL>
+// Whether a window at placements J, J + S, ... <= LAST hits both maps.
+template <int W, int S, int LAST, int N, int J, class TL>
 OG_HD uint32_t window_hits_from(const uint32_t *w, TL maps) {
-  if constexpr (J > 49 - kHashWindow) {
+  if constexpr (J > LAST) {
     return 0u;
   } else {
-    const uint32_t a = key_bits32<5 * J>(w) >> (32 - kHashSliceBits);
-    const uint32_t b = key_bits32<5 * J + 17>(w) >> (32 - kHashSliceBits);
-    const uint32_t hit = (maps[a >> 5] >> (a & 31)) & (maps[kHashSliceWords + (b >> 5)] >> (b & 31));
-    if constexpr (J + STRIDE > 49 - kHashWindow && J != 49 - kHashWindow) {
-      // the last window placement, 49 - kHashWindow, is always probed
-      const uint32_t a2 = key_bits32<5 * (49 - kHashWindow)>(w) >> (32 - kHashSliceBits);
-      const uint32_t b2 = key_bits32<5 * (49 - kHashWindow) + 17>(w) >> (32 - kHashSliceBits);
-      const uint32_t hit2 =
-          (maps[a2 >> 5] >> (a2 & 31)) & (maps[kHashSliceWords + (b2 >> 5)] >> (b2 & 31));
-      return (hit | hit2) & 1u;
-    } else {
-      return (hit & 1u) | window_hits_from<J + STRIDE, STRIDE>(w, maps);
-    }
+    const uint32_t a = key_bits32<5 * J, N>(w) >> (32 - kHashSliceBits);
+    const uint32_t b = key_bits32<5 * (J + W) - kHashSliceBits, N>(w) >> (32 - kHashSliceBits);
+    const uint32_t hit =
+        (maps[a >> 5] >> (a & 31)) & (maps[kHashSliceWords + (b >> 5)] >> (b & 31)) & 1u;
+    return hit | window_hits_from<W, S, LAST, N, J + S>(w, maps);
+  }
+}
+
+// The windows wholly inside the first END symbols of a string of N words
+// (END 49 with the 8 key words: the symbols the key settles; END 56 with the
+// 9 address words: the whole address).
+template <int END, int N, class TL>
+OG_HD bool window_hit(const uint32_t *w, TL maps, uint32_t ws) {
+  switch (ws) {
+#define OG_WINDOW_CASE(W, S) \
+  case 8 * W + S: return window_hits_from<W, S, END - W, N, 0>(w, maps) != 0;
+    OG_WINDOW_CASE(5, 1) OG_WINDOW_CASE(5, 2) OG_WINDOW_CASE(5, 3) OG_WINDOW_CASE(5, 4)
+    OG_WINDOW_CASE(6, 1) OG_WINDOW_CASE(6, 2) OG_WINDOW_CASE(6, 3) OG_WINDOW_CASE(6, 4)
+    OG_WINDOW_CASE(7, 1) OG_WINDOW_CASE(7, 2) OG_WINDOW_CASE(7, 3) OG_WINDOW_CASE(7, 4)
+#undef OG_WINDOW_CASE
+    default: return true;  // no such form: every string goes on to the exact pass
   }
 }
 
@@ -398,7 +407,7 @@ OG_HD uint32_t regex_key2(const uint32_t wa[8], const uint32_t wb[8], const uint
 }
 
 // The key stage of a dictionary of plain substrings behind the window
-// prefilter (`maps`, kHashWindow, the longest word `lmax`): 2 when a word has
+// prefilter (`maps`, its form `ws`, the longest word `lmax`): 2 when a word has
 // matched, 1 when the address stage must decide, 0 when no word can occur.
 // A word that ends at symbol 49 or later starts at 50 - lmax or later, and the
 // automaton of unanchored words forgets everything older than its longest
@@ -414,16 +423,9 @@ OG_HD uint32_t regex_key2(const uint32_t wa[8], const uint32_t wb[8], const uint
 template <class T16, class T32, class TL>
 OG_HD uint32_t regex_key2_prefiltered(const uint32_t wa[8], const uint32_t wb[8],
                                       const uint32_t *recs, T16 dfa, T32 live, uint32_t n,
-                                      TL maps, uint32_t lmax, uint32_t stride) {
+                                      TL maps, uint32_t lmax, uint32_t ws) {
   const uint32_t j0 = lmax < 49 ? 50 - lmax : 1;
-  bool pa, pb;
-  if (stride == 2) {
-    pa = window_hits_from<0, 2>(wa, maps) != 0;
-    pb = window_hits_from<0, 2>(wb, maps) != 0;
-  } else {
-    pa = window_hits_from<0, 1>(wa, maps) != 0;
-    pb = window_hits_from<0, 1>(wb, maps) != 0;
-  }
+  const bool pa = window_hit<49, 8>(wa, maps, ws), pb = window_hit<49, 8>(wb, maps, ws);
   uint32_t ra = pa ? 1u : 0u, rb = pb ? 1u : 0u;
   for (uint32_t r = 0; r < n && (ra != 2 || rb != 2); ++r) {
     const T16 t = wave_uniform(dfa + recs[3 * r]);

@@ -210,10 +210,10 @@ __device__ __noinline__ uint32_t regex_key2_pref_slow(uint32_t a0, uint32_t a1, 
                                                       uint32_t b7, const uint32_t *recs,
                                                       const uint16_t *dfa, const uint32_t *live,
                                                       uint32_t n, lptr32 maps, uint32_t lmax,
-                                                      uint32_t stride) {
+                                                      uint32_t ws) {
   const uint32_t wa[8] = {a0, a1, a2, a3, a4, a5, a6, a7};
   const uint32_t wb[8] = {b0, b1, b2, b3, b4, b5, b6, b7};
-  return regex_key2_prefiltered(wa, wb, recs, (gptr16)dfa, (gptr32)live, n, maps, lmax, stride);
+  return regex_key2_prefiltered(wa, wb, recs, (gptr16)dfa, (gptr32)live, n, maps, lmax, ws);
 }
 
 __device__ __noinline__ bool regex_address_slow(uint32_t w0, uint32_t w1, uint32_t w2,
@@ -362,11 +362,11 @@ __device__ __forceinline__ void key_stage_pair(const Fe &ca, const uint32_t wa[8
     mb = (mb || f.regex_every != 0) && !hb;
     if (__ballot(ma || mb) && (ma || mb)) {
       const uint32_t r =
-          f.hash_lmax
+          f.hash_key
               ? regex_key2_pref_slow(wa[0], wa[1], wa[2], wa[3], wa[4], wa[5], wa[6], wa[7],
                                      wb[0], wb[1], wb[2], wb[3], wb[4], wb[5], wb[6], wb[7],
                                      f.regex, f.dfa, f.regex_live, f.nregex,
-                                     (lptr32)dfa_lds, f.hash_lmax, f.hash_stride)
+                                     (lptr32)dfa_lds, f.hash_lmax, f.hash_ws)
               : regex_key2_slow(wa[0], wa[1], wa[2], wa[3], wa[4], wa[5], wa[6], wa[7], wb[0],
                                 wb[1], wb[2], wb[3], wb[4], wb[5], wb[6], wb[7], f.regex, f.dfa,
                                 f.regex_live, f.nregex, (lptr16)dfa_lds, f.lds_states);
@@ -424,8 +424,12 @@ __device__ __forceinline__ uint32_t parked_share(uint32_t Q) {
 }
 
 // The address stage of one parked entry: packed key (sign in bit 255) and index.
+// Behind the address stage's window prefilter (hash_addr, its maps at `maps`
+// in LDS) the automata walk only where a window of the 56 symbols hits: a
+// wave walks when one of its lanes needs it.
 __device__ __forceinline__ void drain_one(const Fe10 &e, const DevFilterArgs &f,
-                                          const ScanBank *banks, const HitSink &sink) {
+                                          const ScanBank *banks, lptr32 maps,
+                                          const HitSink &sink) {
   uint32_t pk[8];
 #pragma unroll
   for (int l = 0; l < 8; ++l) pk[l] = e.v[l];
@@ -435,12 +439,14 @@ __device__ __forceinline__ void drain_one(const Fe10 &e, const DevFilterArgs &f,
   const uint32_t chk = onion_checksum(pk);
   w[8] = ((chk & 0xffu) << 24) | (((chk >> 8) & 0xffu) << 16) | (0x03u << 8);
   const uint32_t r = scan_address(w, banks, f.nbanks);
+  const bool walk = !f.hash_addr || window_hit<56, 9>(w, maps, f.hash_ws);
   const bool hit =
       r == 1 ||
       (r == 2 && pattern_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8], f.pats,
                               f.npat)) ||
-      (f.nregex && regex_address_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8],
-                                      f.regex, f.dfa, f.dfa_end, f.nregex));
+      (f.nregex && walk &&
+       regex_address_slow(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8], f.regex, f.dfa,
+                          f.dfa_end, f.nregex));
   Fe c;
 #pragma unroll
   for (int l = 0; l < 8; ++l) c.v[l] = pk[l];
@@ -451,9 +457,9 @@ __device__ __forceinline__ void drain_one(const Fe10 &e, const DevFilterArgs &f,
 template <uint32_t MODE>
 __device__ __forceinline__ void drain(const uint32_t *scratch, uint32_t top, uint32_t T,
                                       uint32_t g, uint32_t n, const DevFilterArgs &f,
-                                      const ScanBank *banks, const HitSink &sink) {
-  for (uint32_t j = 0; j < n; ++j)
-    drain_one(ld10(scratch, top - j, T, g), f, banks, sink);  // j is uniform over the active lanes
+                                      const ScanBank *banks, lptr32 maps, const HitSink &sink) {
+  for (uint32_t j = 0; j < n; ++j)  // j is uniform over the active lanes
+    drain_one(ld10(scratch, top - j, T, g), f, banks, maps, sink);
 }
 
 // The sign of x for parked entries that left it out (defer_sign), then the
@@ -501,7 +507,7 @@ __device__ __forceinline__ void drain_deferred(uint32_t *scratch, uint32_t top, 
                                                uint32_t T, uint32_t g, uint32_t n,
                                                uint64_t k_base, cptr table,
                                                const DevFilterArgs &f, const ScanBank *banks,
-                                               const HitSink &sink) {
+                                               lptr32 maps, const HitSink &sink) {
   const uint32_t K = half - 4;  // entries per inversion
   for (uint32_t j0 = 0; j0 < n; j0 += K) {
     const uint32_t cnt = n - j0 < K ? n - j0 : K;
@@ -520,7 +526,7 @@ __device__ __forceinline__ void drain_deferred(uint32_t *scratch, uint32_t top, 
       const Fe10 x = fe10_mul(num, fe10_mul(ld10(scratch, 4 + i, T, g), inv));
       inv = fe10_mul(den, inv);
       e.v[7] |= fe10_parity_product(x) << 31;
-      drain_one(e, f, banks, sink);
+      drain_one(e, f, banks, maps, sink);
     }
   }
 }
@@ -530,7 +536,7 @@ __device__ __forceinline__ void drain_deferred(uint32_t *scratch, uint32_t top, 
 __device__ __forceinline__ void address_stage(const uint32_t *scratch, uint32_t half, uint32_t T,
                                               uint32_t g, uint32_t n, const DevFilterArgs &f,
                                               const ScanBank *banks, const HitSink &sink) {
-  drain<kModePattern>(scratch, kPlanesPerPair * half, T, g, n, f, banks, sink);
+  drain<kModePattern>(scratch, kPlanesPerPair * half, T, g, n, f, banks, nullptr, sink);
 }
 
 // `rounds` windows of one chain (the method is described in og_kernels.hpp).
@@ -662,7 +668,8 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
           st10(scratch, 3, T, g, P.T);
         }
       } else {
-        drain<MODE>(scratch, top, T, g, share, f, banks, sink);
+        drain<MODE>(scratch, top, T, g, share, f, banks,
+                    f.hash_addr ? (lptr32)dfa_lds : (lptr32) nullptr, sink);
       }
     }
     P = pt10_add1(P, st);
@@ -679,7 +686,7 @@ __device__ __forceinline__ void load_filter(ScanBank *lds, uint16_t *dfa_lds,
                                             const DevFilterArgs &f, bool with_maps = false) {
   uint32_t *dst = reinterpret_cast<uint32_t *>(lds);
   for (uint32_t i = threadIdx.x; i < f.nbanks * kBankWords; i += blockDim.x) dst[i] = f.banks[i];
-  if (with_maps && f.hash_lmax) {
+  if (with_maps && f.hash) {
     uint32_t *d = reinterpret_cast<uint32_t *>(dfa_lds);
     for (uint32_t i = threadIdx.x; i < 2 * kHashSliceWords; i += blockDim.x) d[i] = f.hash[i];
   } else if (f.lds_states) {

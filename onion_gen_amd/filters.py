@@ -71,6 +71,16 @@ class FilterSet:
             raise OgError("the set runs no automaton bank or expression")
         return (bool(kh[0]), bool(nd[0])), (bool(kh[1]), bool(nd[1]))
 
+    def address_window(self, address):
+        """The address stage on a 35-byte address: (window, match) - whether
+        the address-stage window prefilter lets the automata walk (always,
+        without one), and whether they accept the address."""
+        window, match = ctypes.c_int(), ctypes.c_int()
+        if lib().og_filters_address_window_host(self.handle, bytes(address), ctypes.byref(window),
+                                                ctypes.byref(match)) != 0:
+            raise OgError("the set runs no automaton")
+        return bool(window.value), bool(match.value)
+
     def close(self):
         if self.handle:
             lib().og_filters_free(self.handle)

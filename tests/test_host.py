@@ -528,3 +528,47 @@ def test_window_prefilter_and_the_address_stage_give_the_whole_walks_result():
     assert checked == len(keys) - len(keys) % 2 and hits > 150
     # 1,000 words: about 5.8e-5 per probe, 22 probes, so about 0.13% of keys
     assert sent <= 20, sent
+
+
+def _address_with(rng, word, at):
+    """35 random address bytes with `word` at symbol `at` of the 56."""
+    alphabet = workload.ALPHABET
+    bits = [rng.getrandbits(1) for _ in range(280)]
+    for i, c in enumerate(word):
+        v = alphabet.index(c)
+        for b in range(5):
+            bits[5 * (at + i) + b] = (v >> (4 - b)) & 1
+    return int("".join(map(str, bits)), 2).to_bytes(35, "big")
+
+
+def test_address_window_prefilter_never_stops_a_matching_walk():
+    """1,000 five-symbol substrings send nearly every candidate to the address
+    stage, which then probes every 5-symbol window of the 56 symbols against
+    two maps and walks the automaton only where one hits.  On addresses with a
+    word planted at every placement (the checksum and version bytes included)
+    the walk always runs and matches; on random addresses it runs about as
+    rarely as the maps' occupancy predicts."""
+    rng = random.Random(41)
+    words = sorted({"".join(rng.choice(workload.ALPHABET) for _ in range(5)) for _ in range(1000)})
+    fs = filters.FilterSet.parse(["contains:" + w for w in words])
+    lines = fs.describe()
+    assert any(l.startswith("window prefilter in the address stage") for l in lines), lines
+    assert any(l.startswith("key stage skipped") for l in lines), lines
+    wordset = set(words)
+    for at in range(52):
+        for _ in range(6):
+            addr = _address_with(rng, rng.choice(words), at)
+            window, match = fs.address_window(addr)
+            assert match and window, (at, addr.hex())
+    walks = matches = 0
+    for _ in range(4000):
+        addr = bytes(rng.getrandbits(8) for _ in range(35))
+        window, match = fs.address_window(addr)
+        text = "".join(workload.ALPHABET[(int.from_bytes(addr, "big") >> (275 - 5 * j)) & 31]
+                       for j in range(56))
+        assert match == any(text[s:s + 5] in wordset for s in range(52))
+        assert window or not match
+        walks += window
+        matches += match
+    # about 0.23% of random addresses pass: 52 probes of about 4.4e-5 each
+    assert walks <= 40 and walks >= matches, (walks, matches)
