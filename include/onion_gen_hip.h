@@ -188,9 +188,11 @@ int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
  * multiplication is made of): `threads` lanes each run eight independent
  * chains of `iters` x 8 instructions of one kind - op 0 v_mad_u64_u32, 1
  * v_mul_lo_u32, 2 v_lshrrev_b64, 3 v_add_u32, 4 v_bitop3_b32, 5
- * v_alignbit_b32 (the last two make up the Keccak rounds).  Writes lane-instructions per
- * second and the shader clock the SIMDs ran at (s_memtime against the
- * constant-rate s_memrealtime, in Hz; 0 when unknown). */
+ * v_alignbit_b32 (the last two make up the Keccak rounds) - or, op 6, one
+ * dependent chain of v_mad_u64_u32 per lane at three waves per SIMD.  Writes
+ * lane-instructions per second and the shader clock the SIMDs ran at
+ * (s_memtime against the constant-rate s_memrealtime, in Hz; 0 when
+ * unknown). */
 int og_hip_issue_bench(int ordinal, int op, uint32_t threads, uint32_t iters, double *ops_per_s,
                        double *clock_hz, char *err, size_t errcap);
 

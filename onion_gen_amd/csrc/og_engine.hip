@@ -228,11 +228,15 @@ __global__ __launch_bounds__(256) void og_radix_kernel(uint32_t iters, uint32_t 
 // field multiplication issues 100 of), 1: v_mul_lo_u32, 2: v_lshrrev_b64,
 // 3: v_add_u32, 4: v_bitop3_b32 (a three-input XOR) and 5: v_alignbit_b32 (a
 // 32-bit half of a 64-bit rotation), the two instructions the Keccak rounds
-// are made of.
+// are made of; 6: v_mad_u64_u32 in one dependent chain per lane, at three
+// waves per SIMD (52 KiB of LDS per workgroup), the search loop's occupancy:
+// whether a column of dependent products stalls the SIMD.
 template <int OP>
 __global__ __launch_bounds__(256) void og_issue_kernel(uint32_t iters, uint64_t *out,
                                                        uint64_t *clocks) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ uint32_t pad[OP == 6 ? 13312 : 1];
+  if (OP == 6 && iters == 0xffffffffu) pad[threadIdx.x % (OP == 6 ? 13312 : 1)] = g;  // keeps it
   uint64_t acc[8];
   uint32_t a[8], b[8];
 #pragma unroll
@@ -256,6 +260,11 @@ __global__ __launch_bounds__(256) void og_issue_kernel(uint32_t iters, uint64_t 
           asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0"
                        : "+v"(acc[c]), "=s"(cc)
                        : "v"(a[c]), "v"(b[c]));
+        } else if (OP == 6) {
+          uint64_t cc;
+          asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0"
+                       : "+v"(acc[0]), "=s"(cc)
+                       : "v"(a[0]), "v"(b[0]));
         } else if (OP == 1) {
           uint32_t lo = (uint32_t)acc[c];
           asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(lo) : "v"(b[c]));
@@ -286,6 +295,7 @@ __global__ __launch_bounds__(256) void og_issue_kernel(uint32_t iters, uint64_t 
   uint64_t x = 0;
 #pragma unroll
   for (int c = 0; c < 8; ++c) x ^= acc[c];
+  if (OP == 6 && iters == 0xffffffffu) x ^= pad[(threadIdx.x + 1) % (OP == 6 ? 13312 : 1)];
   out[g] = x;
 }
 
@@ -921,8 +931,8 @@ int og_hip_radix_bench(int ordinal, int radix, uint32_t threads, uint32_t iters,
 
 int og_hip_issue_bench(int ordinal, int op, uint32_t threads, uint32_t iters, double *ops_per_s,
                        double *clock_hz, char *err, size_t errcap) {
-  if (op < 0 || op > 5 || threads == 0 || threads % 256 || threads > (1u << 22) || iters == 0) {
-    put_err(err, errcap, "op must be 0..5, threads a multiple of 256 up to 2^22, iters positive");
+  if (op < 0 || op > 6 || threads == 0 || threads % 256 || threads > (1u << 22) || iters == 0) {
+    put_err(err, errcap, "op must be 0..6, threads a multiple of 256 up to 2^22, iters positive");
     return -1;
   }
   OG_TRY(hipSetDevice(ordinal));
@@ -942,6 +952,7 @@ int og_hip_issue_bench(int ordinal, int op, uint32_t threads, uint32_t iters, do
       case 2: hipLaunchKernelGGL(og_issue_kernel<2>, dim3(blocks), dim3(256), 0, 0, it, d, clk); break;
       case 4: hipLaunchKernelGGL(og_issue_kernel<4>, dim3(blocks), dim3(256), 0, 0, it, d, clk); break;
       case 5: hipLaunchKernelGGL(og_issue_kernel<5>, dim3(blocks), dim3(256), 0, 0, it, d, clk); break;
+      case 6: hipLaunchKernelGGL(og_issue_kernel<6>, dim3(blocks), dim3(256), 0, 0, it, d, clk); break;
       default: hipLaunchKernelGGL(og_issue_kernel<3>, dim3(blocks), dim3(256), 0, 0, it, d, clk);
     }
   };

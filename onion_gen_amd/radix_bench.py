@@ -19,7 +19,8 @@ P = 2 ** 255 - 19
 RADIXES = {5: "5 x 51 in 64-bit limbs (fiat-crypto form)", 8: "8 x 32 saturated, row-wise",
            9: "9 x 29, carry-free columns", 10: "10 x 25.5 (ref10 shape)"}
 ISSUE_OPS = {0: "v_mad_u64_u32", 1: "v_mul_lo_u32", 2: "v_lshrrev_b64", 3: "v_add_u32",
-             4: "v_bitop3_b32", 5: "v_alignbit_b32"}
+             4: "v_bitop3_b32", 5: "v_alignbit_b32",
+             6: "v_mad_u64_u32, one dependent chain, 3 waves per SIMD"}
 
 
 def radix_input(lane, which, limb):
