@@ -352,19 +352,27 @@ OG_HD bool window_hit(const uint32_t *w, TL maps, uint32_t ws) {
   }
 }
 
-// The state after symbols j0..48 from state s: the walk with a run-time first
-// symbol (the short walk of the prefiltered dictionaries).
+// The states after symbols j0..48 for the two keys of a pair, from the
+// states given (the short walk of the prefiltered dictionaries, with a
+// run-time first symbol), their steps interleaved so that each hides the
+// other's table latency.
 template <class TP>
-OG_HD uint32_t walk_tail(TP t, uint32_t s, const uint32_t w[8], uint32_t j0) {
+OG_HD void walk_tail2(TP t, uint32_t &sa, uint32_t &sb, const uint32_t wa[8],
+                      const uint32_t wb[8], uint32_t j0) {
+  uint32_t fa[8], fb[8];
+  fresh_words<8>(wa, fa);
+  fresh_words<8>(wb, fb);
   switch (j0) {  // the common word lengths 8..12 take the unrolled steps
-    case 42: return walk_from<42, 49>(t, s, w);
-    case 41: return walk_from<41, 49>(t, s, w);
-    case 40: return walk_from<40, 49>(t, s, w);
-    case 39: return walk_from<39, 49>(t, s, w);
-    case 38: return walk_from<38, 49>(t, s, w);
+    case 42: walk2_from<42, 49>(t, sa, sb, fa, fb); return;
+    case 41: walk2_from<41, 49>(t, sa, sb, fa, fb); return;
+    case 40: walk2_from<40, 49>(t, sa, sb, fa, fb); return;
+    case 39: walk2_from<39, 49>(t, sa, sb, fa, fb); return;
+    case 38: walk2_from<38, 49>(t, sa, sb, fa, fb); return;
     default:
-      for (uint32_t j = j0; j < 49; ++j) s = t[(s << 5) | sym_at(w, (int)j)];
-      return s;
+      for (uint32_t j = j0; j < 49; ++j) {
+        sa = t[(sa << 5) | sym_at(fa, (int)j)];
+        sb = t[(sb << 5) | sym_at(fb, (int)j)];
+      }
   }
 }
 
@@ -430,8 +438,8 @@ OG_HD uint32_t regex_key2_prefiltered(const uint32_t wa[8], const uint32_t wb[8]
   for (uint32_t r = 0; r < n && (ra != 2 || rb != 2); ++r) {
     const T16 t = wave_uniform(dfa + recs[3 * r]);
     const T32 lv = live + recs[3 * r + 2];
-    const uint32_t start = recs[3 * r + 1];
-    const uint32_t sa = walk_tail(t, start, wa, j0), sb = walk_tail(t, start, wb, j0);
+    uint32_t sa = recs[3 * r + 1], sb = sa;
+    walk_tail2(t, sa, sb, wa, wb, j0);
     if (sa == 1) ra = 2;
     else if (sa != 0 && !ra && regex_open(t, lv, sa, wa)) ra = 1;
     if (sb == 1) rb = 2;
