@@ -48,14 +48,16 @@ def measured_mad_peak():
 
 def field_muls_per_candidate(half, with_x=False):
     """Field multiplications (squarings included) the kernel performs per
-    candidate: 3 per pair forward (C = T dq and two prefix products), 8 per
-    pair backward (two inverses peeled, two products A and B, two y), the
-    centre (4), one inversion (254 sq + 11 mul) and the stride addition (7)
-    per window of 2*half+1 candidates; +2 per candidate when the sign of x is
-    needed.  Priced at 64 32x32-bit products each, the radix-independent count
-    of an 8 x 32 schoolbook."""
-    window = 11 * half + 4 + 265 + 7
-    return window / (2 * half + 1) + (2 if with_x else 0)
+    candidate: 2 per pair forward (the shared denominator E = ZZ + TT(-dq^2)
+    and the prefix product), 8 per pair backward (two inverses peeled, the
+    double products alpha and beta, two each, and the two y), and per window
+    of 2*half+1 candidates the six products of the centre's coordinates, the
+    centre (4), one inversion (254 sq + 11 mul) and the stride addition (7);
+    +3 per candidate when the sign of x is needed (gamma and delta, two
+    each, and the two x per pair).  Priced at 64 32x32-bit products each, the
+    radix-independent count of an 8 x 32 schoolbook."""
+    window = 10 * half + 6 + 4 + 265 + 7
+    return window / (2 * half + 1) + (3 if with_x else 0)
 
 
 def measured_traffic(threads, half, rounds):
@@ -435,8 +437,8 @@ def main():
     ap.add_argument("--steps", type=int, default=0, help="0: cover 2^34 candidates")
     ap.add_argument("--warmup", type=int, default=2)
     # The shape the autotune walk (og_hip_engine_tune) picks for this workload on
-    # MI355X: 2^20 chains, 1024 pairs per window, one window per launch (128 GB
-    # of scratch planes: three 40-byte elements per pair).
+    # MI355X: 2^20 chains, 1024 pairs per window, one window per launch (86 GB
+    # of scratch planes: two 40-byte elements per pair).
     ap.add_argument("--threads", type=int, default=1 << 20)
     ap.add_argument("--half", type=int, default=1024)
     ap.add_argument("--rounds", type=int, default=1)

@@ -164,7 +164,10 @@ void og_hip_engine_destroy(og_engine *e);
  * a product's output as the search kernels do (17: x mod 2^255, split into
  * limbs without reduction, times one; 18: x times y) and op 19 is bit 0 of the
  * canonical (x mod 2^255, split the same way) times y, as the search takes the
- * sign of x.  Inputs are n raw 32-byte little-endian
+ * sign of x.  Ops 21 and 22 are the products by a table constant the search
+ * loop makes, the constant taken from the element leading each group of 64
+ * (element i & ~63, A = its a, B = its b): 21 is x * B, 22 is x * B + y * A
+ * (one double product).  Inputs are n raw 32-byte little-endian
  * words (any value below 2^256); outputs are canonical encodings.  The _host
  * variant runs the same code on the CPU. */
 int og_hip_field_op(int ordinal, int op, const uint8_t *a, const uint8_t *b, uint8_t *out,

@@ -63,27 +63,50 @@ OG_HD Fe10 fe10_carry64(uint64_t h[10]) {
   return r;
 }
 
+// The scaled limb of a wave-uniform constant g that multiplies limb i of f in
+// column k of a product: odd-by-odd terms land half a bit high (x2), and terms
+// that wrap past limb 9 re-enter as 19 (2^255 = 19).  38 * 2^25 < 2^31, so
+// every scaled limb fits 32 bits.
+OG_HD uint32_t fe10_cterm(const Fe10 &g, int k, int i) {
+  const int j = k - i;
+  const int jj = j < 0 ? j + 10 : j;
+  const uint32_t scale = ((i & 1) && (jj & 1) ? 2u : 1u) * (j < 0 ? 19u : 1u);
+  return scale * g.v[jj];
+}
+
+#ifndef __HIP_DEVICE_COMPILE__
 // The product by a tight, wave-uniform g (a table constant): the same
 // partial products as fe10_mul, with the doubling of odd-by-odd terms moved
 // onto g, so that every scaled form of g (2g, 19g, 38g) is scalar work and
 // v_mad_u64_u32 reads it from an SGPR; f needs no preparation at all.
-// 38 * 2^25 < 2^31, so every scaled limb fits 32 bits.
 OG_HD Fe10 fe10_mulc(const Fe10 &f, const Fe10 &g) {
   uint64_t h[10];
 #pragma unroll
   for (int k = 0; k < 10; ++k) {
     uint64_t acc = 0;
 #pragma unroll
-    for (int i = 0; i < 10; ++i) {
-      const int j = k - i;
-      const int jj = j < 0 ? j + 10 : j;
-      const uint32_t scale = ((i & 1) && (jj & 1) ? 2u : 1u) * (j < 0 ? 19u : 1u);
-      acc = mad64(f.v[i], scale * g.v[jj], acc);
-    }
+    for (int i = 0; i < 10; ++i) acc = mad64(f.v[i], fe10_cterm(g, k, i), acc);
     h[k] = acc;
   }
   return fe10_carry64(h);
 }
+
+// f1*g1 + f2*g2 for tight f1, f2 and wave-uniform tight g1, g2, with one
+// carry pass: each column sums twenty products (below 2^61.6).
+OG_HD Fe10 fe10_mulc2(const Fe10 &f1, const Fe10 &g1, const Fe10 &f2, const Fe10 &g2) {
+  uint64_t h[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) acc = mad64(f1.v[i], fe10_cterm(g1, k, i), acc);
+#pragma unroll
+    for (int i = 0; i < 10; ++i) acc = mad64(f2.v[i], fe10_cterm(g2, k, i), acc);
+    h[k] = acc;
+  }
+  return fe10_carry64(h);
+}
+#endif
 
 #ifdef __HIP_DEVICE_COMPILE__
 // The device product runs column by column, each column's ten partial
@@ -111,6 +134,95 @@ __device__ __forceinline__ uint64_t fe10_column(const uint32_t a[10], const uint
       : "v"(a[0]), "v"(b[0]), "v"(a[1]), "v"(b[1]), "v"(a[2]), "v"(b[2]), "v"(a[3]), "v"(b[3]),
         "v"(a[4]), "v"(b[4]), "v"(a[5]), "v"(b[5]), "v"(a[6]), "v"(b[6]), "v"(a[7]), "v"(b[7]),
         "v"(a[8]), "v"(b[8]), "v"(a[9]), "v"(b[9]), "v"(c));
+  return r;
+}
+
+// The same column with the b operands in SGPRs: the scaled limbs of a
+// wave-uniform constant (v_mad_u64_u32 reads one SGPR per instruction).
+__device__ __forceinline__ uint64_t fe10_column_s(const uint32_t a[10], const uint32_t b[10],
+                                                  uint64_t c) {
+  uint64_t r, cc;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %22\n\t"
+      "v_mad_u64_u32 %0, %1, %4, %5, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %6, %7, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %8, %9, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %10, %11, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %12, %13, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %14, %15, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %16, %17, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %18, %19, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %20, %21, %0"
+      : "=&v"(r), "=&s"(cc)
+      : "v"(a[0]), "s"(b[0]), "v"(a[1]), "s"(b[1]), "v"(a[2]), "s"(b[2]), "v"(a[3]), "s"(b[3]),
+        "v"(a[4]), "s"(b[4]), "v"(a[5]), "s"(b[5]), "v"(a[6]), "s"(b[6]), "v"(a[7]), "s"(b[7]),
+        "v"(a[8]), "s"(b[8]), "v"(a[9]), "s"(b[9]), "v"(c));
+  return r;
+}
+
+// The column carries of a product summed column by column: limb k keeps the
+// low w bits of the column sum, the rest rides into column k + 1, and the
+// carry out of limb 9 re-enters limb 0 as 19 and moves on by one limb.
+__device__ __forceinline__ void fe10_take(Fe10 &r, int k, uint64_t &acc) {
+  const int w = fe10_width(k);
+  r.v[k] = (uint32_t)acc & ((1u << w) - 1);
+  acc >>= w;
+}
+
+__device__ __forceinline__ void fe10_fold(Fe10 &r, uint64_t acc) {
+  const uint64_t h0 = (uint64_t)r.v[0] + 19 * acc;
+  r.v[1] += (uint32_t)(h0 >> 26);
+  r.v[0] = (uint32_t)h0 & ((1u << 26) - 1);
+}
+
+// The product by a tight, wave-uniform g (a table constant): the same
+// partial products as fe10_mul, with the doubling of odd-by-odd terms moved
+// onto g, so that every scaled form of g (2g, 19g, 38g) is scalar work and
+// v_mad_u64_u32 reads it from an SGPR; f needs no preparation at all.  The
+// columns are summed on the incoming carry, as in fe10_mul.
+__device__ __forceinline__ Fe10 fe10_mulc(const Fe10 &f, const Fe10 &g) {
+  Fe10 r;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    uint32_t b[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) b[i] = fe10_cterm(g, k, i);
+    if (k == 0) {
+#pragma unroll
+      for (int i = 0; i < 10; ++i) acc = mad64(f.v[i], b[i], acc);
+    } else {
+      acc = fe10_column_s(f.v, b, acc);
+    }
+    fe10_take(r, k, acc);
+  }
+  fe10_fold(r, acc);
+  return r;
+}
+
+// f1*g1 + f2*g2 for tight f1, f2 and wave-uniform tight g1, g2, with one
+// carry pass: each column sums twenty products (below 2^61.6).
+__device__ __forceinline__ Fe10 fe10_mulc2(const Fe10 &f1, const Fe10 &g1, const Fe10 &f2,
+                                           const Fe10 &g2) {
+  Fe10 r;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    uint32_t b1[10], b2[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      b1[i] = fe10_cterm(g1, k, i);
+      b2[i] = fe10_cterm(g2, k, i);
+    }
+    if (k == 0) {
+#pragma unroll
+      for (int i = 0; i < 10; ++i) acc = mad64(f1.v[i], b1[i], acc);
+    } else {
+      acc = fe10_column_s(f1.v, b1, acc);
+    }
+    acc = fe10_column_s(f2.v, b2, acc);
+    fe10_take(r, k, acc);
+  }
+  fe10_fold(r, acc);
   return r;
 }
 

@@ -54,7 +54,7 @@ struct SlotDesc {
 
 struct BatchArgs {
   const uint32_t *table;
-  uint32_t *scratch;    // [scratch_planes(half)][10][slots*chains]
+  uint32_t *scratch;    // [scratch_planes(half, true)][10][slots*chains]
   const SlotDesc *slots;
   uint32_t nslots, chains, half, rounds;  // chains: per unit slot
   uint32_t *counter;    // [0]: ring records; [1 + slot]: every hit of the search led there
@@ -312,8 +312,8 @@ std::string batch_tuning_key(int ordinal) {
 // the scratch planes, the offset table, the ring, and the chain state.
 uint64_t batch_bytes(uint32_t slots, uint32_t chains, uint32_t half, uint32_t ring) {
   const uint64_t T = (uint64_t)slots * chains;
-  return scratch_planes(half) * 40 * T + (uint64_t)half * 120 + (uint64_t)ring * sizeof(HitRec) +
-         160 * T;
+  return scratch_planes(half, true) * 40 * T + (uint64_t)half * kTableWords * 4 +
+         (uint64_t)ring * sizeof(HitRec) + 160 * T;
 }
 
 }  // namespace
@@ -377,8 +377,8 @@ int og_batch_create(int ordinal, const og_batch_opts *opts, og_batch **out, char
   OG_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   OG_TRY(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
   const std::vector<uint32_t> table = b->curve.offset_table10(o.half);
-  OG_TRY(hipMalloc(&b->d_table, std::max<size_t>(table.size(), 30) * 4));
-  OG_TRY(hipMalloc(&b->d_scratch, scratch_planes(o.half) * 10 * T * 4));
+  OG_TRY(hipMalloc(&b->d_table, std::max<size_t>(table.size(), kTableWords) * 4));
+  OG_TRY(hipMalloc(&b->d_scratch, scratch_planes(o.half, true) * 10 * T * 4));
   OG_TRY(hipMalloc(&b->d_slots, o.slots * sizeof(SlotDesc)));
   OG_TRY(hipHostMalloc(&b->h_slots, o.slots * sizeof(SlotDesc), hipHostMallocDefault));
   OG_TRY(hipMalloc(&b->d_counter, (1 + (size_t)o.slots) * 4));

@@ -149,10 +149,39 @@ __host__ __device__ inline void field_op_one(int op, const uint32_t *a, const ui
   for (int l = 0; l < 8; ++l) out[l] = r.v[l];
 }
 
+// Ops 21 and 22: products by a wave-uniform constant as the search loop takes
+// them from its table, the constants being the operands of the element that
+// leads the element's group of 64 (i & ~63), A = its a and B = its b:
+// op 21 is x * B, op 22 is x * B + y * A (one double product).
+__host__ __device__ inline void field_op_const(int op, const uint32_t *a, const uint32_t *b,
+                                               const uint32_t *A, const uint32_t *B,
+                                               uint32_t *out) {
+  Fe x, y, ca, cb;
+  for (int l = 0; l < 8; ++l) {
+    x.v[l] = a[l];
+    y.v[l] = b[l];
+    ca.v[l] = A[l];
+    cb.v[l] = B[l];
+  }
+  const Fe10 x10 = fe10_from(x), y10 = fe10_from(y), a10 = fe10_from(ca), b10 = fe10_from(cb);
+  const Fe r = fe10_canon(op == 21 ? fe10_mulc(x10, b10) : fe10_mulc2(x10, b10, y10, a10));
+  for (int l = 0; l < 8; ++l) out[l] = r.v[l];
+}
+
 __global__ __launch_bounds__(256) void og_field_kernel(int op, const uint32_t *a, const uint32_t *b,
                                                        uint32_t *out, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (op == 21 || op == 22) {
+    // every wave's first lane is element i & ~63, and it is active
+    uint32_t A[8], B[8];
+    for (int l = 0; l < 8; ++l) {
+      A[l] = __builtin_amdgcn_readfirstlane(a[(size_t)i * 8 + l]);
+      B[l] = __builtin_amdgcn_readfirstlane(b[(size_t)i * 8 + l]);
+    }
+    field_op_const(op, a + (size_t)i * 8, b + (size_t)i * 8, A, B, out + (size_t)i * 8);
+    return;
+  }
   field_op_one(op, a + (size_t)i * 8, b + (size_t)i * 8, out + (size_t)i * 8);
 }
 
@@ -456,13 +485,15 @@ uint32_t survivor_room(uint64_t batch, const FilterSet *fs, const DevFilterHost 
 }
 
 // Device bytes an engine of this shape holds: chain state (4 coordinates of
-// 10 limbs), the offset table, the scratch planes, the survivor ring, the
-// parked counts of a pattern launch and the filter tables.
-uint64_t shape_bytes(uint32_t threads, uint32_t half, uint32_t cap, bool pattern,
+// 10 limbs), the offset table, the scratch planes (the pattern kernels' with
+// their parking planes), the survivor ring, the parked counts of a one-round
+// pattern launch and the filter tables.
+uint64_t shape_bytes(uint32_t threads, uint32_t half, uint32_t rounds, uint32_t cap, bool pattern,
                      uint64_t filter_bytes) {
-  return (uint64_t)160 * threads + (uint64_t)120 * std::max<uint32_t>(half, 1) +
-         (uint64_t)scratch_planes(half) * 40 * threads + 4 + (uint64_t)cap * sizeof(HitRec) +
-         (pattern ? (uint64_t)4 * threads : 0) + filter_bytes;
+  return (uint64_t)160 * threads + (uint64_t)kTableWords * 4 * std::max<uint32_t>(half, 1) +
+         (uint64_t)scratch_planes(half, pattern) * 40 * threads + 4 +
+         (uint64_t)cap * sizeof(HitRec) + (pattern && rounds == 1 ? (uint64_t)4 * threads : 0) +
+         filter_bytes;
 }
 
 // The memory an engine may take when its shape is chosen for it: the
@@ -537,11 +568,12 @@ int create_engine(int ordinal, const uint8_t start_xyzt[128], const og_engine_op
   put_fe10(a.stride + 20, stride.t2d);
 
   // ---- buffers
-  const size_t scratch_words = scratch_planes(o.half) * 10 * o.threads;
+  const size_t scratch_words =
+      scratch_planes(o.half, (e->filt.mode & kModePattern) != 0) * 10 * o.threads;
   e->cap = o.hit_cap ? o.hit_cap
                      : survivor_room((uint64_t)o.threads * S * o.rounds, fs, e->filt);
   OG_TRY(hipMalloc(&e->d_state, (size_t)40 * o.threads * 4));
-  OG_TRY(hipMalloc(&e->d_table, std::max<size_t>(table.size(), 30) * 4));
+  OG_TRY(hipMalloc(&e->d_table, std::max<size_t>(table.size(), kTableWords) * 4));
   OG_TRY(hipMalloc(&e->d_scratch, std::max<size_t>(scratch_words, 1) * 4));
   OG_TRY(hipMalloc(&e->d_counter, 4));
   OG_TRY(hipMalloc(&e->d_hits, (size_t)e->cap * sizeof(HitRec)));
@@ -573,7 +605,7 @@ int create_engine(int ordinal, const uint8_t start_xyzt[128], const og_engine_op
   }
   a.hits = e->d_hits;
   a.cap = e->cap;
-  e->bytes = shape_bytes(o.threads, o.half, e->cap, e->d_parked != nullptr,
+  e->bytes = shape_bytes(o.threads, o.half, o.rounds, e->cap, (e->filt.mode & kModePattern) != 0,
                          e->filt.device_bytes());
 
   // ---- per-chain start points on the device
@@ -678,7 +710,7 @@ int tune(int ordinal, const uint8_t start_xyzt[128], const FilterSet *fs, const 
   const bool pattern = fs && (!fs->general.empty() || !fs->regexes.empty());
   for (size_t i = 0; i < walk.size(); ++i) {
     const og_engine_opts &o = walk[i];
-    sizes[i] = shape_bytes(o.threads, o.half, o.hit_cap, pattern && o.rounds == 1, fbytes);
+    sizes[i] = shape_bytes(o.threads, o.half, o.rounds, o.hit_cap, pattern, fbytes);
     if (budget && sizes[i] > budget) {
       last = "every shape exceeds the memory budget of " + std::to_string(budget) + " bytes";
       continue;
@@ -748,8 +780,9 @@ int create_tuned(int ordinal, const uint8_t start_xyzt[128], const og_engine_opt
     // a remembered shape above this engine's budget is measured again, as one
     // that no longer builds is (src/gpu/mod.rs:371-381)
     const bool pattern = fs && (!fs->general.empty() || !fs->regexes.empty());
-    const uint64_t est = shape_bytes(o.threads, o.half, o.hit_cap ? o.hit_cap : (1u << 20),
-                                     pattern && o.rounds == 1, filter_bytes(fs, bitmap_words));
+    const uint64_t est = shape_bytes(o.threads, o.half, o.rounds,
+                                     o.hit_cap ? o.hit_cap : (1u << 20), pattern,
+                                     filter_bytes(fs, bitmap_words));
     if ((!budget || est <= budget) &&
         create_engine(ordinal, start_xyzt, &o, fs, bitmap, bitmap_words, bitmap_bits, out, err,
                       errcap) == 0)
@@ -988,8 +1021,8 @@ int og_hip_issue_bench(int ordinal, int op, uint32_t threads, uint32_t iters, do
   return 0;
 }
 
-// ops 0..5, 10..20, as include/onion_gen_hip.h lists them
-static bool field_op_known(int op) { return (op >= 0 && op <= 5) || (op >= 10 && op <= 20); }
+// ops 0..5, 10..22, as include/onion_gen_hip.h lists them
+static bool field_op_known(int op) { return (op >= 0 && op <= 5) || (op >= 10 && op <= 22); }
 
 int og_field_op_host(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, size_t n) {
   if (!field_op_known(op) || !a || !b || !out) return -1;
@@ -997,7 +1030,14 @@ int og_field_op_host(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, s
     uint32_t x[8], y[8], r[8];
     std::memcpy(x, a + 32 * i, 32);
     std::memcpy(y, b + 32 * i, 32);
-    field_op_one(op, x, y, r);
+    if (op == 21 || op == 22) {
+      uint32_t A[8], B[8];
+      std::memcpy(A, a + 32 * (i & ~(size_t)63), 32);
+      std::memcpy(B, b + 32 * (i & ~(size_t)63), 32);
+      field_op_const(op, x, y, A, B, r);
+    } else {
+      field_op_one(op, x, y, r);
+    }
     std::memcpy(out + 32 * i, r, 32);
   }
   return 0;

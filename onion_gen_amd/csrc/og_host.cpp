@@ -144,13 +144,14 @@ std::vector<uint32_t> HostCurve::offset_table(uint32_t half) const {
 
 std::vector<uint32_t> HostCurve::offset_table10(uint32_t half) const {
   const std::vector<uint32_t> t8 = offset_table(half);
-  std::vector<uint32_t> out((size_t)half * 30);
-  for (uint32_t m = 0; m < half; ++m)
-    for (int c = 0; c < 3; ++c) {
-      Fe x;
-      for (int l = 0; l < 8; ++l) x.v[l] = t8[(size_t)m * 24 + 8 * c + l];
-      put_fe10(&out[(size_t)m * 30 + 10 * c], x);
-    }
+  std::vector<uint32_t> out((size_t)half * kTableWords);
+  for (uint32_t m = 0; m < half; ++m) {
+    Fe c[6];
+    for (int i = 0; i < 3; ++i)
+      for (int l = 0; l < 8; ++l) c[i].v[l] = t8[(size_t)m * 24 + 8 * i + l];
+    offset_terms(c);
+    for (int i = 0; i < 6; ++i) put_fe10(&out[(size_t)m * kTableWords + 10 * i], c[i]);
+  }
   return out;
 }
 

@@ -2,32 +2,42 @@
 //
 // Chain c of a search owns window w of 2H+1 consecutive candidates per round:
 //     k = w*(2H+1) + H + j,  j = -H .. H,  centre P_w = P0 + (w(2H+1) + H) * 8G.
-// For an affine offset Q_m = m*8G (m = 1..H) with precomputed dq = d*xq*yq, the
-// unified Edwards addition gives the y coordinates of both neighbours of a
-// projective P = (X:Y:Z:T):
-//     y(P + Q) = (Y*yq + X*xq) / (Z - T*dq)
-//     y(P - Q) = (Y*yq - X*xq) / (Z + T*dq)
-// (affine y3 = (y1y2 + x1x2)/(1 - d x1x2y1y2), multiplied through by Z).  Three
-// multiplications serve two candidates, the denominators never vanish on the
-// curve (the formula is complete) and P is never normalised.  This is the
-// reference's pair idea (src/pairs.rs:1-19, cuda-kernel/src/lib.rs:118-121)
-// with the complete formula in place of its dedicated one.
+// For an affine offset Q_m = m*8G (m = 1..H) with dq = d*xq*yq, the unified
+// Edwards addition gives both neighbours of a projective P = (X:Y:Z:T):
+//     y(P + Q) = (Y*yq + X*xq) / (Z - T*dq),   x(P + Q) = (X*yq + Y*xq) / (Z + T*dq)
+//     y(P - Q) = (Y*yq - X*xq) / (Z + T*dq),   x(P - Q) = (X*yq - Y*xq) / (Z - T*dq)
+// (affine y3 = (y1y2 + x1x2)/(1 - d x1x2y1y2), multiplied through by Z).  The
+// denominators never vanish on the curve (the formula is complete) and P is
+// never normalised.  This is the reference's pair idea (src/pairs.rs:1-19,
+// cuda-kernel/src/lib.rs:118-121) with the complete formula in place of its
+// dedicated one.
 //
-// Montgomery's trick runs over the denominators in pair order.  The forward
-// pass computes C = T*dq, dp = Z - C, dm = Z + C and the running product, and
-// stores three field elements per pair: the prefix D0 before the pair, D1 =
-// D0*dp and C.  After one inversion of the whole product (times Z, for the
-// centre) the backward pass peels the inverses off:
-//     1/dm = inv*D1,  inv *= dm,  1/dp = inv*D0,  inv *= dp
-// and forms y(P - Q) = (A - B)/dm, y(P + Q) = (A + B)/dp with A = Y*yq,
-// B = X*xq.  Per pair: 3 multiplications forward, 8 backward; a candidate
-// costs 60 bytes of scratch written once and read once.  Keeping C costs one
-// element per pair and saves recomputing it (the reference recomputes its
-// denominators, src/pairs.rs:182-207, which suits a CPU's cache; here HBM has
-// the room and the bandwidth, and the multiply issue rate is the bound).
-// Because 1/dm and 1/dp are explicit, the x coordinate of a candidate costs
-// two more multiplications only where a filter needs the sign of x:
-//     x(P + Q) = (X*yq + Y*xq)/dm,   x(P - Q) = (X*yq - Y*xq)/dp.
+// The pair shares one denominator, their product
+//     E = (Z - T*dq)(Z + T*dq) = Z^2 + T^2 * (-dq^2),
+// and each numerator is taken over E by the other denominator.  Multiplied
+// out, both numerators are sums of the window's four products YZ, TX, XZ, TY
+// (fixed for the window) times table constants:
+//     y(P +- Q) = (alpha +- beta) / E,  alpha = YZ*yq + TX*(xq dq),  beta = XZ*xq + TY*(yq dq)
+//     x(P +- Q) = (gamma +- delta) / E, gamma = XZ*yq - TY*(xq dq),  delta = YZ*xq - TX*(yq dq)
+// so the table holds, per offset, xq, yq, dq, xq*dq, yq*dq and -dq^2.
+//
+// Montgomery's trick runs over the E of each pair.  The forward pass forms
+// E = ZZ + TT*(-dq^2) and the running product, and stores two field elements
+// per pair: the prefix D before the pair and E.  After one inversion of the
+// whole product (times Z, for the centre) the backward pass peels the
+// inverses off (1/E = inv*D, inv *= E) and forms the two y from alpha and beta,
+// each a double product by constants with one carry pass.  Per pair: one
+// product by a constant and one multiplication forward; two multiplications,
+// two double products by constants and two multiplications backward: 1,000
+// partial products for two candidates, against 1,100 for the three
+// constant products (T*dq, Y*yq, X*xq) and eight multiplications of the
+// separate-denominator form.  A candidate costs 40 bytes of scratch written
+// once and read once.  Keeping E costs one element per pair and saves
+// recomputing it (the reference recomputes its denominators,
+// src/pairs.rs:182-207, which suits a CPU's cache; here HBM has the room and
+// the bandwidth, and the multiply issue rate is the bound).  The x coordinate
+// of a candidate costs gamma and delta and one more multiplication, only where
+// a filter needs the sign of x.
 #pragma once
 #include "fe10.hpp"
 #include "ge.hpp"
@@ -50,10 +60,30 @@ enum : uint32_t {
   kModePattern = 4,  // general forms and expressions: the bank automaton of og_scan.hpp
 };
 
-// Scratch planes (field elements of 10 limbs) per pair: D0, D1, C; a window
-// takes one more, where the first deferred address-stage candidate parks.
+// Scratch planes (field elements of 10 limbs): the chain's two per pair (D
+// and E, planes 2m and 2m + 1), and for the pattern kernels the parking
+// planes, which count down from plane 3H (kPlanesPerPair * H) to at least
+// plane H, so that parked entries never reach a plane the backward pass has
+// yet to read and planes 0 .. H - 1 stay free for the address kernel's
+// derivation of deferred signs.  The key forms never park: 2H planes.
+constexpr uint32_t kChainPlanes = 2;
 constexpr uint32_t kPlanesPerPair = 3;
-inline size_t scratch_planes(uint32_t half) { return (size_t)kPlanesPerPair * half + 1; }
+inline size_t scratch_planes(uint32_t half, bool pattern) {
+  return pattern ? (size_t)kPlanesPerPair * half + 1 : (size_t)kChainPlanes * half;
+}
+
+// The offset table: per offset m (Q = (m + 1) * 8G), six field elements of
+// ten limbs.
+constexpr uint32_t kTableWords = 60;
+constexpr uint32_t kTabX = 0, kTabY = 10, kTabDq = 20, kTabXdq = 30, kTabYdq = 40, kTabNdq2 = 50;
+
+// The derived constants of an offset: c[0..2] = xq, yq, dq (canonical) in,
+// c[3..5] = xq*dq, yq*dq, -dq^2 (canonical) out.
+OG_HD void offset_terms(Fe c[6]) {
+  c[3] = fe_canon(fe_mul(c[0], c[2]));
+  c[4] = fe_canon(fe_mul(c[1], c[2]));
+  c[5] = fe_canon(fe_sub(fe_zero(), fe_mul(c[2], c[2])));
+}
 
 constexpr int kMaxBuckets = 16;
 constexpr int kMaxPatterns = 64;  // symbol-wise records; the banks bound the set
@@ -130,8 +160,8 @@ struct DevFilterArgs {
 
 struct SearchArgs {
   uint32_t *state;        // [4 coords][10 limbs][threads]
-  const uint32_t *table;  // [half][xq, yq, dq][10]
-  uint32_t *scratch;      // [scratch_planes(half)][10][threads]
+  const uint32_t *table;  // [half][kTableWords]
+  uint32_t *scratch;      // [scratch_planes(half, pattern)][10][threads]
   uint32_t threads, half, rounds;
   uint64_t k_base;        // first candidate of this launch
   uint32_t stride[30];    // (y+x, y-x, 2dxy) of threads*(2H+1)*8G, Z = 1

@@ -40,7 +40,7 @@ namespace {
 struct TableArgs {
   uint32_t eight[32];  // 8G cached: y+x, y-x, Z, 2dT (8 x 32)
   uint32_t d[8];
-  uint32_t *table;     // [half][x, y, d x y][10 limbs]
+  uint32_t *table;     // [half][kTableWords]: x, y, d x y and the derived terms
   uint32_t half;
   uint32_t *counter;
 };
@@ -66,10 +66,11 @@ __global__ __launch_bounds__(256) void og_noalloc_table_kernel(TableArgs a) {
   const Fe zinv = fe_invert(q.Z);
   const Fe x = fe_canon(fe_mul(q.X, zinv)), y = fe_canon(fe_mul(q.Y, zinv));
   const Fe dxy = fe_canon(fe_mul(d, fe_mul(x, y)));
-  const Fe c[3] = {x, y, dxy};
-  for (int i = 0; i < 3; ++i) {
+  Fe c[6] = {x, y, dxy};
+  offset_terms(c);
+  for (int i = 0; i < 6; ++i) {
     const Fe10 r = fe10_split(c[i]);
-    for (int l = 0; l < 10; ++l) a.table[(size_t)m * 30 + 10 * i + l] = r.v[l];
+    for (int l = 0; l < 10; ++l) a.table[(size_t)m * kTableWords + 10 * i + l] = r.v[l];
   }
 }
 
@@ -156,8 +157,9 @@ const HostCurve &curve() {
 extern "C" {
 
 size_t og_hip_search_noalloc_bytes(uint32_t threads, uint32_t half) {
-  return 256 + align256((size_t)kNoallocRing * sizeof(HitRec)) + align256((size_t)half * 120) +
-         align256((size_t)160 * threads) + scratch_planes(half) * 40 * (size_t)threads;
+  return 256 + align256((size_t)kNoallocRing * sizeof(HitRec)) +
+         align256((size_t)half * kTableWords * 4) + align256((size_t)160 * threads) +
+         scratch_planes(half, false) * 40 * (size_t)threads;
 }
 
 size_t og_hip_search_noalloc_max(void) { return kNoallocBytes; }
@@ -215,7 +217,7 @@ int og_hip_search_noalloc(int ordinal, const uint8_t start_xyzt[128], uint64_t k
   HitRec *ring = reinterpret_cast<HitRec *>(base + 256);
   size_t off = 256 + align256((size_t)kNoallocRing * sizeof(HitRec));
   uint32_t *table = reinterpret_cast<uint32_t *>(base + off);
-  off += align256((size_t)half * 120);
+  off += align256((size_t)half * kTableWords * 4);
   uint32_t *state = reinterpret_cast<uint32_t *>(base + off);
   off += align256((size_t)160 * threads);
   uint32_t *planes = reinterpret_cast<uint32_t *>(base + off);

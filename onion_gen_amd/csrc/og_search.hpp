@@ -385,7 +385,7 @@ __device__ __forceinline__ void key_stage_pair(const Fe &ca, const uint32_t wa[8
 // placement possible computes the sign of x (bit 0 of its canonical form) and
 // parks the candidate - packed key with the sign in bit 255, and its index -
 // in a scratch plane the backward pass has already consumed, counting down
-// from the plane past the window's 3H.  After the backward pass the parked
+// from plane 3H.  After the backward pass the parked
 // candidates are drained (SHA3-256, the exact automaton pass) with little
 // else live, so the Keccak state never competes with the chain's.
 //
@@ -396,9 +396,9 @@ __device__ __forceinline__ void key_stage_pair(const Fe &ca, const uint32_t wa[8
 // steps and scatter every store over 64 planes), and the drain gives every
 // lane the same share: lane i takes entries i, i+64, ..., in planes 3H, 3H-1,
 // ... of its own column.  After the centre and pairs H-1..m the wave has at
-// most 64(2(H-m)+1) entries, in at most 2(H-m)+1 planes, and
-// 3H - 2(H-m) >= 3m, so an entry never reaches a plane the backward pass has
-// yet to read.  `mask` is the wave's ballot of `want`.  `sign` is the sign of
+// most 64(2(H-m)+1) entries, in at most 2(H-m)+1 planes, the lowest of them
+// 3H - 2(H-m) = H + 2m, above the planes 0 .. 2m-1 of the pairs the backward
+// pass has yet to read, and never below plane H.  `mask` is the wave's ballot of `want`.  `sign` is the sign of
 // x, or 0 when the address kernel is to derive it (DevFilterArgs::defer_sign).
 __device__ __forceinline__ void park(uint32_t *scratch, uint32_t top, uint32_t T, uint32_t g,
                                      uint32_t &Q, uint64_t mask, bool want, const Fe &c,
@@ -495,8 +495,8 @@ __device__ __forceinline__ void sign_terms(const Fe10 &e, const uint32_t *scratc
   // table read in bounds
   uint32_t m = (uint32_t)(minus ? kc - k : k - kc) - 1u;
   m = m < half ? m : half - 1u;
-  const Fe10 xq = ld10_const(table + m * 30), yq = ld10_const(table + m * 30 + 10);
-  const Fe10 dq = ld10_const(table + m * 30 + 20);
+  const cptr q = table + m * kTableWords;
+  const Fe10 xq = ld10_const(q + kTabX), yq = ld10_const(q + kTabY), dq = ld10_const(q + kTabDq);
   const Fe10 C = fe10_mul(P.T, dq);
   const Fe10 XY = fe10_mul(P.X, yq), YX = fe10_mul(P.Y, xq);
   num = minus ? fe10_sub(XY, YX) : fe10_add(XY, YX);
@@ -560,17 +560,18 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
   for (uint32_t r = 0; r < rounds; ++r) {
     const uint64_t w = (uint64_t)r * chains + local;
     const uint64_t kc = k_base + w * S + H;
+    // the window's products of the centre's coordinates
+    const Fe10 YZ = fe10_mul(P.Y, P.Z), XZ = fe10_mul(P.X, P.Z);
+    const Fe10 TX = fe10_mul(P.T, P.X), TY = fe10_mul(P.T, P.Y);
     Fe10 D = fe10_one();
-    for (uint32_t m = 0; m < H; ++m) {
-      const Fe10 dq = ld10_const(table + m * 30 + 20);
-      const Fe10 C = fe10_mulc(P.T, dq);
-      const Fe10 dp = fe10_sub(P.Z, C);
-      const Fe10 dm = fe10_add(P.Z, C);
-      st10(scratch, kPlanesPerPair * m, T, g, D);
-      D = fe10_mul(dp, D);
-      st10(scratch, kPlanesPerPair * m + 1, T, g, D);
-      st10(scratch, kPlanesPerPair * m + 2, T, g, C);
-      D = fe10_mul(dm, D);
+    {
+      const Fe10 ZZ = fe10_sq(P.Z), TT = fe10_sq(P.T);
+      for (uint32_t m = 0; m < H; ++m) {
+        const Fe10 E = fe10_add(ZZ, fe10_mulc(TT, ld10_const(table + m * kTableWords + kTabNdq2)));
+        st10(scratch, kChainPlanes * m, T, g, D);
+        st10(scratch, kChainPlanes * m + 1, T, g, E);
+        D = fe10_mul(E, D);
+      }
     }
     // inv = 1/(D Z): the centre takes 1/Z = inv D, and inv Z = 1/D starts
     // the backward pass
@@ -598,32 +599,25 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
       }
     }
     inv = fe10_mul(inv, P.Z);
-    // Software-pipelined by one pair: the three elements of pair m-1 are
-    // loaded as soon as pair m's are consumed, so their latency hides behind
-    // the four multiplications that form pair m's candidates.
-    Fe10 D0 = ld10(scratch, kPlanesPerPair * (H - 1), T, g);
-    Fe10 D1 = ld10(scratch, kPlanesPerPair * (H - 1) + 1, T, g);
-    Fe10 C = ld10(scratch, kPlanesPerPair * (H - 1) + 2, T, g);
+    // Software-pipelined by one pair: the two elements of pair m-1 are loaded
+    // as soon as pair m's are consumed, so their latency hides behind the
+    // products that form pair m's candidates.
+    Fe10 D0 = ld10(scratch, kChainPlanes * (H - 1), T, g);
+    Fe10 E0 = ld10(scratch, kChainPlanes * (H - 1) + 1, T, g);
     for (uint32_t m = H; m-- > 0;) {
-      const Fe10 dp = fe10_sub(P.Z, C);
-      const Fe10 dm = fe10_add(P.Z, C);
-      // entry: inv = 1/(D0 dp dm)
-      const Fe10 idm = fe10_mul(D1, inv);
-      inv = fe10_mul(dm, inv);
-      const Fe10 idp = fe10_mul(D0, inv);
-      inv = fe10_mul(dp, inv);
+      // entry: inv = 1/(D0 E0)
+      const Fe10 iE = fe10_mul(D0, inv);
+      inv = fe10_mul(E0, inv);
       {
-        const size_t next = kPlanesPerPair * (m ? m - 1 : 0);  // pair 0 reloads itself
+        const size_t next = kChainPlanes * (m ? m - 1 : 0);  // pair 0 reloads itself
         D0 = ld10(scratch, next, T, g);
-        D1 = ld10(scratch, next + 1, T, g);
-        C = ld10(scratch, next + 2, T, g);
+        E0 = ld10(scratch, next + 1, T, g);
       }
-      const Fe10 xq = ld10_const(table + m * 30);
-      const Fe10 yq = ld10_const(table + m * 30 + 10);
-      const Fe10 A = fe10_mulc(P.Y, yq);
-      const Fe10 B = fe10_mulc(P.X, xq);
-      const Fe10 ym = fe10_mul(fe10_sub(A, B), idm);  // y(P - Q)
-      const Fe10 yp = fe10_mul(fe10_add(A, B), idp);  // y(P + Q)
+      const cptr q = table + m * kTableWords;
+      const Fe10 al = fe10_mulc2(YZ, ld10_const(q + kTabY), TX, ld10_const(q + kTabXdq));
+      const Fe10 be = fe10_mulc2(XZ, ld10_const(q + kTabX), TY, ld10_const(q + kTabYdq));
+      const Fe10 yp = fe10_mul(fe10_add(al, be), iE);  // y(P + Q)
+      const Fe10 ym = fe10_mul(fe10_sub(al, be), iE);  // y(P - Q)
       const uint64_t km = kc - (m + 1), kp = kc + (m + 1);
       if (kPat) {
         const Fe cm = fe10_canon_product(ym), cp = fe10_canon_product(yp);
@@ -637,15 +631,17 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
           if (mm) park(scratch, top, T, g, parked, mm, nm, cm, 0u, km);
           if (mp) park(scratch, top, T, g, parked, mp, np, cp, 0u, kp);
         } else if (mm | mp) {
-          // x(P - Q) has the denominator dp, x(P + Q) the denominator dm
-          const Fe10 XY = fe10_mul(P.X, yq);
-          const Fe10 YX = fe10_mul(P.Y, xq);
+          // x(P +- Q) = (gamma +- delta) / E
+          const Fe10 ga = fe10_carry(fe10_sub(fe10_mulc(XZ, ld10_const(q + kTabY)),
+                                              fe10_mulc(TY, ld10_const(q + kTabXdq))));
+          const Fe10 de = fe10_carry(fe10_sub(fe10_mulc(YZ, ld10_const(q + kTabX)),
+                                              fe10_mulc(TX, ld10_const(q + kTabYdq))));
           if (mm)
             park(scratch, top, T, g, parked, mm, nm, cm,
-                 fe10_parity_product(fe10_mul(fe10_sub(XY, YX), idp)), km);
+                 fe10_parity_product(fe10_mul(fe10_sub(ga, de), iE)), km);
           if (mp)
             park(scratch, top, T, g, parked, mp, np, cp,
-                 fe10_parity_product(fe10_mul(fe10_add(XY, YX), idm)), kp);
+                 fe10_parity_product(fe10_mul(fe10_add(ga, de), iE)), kp);
         }
         report(hm, km, cm, sink);
         report(hp, kp, cp, sink);

@@ -95,6 +95,16 @@ def test_field_kernels_match_big_integers():
     assert field_op(18, a, b) == [x * y % P for x, y in zip(a, b)]
     assert field_op(19, a, b) == [(x % 2 ** 255) * y % P & 1 for x, y in zip(a, b)]
     assert field_op(19, above, [1] * len(above)) == [(x % 2 ** 255) % P & 1 for x in above]
+    # products by a constant as the search loop makes them from its table (the
+    # constant is the operand of element i & ~63, uniform over a wave): one
+    # product, and a double product with one carry pass
+    ra = [rng.getrandbits(256) for _ in range(4096)] + a[:1000]
+    rb = [rng.getrandbits(256) for _ in range(4096)] + b[:1000]
+    lead_a = [ra[i & ~63] for i in range(len(ra))]
+    lead_b = [rb[i & ~63] for i in range(len(rb))]
+    assert field_op(21, ra, rb) == [x * c % P for x, c in zip(ra, lead_b)]
+    assert field_op(22, ra, rb) == [(x * c + y * e) % P
+                                    for x, y, c, e in zip(ra, rb, lead_b, lead_a)]
 
 
 def test_device_checksum_matches_sha3():

@@ -104,6 +104,13 @@ def test_field_arithmetic_both_forms_on_the_host():
     assert host_field_op(19, a, b) == [(x % 2 ** 255) * y % P & 1 for x, y in zip(a, b)]
     # products that land on [p, 2^255) before reduction: x times one
     assert host_field_op(19, above, [1] * len(above)) == [(x % 2 ** 255) % P & 1 for x in above]
+    # products by a constant (the operand of element i & ~63): one product and a
+    # double product with one carry pass, as the search loop takes them
+    lead_a = [a[i & ~63] for i in range(len(a))]
+    lead_b = [b[i & ~63] for i in range(len(b))]
+    assert host_field_op(21, a, b) == [x * c % P for x, c in zip(a, lead_b)]
+    assert host_field_op(22, a, b) == [(x * c + y * e) % P
+                                       for x, y, c, e in zip(a, b, lead_b, lead_a)]
     # the largest limbs a tight value can carry: all-ones limbs
     big = [sum(((1 << (26 if i % 2 == 0 else 25)) - 1) << [0, 26, 51, 77, 102, 128, 153, 179, 204, 230][i]
                for i in range(10))]
