@@ -46,17 +46,21 @@ def measured_mad_peak():
     return max((issue_rate(0, 1 << 20, 2048) for _ in range(3)), key=lambda rc: rc[0])
 
 
-def field_muls_per_candidate(half, with_x=False):
+def field_muls_per_candidate(half, key_form=True, with_x=False):
     """Field multiplications (squarings included) the kernel performs per
     candidate: 2 per pair forward (the shared denominator E = ZZ + TT(-dq^2)
     and the prefix product), 8 per pair backward (two inverses peeled, the
     double products alpha and beta, two each, and the two y), and per window
     of 2*half+1 candidates the six products of the centre's coordinates, the
-    centre (4), one inversion (254 sq + 11 mul) and the stride addition (7);
-    +3 per candidate when the sign of x is needed (gamma and delta, two
-    each, and the two x per pair).  Priced at 64 32x32-bit products each, the
-    radix-independent count of an 8 x 32 schoolbook."""
-    window = 10 * half + 6 + 4 + 265 + 7
+    centre (4), one inversion (254 sq + 11 mul) and the stride addition (7).
+    The key forms make only four columns of each y (40 of its 100 products,
+    the probe word of fe10_mul_low32), so their pair counts 8.8; the whole y
+    of the rare lanes whose probe passes is left out.  +3 per candidate when
+    the sign of x is needed (gamma and delta, two each, and the two x per
+    pair).  Priced at 64 32x32-bit products each, the radix-independent count
+    of an 8 x 32 schoolbook."""
+    pair = 8.8 if key_form else 10.0
+    window = pair * half + 6 + 4 + 265 + 7
     return window / (2 * half + 1) + (3 if with_x else 0)
 
 
@@ -521,7 +525,8 @@ def main():
                    fs.match_address(key.address_bytes(r.public_key)) >= 0)
     clamping = sum(1 for r in recs if r.status == key.HIT_CLAMPING)
 
-    F = field_muls_per_candidate(args.half, with_x=text.startswith("suffix:"))
+    F = field_muls_per_candidate(args.half, key_form=eng.kernels() == 1,
+                                 with_x=text.startswith("suffix:"))
     kernel_rate = examined / (kernel_ms * 1e-3)
     achieved = kernel_rate * F * 64  # 32x32->64 partial products per second
     out = {

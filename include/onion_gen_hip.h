@@ -121,6 +121,12 @@ void og_hip_engine_shape(const og_engine *e, uint32_t *threads, uint32_t *half, 
  * in the reference, whose CPU path forms it per candidate (src/batch.rs). */
 int og_hip_engine_sign_stage(const og_engine *e);
 
+/* The search kernels an engine runs: 1 the key forms (prefix compare, bitmap
+ * and exact table: no parking planes, the four-column probe of y), 2 the
+ * pattern forms (the automata, the checksum stage); -1 for a null engine.  No
+ * counterpart in the reference. */
+int og_hip_engine_kernels(const og_engine *e);
+
 /* The device ordinal an engine runs on. */
 int og_hip_engine_device(const og_engine *e);
 
@@ -167,7 +173,9 @@ void og_hip_engine_destroy(og_engine *e);
  * sign of x.  Ops 21 and 22 are the products by a table constant the search
  * loop makes, the constant taken from the element leading each group of 64
  * (element i & ~63, A = its a, B = its b): 21 is x * B, 22 is x * B + y * A
- * (one double product).  Inputs are n raw 32-byte little-endian
+ * (one double product).  Op 23 is the key forms' probe word for the product of
+ * op 16's operands: the canonical product's low 32 bits are the output plus 0,
+ * 19 or 38.  Inputs are n raw 32-byte little-endian
  * words (any value below 2^256); outputs are canonical encodings.  The _host
  * variant runs the same code on the CPU. */
 int og_hip_field_op(int ordinal, int op, const uint8_t *a, const uint8_t *b, uint8_t *out,

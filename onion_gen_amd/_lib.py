@@ -98,6 +98,7 @@ SIGNATURES.update({
     "og_hip_engine_shape": (None, [_c.c_void_p, _c.POINTER(_c.c_uint32),
                                    _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint32)]),
     "og_hip_engine_sign_stage": (_c.c_int, [_c.c_void_p]),
+    "og_hip_engine_kernels": (_c.c_int, [_c.c_void_p]),
     "og_tuning_path": (_c.c_int, [_c.c_char_p, _c.c_size_t]),
     "og_tuning_get": (_c.c_int, [_c.c_char_p, _c.POINTER(TuningShape)]),
     "og_tuning_put": (_c.c_int, [_c.c_char_p, _c.POINTER(TuningShape)]),

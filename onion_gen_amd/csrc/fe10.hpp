@@ -292,6 +292,41 @@ OG_HD Fe10 fe10_mul(const Fe10 &f, const Fe10 &g) {
 }
 #endif
 
+// The low 32 bits of the canonical form of f*g, up to two steps of 19, from
+// four of the product's ten columns: the canonical low word is L, L + 19 or
+// L + 38 for the L returned.  fe10_mul's result is V mod 2^255 + 19c for the
+// column sums' integer V = sum S_k 2^a_k and c = floor(V / 2^255), so its low
+// word is S_0 + S_1 2^26 + 19c mod 2^32 (the other columns start at bit 51),
+// and the canonical form subtracts p at most once (+19 in the low word).  c is
+// taken from columns 8 and 9 alone: columns 0..7 add below 2^244, less than
+// 2^-11 of a unit of c, so the true c is c or c + 1.  Same operand bounds as
+// fe10_mul.  A filter on the first four key bytes probes all three words and
+// forms the whole product only where one passes: 40 of the 100 products.
+OG_HD uint32_t fe10_mul_low32(const Fe10 &f, const Fe10 &g) {
+  uint32_t g19[10], f2[10];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    g19[i] = 19u * g.v[i];
+    f2[i] = (i & 1) ? 2u * f.v[i] : f.v[i];
+  }
+  uint64_t S[4];
+  const int ks[4] = {0, 1, 8, 9};
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int k = ks[c];
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      const int j = k - i;
+      const int jj = j < 0 ? j + 10 : j;
+      acc = mad64(((i & 1) && (jj & 1)) ? f2[i] : f.v[i], j < 0 ? g19[jj] : g.v[j], acc);
+    }
+    S[c] = acc;
+  }
+  const uint32_t carry = (uint32_t)((S[3] + (S[2] >> 26)) >> 25);
+  return (uint32_t)S[0] + ((uint32_t)S[1] << 26) + 19u * carry;
+}
+
 OG_HD Fe10 fe10_sq(const Fe10 &f) { return fe10_mul(f, f); }
 
 OG_HD Fe10 fe10_add(const Fe10 &a, const Fe10 &b) {

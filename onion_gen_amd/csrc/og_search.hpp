@@ -74,8 +74,8 @@ __device__ __forceinline__ bool bitmap_bit(const DevFilterArgs &f, uint32_t head
 }
 
 // The cheap test every candidate gets: on the low 32 bits of the key (its
-// first four bytes), for both the value and the value minus p.  Never false
-// for a candidate the exact test accepts.
+// first four bytes), for the three words fe10_mul_low32 leaves open (L, L + 19
+// and L + 38).  Never false for a candidate the exact test accepts.
 // A MODE with both key bits set leaves the key form to the filter at run time
 // (f.mode, wave-uniform): one instantiation serves every key form, which keeps
 // the slot engine's kernels to one search loop each.
@@ -100,20 +100,23 @@ __device__ __forceinline__ bool coarse_bit(const uint32_t *coarse, uint32_t cbit
 }
 
 template <uint32_t MODE>
-__device__ __forceinline__ bool maybe_key(const Fe10 &y, const DevFilterArgs &f,
+__device__ __forceinline__ bool maybe_key(uint32_t lo, const DevFilterArgs &f,
                                           const uint32_t *coarse = nullptr) {
-  const uint32_t lo = fe10_low32(y), lo19 = lo + 19u;
+  const uint32_t lo19 = lo + 19u, lo38 = lo + 38u;
   if (key_is_prefix<MODE>(f))
-    return (((lo ^ f.want[0]) & f.mask[0]) == 0) | (((lo19 ^ f.want[0]) & f.mask[0]) == 0);
+    return (((lo ^ f.want[0]) & f.mask[0]) == 0) | (((lo19 ^ f.want[0]) & f.mask[0]) == 0) |
+           (((lo38 ^ f.want[0]) & f.mask[0]) == 0);
   if (key_is_bitmap<MODE>(f)) {
-    const uint32_t h1 = bswap32(lo), h2 = bswap32(lo19);
-    if (!coarse) return bitmap_bit(f, h1) | bitmap_bit(f, h2);
+    const uint32_t h1 = bswap32(lo), h2 = bswap32(lo19), h3 = bswap32(lo38);
+    if (!coarse) return bitmap_bit(f, h1) | bitmap_bit(f, h2) | bitmap_bit(f, h3);
     const bool c1 = coarse_bit(coarse, f.coarse_bits, h1);
     const bool c2 = coarse_bit(coarse, f.coarse_bits, h2);
-    if (f.coarse_bits == f.bits) return c1 | c2;
+    const bool c3 = coarse_bit(coarse, f.coarse_bits, h3);
+    if (f.coarse_bits == f.bits) return c1 | c2 | c3;
     bool r = false;
     if (c1) r = bitmap_bit(f, h1);
     if (c2) r |= bitmap_bit(f, h2);
+    if (c3) r |= bitmap_bit(f, h3);
     return r;
   }
   return false;
@@ -275,18 +278,19 @@ __device__ __forceinline__ void report(bool hit, uint64_t k, const Fe &c, const 
   append(hit && rank < sink.budget, sink.slot, k, c, sink.counter, sink.hits, sink.cap);
 }
 
-// Key forms only: the low-32-bit probe, and the exact test on the lanes the
-// wave ballot names.
+// Key forms only, for the candidate whose y is a*b: the low-32-bit probe from
+// four columns of the product, and the whole product and the exact test on
+// the lanes the wave ballot names.
 template <uint32_t MODE>
-__device__ __forceinline__ void test_key_only(const Fe10 &y, uint64_t k, const DevFilterArgs &f,
-                                              const HitSink &sink,
+__device__ __forceinline__ void test_key_only(const Fe10 &a, const Fe10 &b, uint64_t k,
+                                              const DevFilterArgs &f, const HitSink &sink,
                                               const uint32_t *coarse = nullptr) {
-  const bool maybe = maybe_key<MODE>(y, f, coarse);
+  const bool maybe = maybe_key<MODE>(fe10_mul_low32(a, b), f, coarse);
   if (__ballot(maybe) == 0) return;  // the common case: no lane needs more
   bool hit = false;
   Fe c;
   if (maybe) {
-    c = fe10_canon_product(y);
+    c = fe10_canon_product(fe10_mul(a, b));
     hit = exact_key<MODE>(c, f);
     if (f.key_proxy && hit) {
       // an expression's prefix: its automata settle the key's 49 symbols
@@ -580,9 +584,8 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
     uint32_t parked = 0;  // the wave's parked entries (uniform)
     {
       const Fe10 iz = fe10_mul(inv, D);
-      const Fe10 y = fe10_mul(P.Y, iz);
       if (kPat) {
-        const Fe c = fe10_canon_product(y);
+        const Fe c = fe10_canon_product(fe10_mul(P.Y, iz));
         uint32_t kw[8];
         key_words(c, kw);
         bool hit, need;
@@ -595,7 +598,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         }
         report(hit, kc, c, sink);
       } else {
-        test_key_only<MODE>(y, kc, f, sink, coarse);
+        test_key_only<MODE>(P.Y, iz, kc, f, sink, coarse);
       }
     }
     inv = fe10_mul(inv, P.Z);
@@ -616,11 +619,12 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
       const cptr q = table + m * kTableWords;
       const Fe10 al = fe10_mulc2(YZ, ld10_const(q + kTabY), TX, ld10_const(q + kTabXdq));
       const Fe10 be = fe10_mulc2(XZ, ld10_const(q + kTabX), TY, ld10_const(q + kTabYdq));
-      const Fe10 yp = fe10_mul(fe10_add(al, be), iE);  // y(P + Q)
-      const Fe10 ym = fe10_mul(fe10_sub(al, be), iE);  // y(P - Q)
+      // y(P + Q) = up iE, y(P - Q) = um iE
+      const Fe10 up = fe10_add(al, be), um = fe10_sub(al, be);
       const uint64_t km = kc - (m + 1), kp = kc + (m + 1);
       if (kPat) {
-        const Fe cm = fe10_canon_product(ym), cp = fe10_canon_product(yp);
+        const Fe cm = fe10_canon_product(fe10_mul(um, iE));
+        const Fe cp = fe10_canon_product(fe10_mul(up, iE));
         uint32_t wm[8], wp[8];
         key_words(cm, wm);
         key_words(cp, wp);
@@ -646,8 +650,8 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         report(hm, km, cm, sink);
         report(hp, kp, cp, sink);
       } else {
-        test_key_only<MODE>(ym, km, f, sink, coarse);
-        test_key_only<MODE>(yp, kp, f, sink, coarse);
+        test_key_only<MODE>(um, iE, km, f, sink, coarse);
+        test_key_only<MODE>(up, iE, kp, f, sink, coarse);
       }
     }
     if (kPat) {

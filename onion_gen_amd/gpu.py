@@ -127,6 +127,11 @@ class Engine:
         address kernel (og_hip_engine_sign_stage)."""
         return lib().og_hip_engine_sign_stage(self._h)
 
+    def kernels(self):
+        """The search kernels this engine runs: 1 the key forms, 2 the pattern
+        forms (og_hip_engine_kernels)."""
+        return lib().og_hip_engine_kernels(self._h)
+
     def restart(self, start_xyzt):
         """Starts over from another start point (og_hip_engine_restart)."""
         e = err_buf()

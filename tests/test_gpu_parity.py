@@ -105,6 +105,11 @@ def test_field_kernels_match_big_integers():
     assert field_op(21, ra, rb) == [x * c % P for x, c in zip(ra, lead_b)]
     assert field_op(22, ra, rb) == [(x * c + y * e) % P
                                     for x, y, c, e in zip(ra, rb, lead_b, lead_a)]
+    # the key forms' probe word: the canonical low 32 bits of (2x - y)(x - y)
+    # minus it is 0, 19 or 38
+    got = field_op(23, a, b)
+    assert all((((2 * x - y) * (x - y) % P) - w) % 2 ** 32 in (0, 19, 38)
+               for x, y, w in zip(a, b, got))
 
 
 def test_device_checksum_matches_sha3():

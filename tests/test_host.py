@@ -111,6 +111,11 @@ def test_field_arithmetic_both_forms_on_the_host():
     assert host_field_op(21, a, b) == [x * c % P for x, c in zip(a, lead_b)]
     assert host_field_op(22, a, b) == [(x * c + y * e) % P
                                        for x, y, c, e in zip(a, b, lead_b, lead_a)]
+    # the key forms' probe word: the canonical low 32 bits of (2x - y)(x - y)
+    # minus it is 0, 19 or 38
+    got = host_field_op(23, a, b)
+    assert all((((2 * x - y) * (x - y) % P) - w) % 2 ** 32 in (0, 19, 38)
+               for x, y, w in zip(a, b, got))
     # the largest limbs a tight value can carry: all-ones limbs
     big = [sum(((1 << (26 if i % 2 == 0 else 25)) - 1) << [0, 26, 51, 77, 102, 128, 153, 179, 204, 230][i]
                for i in range(10))]
