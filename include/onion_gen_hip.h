@@ -174,8 +174,11 @@ void og_hip_engine_destroy(og_engine *e);
  * loop makes, the constant taken from the element leading each group of 64
  * (element i & ~63, A = its a, B = its b): 21 is x * B, 22 is x * B + y * A
  * (one double product).  Op 23 is the key forms' probe word for the product of
- * op 16's operands: the canonical product's low 32 bits are the output plus 0,
- * 19 or 38.  Inputs are n raw 32-byte little-endian
+ * op 16's operands, from four of its columns, in output word 0, and in word 1
+ * whether it is exact: the canonical product's low 32 bits are word 0 when
+ * word 1 is 1, else word 0 plus 0, 19 or 38.  Op 24 is op 19 taken as the
+ * search takes it, from four columns where they settle it.  Inputs are n raw
+ * 32-byte little-endian
  * words (any value below 2^256); outputs are canonical encodings.  The _host
  * variant runs the same code on the CPU. */
 int og_hip_field_op(int ordinal, int op, const uint8_t *a, const uint8_t *b, uint8_t *out,

@@ -137,6 +137,27 @@ __device__ __forceinline__ uint64_t fe10_column(const uint32_t a[10], const uint
   return r;
 }
 
+// A column summed from zero (the addend of the first product is the inline
+// constant 0).
+__device__ __forceinline__ uint64_t fe10_column0(const uint32_t a[10], const uint32_t b[10]) {
+  uint64_t r, cc;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, 0\n\t"
+      "v_mad_u64_u32 %0, %1, %4, %5, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %6, %7, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %8, %9, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %10, %11, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %12, %13, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %14, %15, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %16, %17, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %18, %19, %0\n\t"
+      "v_mad_u64_u32 %0, %1, %20, %21, %0"
+      : "=&v"(r), "=&s"(cc)
+      : "v"(a[0]), "v"(b[0]), "v"(a[1]), "v"(b[1]), "v"(a[2]), "v"(b[2]), "v"(a[3]), "v"(b[3]),
+        "v"(a[4]), "v"(b[4]), "v"(a[5]), "v"(b[5]), "v"(a[6]), "v"(b[6]), "v"(a[7]), "v"(b[7]),
+        "v"(a[8]), "v"(b[8]), "v"(a[9]), "v"(b[9]));
+  return r;
+}
+
 // The same column with the b operands in SGPRs: the scaled limbs of a
 // wave-uniform constant (v_mad_u64_u32 reads one SGPR per instruction).
 __device__ __forceinline__ uint64_t fe10_column_s(const uint32_t a[10], const uint32_t b[10],
@@ -292,17 +313,24 @@ OG_HD Fe10 fe10_mul(const Fe10 &f, const Fe10 &g) {
 }
 #endif
 
-// The low 32 bits of the canonical form of f*g, up to two steps of 19, from
-// four of the product's ten columns: the canonical low word is L, L + 19 or
-// L + 38 for the L returned.  fe10_mul's result is V mod 2^255 + 19c for the
-// column sums' integer V = sum S_k 2^a_k and c = floor(V / 2^255), so its low
-// word is S_0 + S_1 2^26 + 19c mod 2^32 (the other columns start at bit 51),
-// and the canonical form subtracts p at most once (+19 in the low word).  c is
-// taken from columns 8 and 9 alone: columns 0..7 add below 2^244, less than
-// 2^-11 of a unit of c, so the true c is c or c + 1.  Same operand bounds as
-// fe10_mul.  A filter on the first four key bytes probes all three words and
-// forms the whole product only where one passes: 40 of the 100 products.
-OG_HD uint32_t fe10_mul_low32(const Fe10 &f, const Fe10 &g) {
+// The low 32 bits of the canonical form of f*g from four of the product's
+// ten columns, exact unless `exact` comes back false.  fe10_mul's result is
+// V mod 2^255 + 19c for the column sums' integer V = sum S_k 2^a_k and
+// c = floor(V / 2^255); columns 2..9 start at bit 51 or above, so its low
+// word is S_0 + S_1 2^26 + 19c mod 2^32.  c is taken from columns 8 and 9
+// alone, as floor(t / 2^51) for t = S_9 2^26 + S_8: columns 0..7 add below
+// 2^244, under 2^40 in units of S_8's weight 2^204, so c can only be one more
+// when t mod 2^51 is within 2^40 of 2^51.  The canonical form differs from
+// the result (by -p, +19 in the low word) only when the result is at least
+// p, which needs V mod 2^255 within 2^37 of 2^255, the same condition.  So
+// the returned word is the canonical low word whenever t mod 2^51 is below
+// 2^51 - 2^41, that is unless bits 15..24 of S_9 + floor(S_8 / 2^26) are all
+// ones (about one product in 1,024); otherwise the canonical low word is the
+// returned one plus 0, 19 or 38.  Same operand bounds as fe10_mul.  The key
+// forms probe this word and the search loop takes the sign of x from its bit
+// 0, forming the whole product only where `exact` is false or a probe
+// passes: 40 of its 100 products.
+OG_HD uint32_t fe10_mul_low32(const Fe10 &f, const Fe10 &g, bool &exact) {
   uint32_t g19[10], f2[10];
 #pragma unroll
   for (int i = 0; i < 10; ++i) {
@@ -314,18 +342,31 @@ OG_HD uint32_t fe10_mul_low32(const Fe10 &f, const Fe10 &g) {
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const int k = ks[c];
-    uint64_t acc = 0;
+    uint32_t a[10], b[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
       const int j = k - i;
       const int jj = j < 0 ? j + 10 : j;
-      acc = mad64(((i & 1) && (jj & 1)) ? f2[i] : f.v[i], j < 0 ? g19[jj] : g.v[j], acc);
+      a[i] = ((i & 1) && (jj & 1)) ? f2[i] : f.v[i];
+      b[i] = j < 0 ? g19[jj] : g.v[j];
     }
+#ifdef __HIP_DEVICE_COMPILE__
+    // ten v_mad_u64_u32 from zero: of columns 0 and 1 only the low word is
+    // used, which the compiler would otherwise form with 32-bit multiplies
+    // and adds, nineteen instructions
+    S[c] = fe10_column0(a, b);
+#else
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) acc = mad64(a[i], b[i], acc);
     S[c] = acc;
+#endif
   }
-  const uint32_t carry = (uint32_t)((S[3] + (S[2] >> 26)) >> 25);
-  return (uint32_t)S[0] + ((uint32_t)S[1] << 26) + 19u * carry;
+  const uint64_t acc9 = S[3] + (S[2] >> 26);
+  exact = ((uint32_t)acc9 & (0x3ffu << 15)) != (0x3ffu << 15);
+  return (uint32_t)S[0] + ((uint32_t)S[1] << 26) + 19u * (uint32_t)(acc9 >> 25);
 }
+
 
 OG_HD Fe10 fe10_sq(const Fe10 &f) { return fe10_mul(f, f); }
 
@@ -408,6 +449,16 @@ OG_HD uint32_t fe10_parity_product(const Fe10 &t) {
 #pragma unroll
   for (int i = 1; i < 10; ++i) q = (t.v[i] + q) >> fe10_width(i);
   return (t.v[0] ^ q) & 1u;
+}
+
+// Bit 0 of the canonical form of f*g (the sign of x in a packed key): from
+// four columns where that settles it, else from the whole product.
+OG_HD uint32_t fe10_parity_mul(const Fe10 &f, const Fe10 &g) {
+  bool exact;
+  const uint32_t lo = fe10_mul_low32(f, g, exact);
+  uint32_t sign = lo & 1u;
+  if (!exact) sign = fe10_parity_product(fe10_mul(f, g));
+  return sign;
 }
 
 // The limbs of x mod 2^255 as they stand, with no reduction mod p: a value in

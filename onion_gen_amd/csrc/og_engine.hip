@@ -143,11 +143,18 @@ __host__ __device__ inline void field_op_one(int op, const uint32_t *a, const ui
       // (x mod 2^255 split as it stands, as for op 17)
       case 19: r = fe_zero(); r.v[0] = fe10_parity_product(fe10_mul(fe10_split(x), y10)); break;
       // the key forms' probe word from four columns of (2x - y)(x - y), the
-      // loosest operands of op 16
-      case 23:
+      // loosest operands of op 16, and whether it is exact (word 1)
+      case 23: {
+        bool exact;
         r = fe_zero();
-        r.v[0] = fe10_mul_low32(fe10_sub(fe10_add(x10, x10), y10), fe10_sub(x10, y10));
+        r.v[0] = fe10_mul_low32(fe10_sub(fe10_add(x10, x10), y10), fe10_sub(x10, y10), exact);
+        r.v[1] = exact ? 1u : 0u;
         break;
+      }
+      // bit 0 of the canonical x times y as the search takes the sign of x,
+      // from four columns where they settle it (x mod 2^255 split as it stands,
+      // as for op 19)
+      case 24: r = fe_zero(); r.v[0] = fe10_parity_mul(fe10_split(x), y10); break;
       default: r10 = x10; break;
     }
     if (op < 17) r = fe10_canon(r10);
@@ -1032,8 +1039,8 @@ int og_hip_issue_bench(int ordinal, int op, uint32_t threads, uint32_t iters, do
   return 0;
 }
 
-// ops 0..5, 10..23, as include/onion_gen_hip.h lists them
-static bool field_op_known(int op) { return (op >= 0 && op <= 5) || (op >= 10 && op <= 23); }
+// ops 0..5, 10..24, as include/onion_gen_hip.h lists them
+static bool field_op_known(int op) { return (op >= 0 && op <= 5) || (op >= 10 && op <= 24); }
 
 int og_field_op_host(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, size_t n) {
   if (!field_op_known(op) || !a || !b || !out) return -1;

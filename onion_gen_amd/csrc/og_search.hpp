@@ -74,8 +74,9 @@ __device__ __forceinline__ bool bitmap_bit(const DevFilterArgs &f, uint32_t head
 }
 
 // The cheap test every candidate gets: on the low 32 bits of the key (its
-// first four bytes), for the three words fe10_mul_low32 leaves open (L, L + 19
-// and L + 38).  Never false for a candidate the exact test accepts.
+// first four bytes), taken from four columns of its product (fe10_mul_low32).
+// Never false for a candidate the exact test accepts: where that word is not
+// settled the probe passes.
 // A MODE with both key bits set leaves the key form to the filter at run time
 // (f.mode, wave-uniform): one instantiation serves every key form, which keeps
 // the slot engine's kernels to one search loop each.
@@ -100,24 +101,17 @@ __device__ __forceinline__ bool coarse_bit(const uint32_t *coarse, uint32_t cbit
 }
 
 template <uint32_t MODE>
-__device__ __forceinline__ bool maybe_key(uint32_t lo, const DevFilterArgs &f,
+__device__ __forceinline__ bool maybe_key(const Fe10 &a, const Fe10 &b, const DevFilterArgs &f,
                                           const uint32_t *coarse = nullptr) {
-  const uint32_t lo19 = lo + 19u, lo38 = lo + 38u;
-  if (key_is_prefix<MODE>(f))
-    return (((lo ^ f.want[0]) & f.mask[0]) == 0) | (((lo19 ^ f.want[0]) & f.mask[0]) == 0) |
-           (((lo38 ^ f.want[0]) & f.mask[0]) == 0);
+  bool exact;
+  const uint32_t lo = fe10_mul_low32(a, b, exact);
+  if (key_is_prefix<MODE>(f)) return !exact || ((lo ^ f.want[0]) & f.mask[0]) == 0;
   if (key_is_bitmap<MODE>(f)) {
-    const uint32_t h1 = bswap32(lo), h2 = bswap32(lo19), h3 = bswap32(lo38);
-    if (!coarse) return bitmap_bit(f, h1) | bitmap_bit(f, h2) | bitmap_bit(f, h3);
-    const bool c1 = coarse_bit(coarse, f.coarse_bits, h1);
-    const bool c2 = coarse_bit(coarse, f.coarse_bits, h2);
-    const bool c3 = coarse_bit(coarse, f.coarse_bits, h3);
-    if (f.coarse_bits == f.bits) return c1 | c2 | c3;
-    bool r = false;
-    if (c1) r = bitmap_bit(f, h1);
-    if (c2) r |= bitmap_bit(f, h2);
-    if (c3) r |= bitmap_bit(f, h3);
-    return r;
+    const uint32_t h = bswap32(lo);
+    if (!coarse) return !exact || bitmap_bit(f, h);
+    const bool c = !exact || coarse_bit(coarse, f.coarse_bits, h);
+    if (f.coarse_bits == f.bits || !exact) return c;
+    return c && bitmap_bit(f, h);
   }
   return false;
 }
@@ -285,7 +279,7 @@ template <uint32_t MODE>
 __device__ __forceinline__ void test_key_only(const Fe10 &a, const Fe10 &b, uint64_t k,
                                               const DevFilterArgs &f, const HitSink &sink,
                                               const uint32_t *coarse = nullptr) {
-  const bool maybe = maybe_key<MODE>(fe10_mul_low32(a, b), f, coarse);
+  const bool maybe = maybe_key<MODE>(a, b, f, coarse);
   if (__ballot(maybe) == 0) return;  // the common case: no lane needs more
   bool hit = false;
   Fe c;
@@ -527,9 +521,9 @@ __device__ __forceinline__ void drain_deferred(uint32_t *scratch, uint32_t top, 
       Fe10 e = ld10(scratch, top - (j0 + i), T, g);
       Fe10 num, den;
       sign_terms(e, scratch, T, k_base, half, table, num, den);
-      const Fe10 x = fe10_mul(num, fe10_mul(ld10(scratch, 4 + i, T, g), inv));
+      const uint32_t sign = fe10_parity_mul(num, fe10_mul(ld10(scratch, 4 + i, T, g), inv));
       inv = fe10_mul(den, inv);
-      e.v[7] |= fe10_parity_product(x) << 31;
+      e.v[7] |= sign << 31;
       drain_one(e, f, banks, maps, sink);
     }
   }
@@ -593,7 +587,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         const uint64_t mn = __ballot(need);
         if (mn) {
           uint32_t sign = 0;
-          if (!defer) sign = fe10_parity_product(fe10_mul(P.X, iz));
+          if (!defer) sign = fe10_parity_mul(P.X, iz);
           park(scratch, top, T, g, parked, mn, need, c, sign, kc);
         }
         report(hit, kc, c, sink);
@@ -641,11 +635,11 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
           const Fe10 de = fe10_carry(fe10_sub(fe10_mulc(YZ, ld10_const(q + kTabX)),
                                               fe10_mulc(TX, ld10_const(q + kTabYdq))));
           if (mm)
-            park(scratch, top, T, g, parked, mm, nm, cm,
-                 fe10_parity_product(fe10_mul(fe10_sub(ga, de), iE)), km);
+            park(scratch, top, T, g, parked, mm, nm, cm, fe10_parity_mul(fe10_sub(ga, de), iE),
+                 km);
           if (mp)
-            park(scratch, top, T, g, parked, mp, np, cp,
-                 fe10_parity_product(fe10_mul(fe10_add(ga, de), iE)), kp);
+            park(scratch, top, T, g, parked, mp, np, cp, fe10_parity_mul(fe10_add(ga, de), iE),
+                 kp);
         }
         report(hm, km, cm, sink);
         report(hp, kp, cp, sink);
