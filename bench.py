@@ -52,7 +52,10 @@ def field_muls_per_candidate(half, key_form=True, with_x=False):
     and the prefix product), 8 per pair backward (two inverses peeled, the
     double products alpha and beta, two each, and the two y), and per window
     of 2*half+1 candidates the six products of the centre's coordinates, the
-    centre (4), one inversion (254 sq + 11 mul) and the stride addition (7).
+    centre (4), one inversion and the stride addition (7).  The inversion is
+    the divstep recurrence, whose 20 batches make 90 products of 30-bit limbs
+    each (the updates of f, g, d and e): 1,800 products, 28.1 multiplications'
+    worth; its divsteps are logic, not products.
     The key forms make only four columns of each y (40 of its 100 products,
     the probe word of fe10_mul_low32), so their pair counts 8.8; the whole y
     of the rare lanes whose probe passes is left out.  +3 per candidate when
@@ -60,7 +63,7 @@ def field_muls_per_candidate(half, key_form=True, with_x=False):
     pair).  Priced at 64 32x32-bit products each, the radix-independent count
     of an 8 x 32 schoolbook."""
     pair = 8.8 if key_form else 10.0
-    window = pair * half + 6 + 4 + 265 + 7
+    window = pair * half + 6 + 4 + 1800 / 64 + 7
     return window / (2 * half + 1) + (3 if with_x else 0)
 
 

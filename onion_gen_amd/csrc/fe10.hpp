@@ -195,6 +195,20 @@ __device__ __forceinline__ void fe10_fold(Fe10 &r, uint64_t acc) {
   r.v[0] = (uint32_t)h0 & ((1u << 26) - 1);
 }
 
+// The same fold for a carry below 2^32, one v_mad_u64_u32 where the 64-bit
+// carry takes two more instructions.  Column 9 has no wrapped and no doubled
+// terms, so its sum is below 10 F G 2^51 (plus a carry in below 2^38) for
+// operands below F and G units, and the carry out of limb 9 (that sum over
+// 2^25) is below 2^32 whenever F G < 6.4: every product by a constant (F <= 5,
+// G = 1, or two of them with F = G = 1) and every multiplication of the
+// search loops (F G <= 3).  The curve addition's products (up to 4 x 3
+// units) take fe10_mul_wide.
+__device__ __forceinline__ void fe10_fold32(Fe10 &r, uint64_t acc) {
+  const uint64_t h0 = mad64((uint32_t)acc, 19u, (uint64_t)r.v[0]);
+  r.v[1] += (uint32_t)(h0 >> 26);
+  r.v[0] = (uint32_t)h0 & ((1u << 26) - 1);
+}
+
 // The product by a tight, wave-uniform g (a table constant): the same
 // partial products as fe10_mul, with the doubling of odd-by-odd terms moved
 // onto g, so that every scaled form of g (2g, 19g, 38g) is scalar work and
@@ -216,7 +230,7 @@ __device__ __forceinline__ Fe10 fe10_mulc(const Fe10 &f, const Fe10 &g) {
     }
     fe10_take(r, k, acc);
   }
-  fe10_fold(r, acc);
+  fe10_fold32(r, acc);
   return r;
 }
 
@@ -243,11 +257,12 @@ __device__ __forceinline__ Fe10 fe10_mulc2(const Fe10 &f1, const Fe10 &g1, const
     acc = fe10_column_s(f2.v, b2, acc);
     fe10_take(r, k, acc);
   }
-  fe10_fold(r, acc);
+  fe10_fold32(r, acc);
   return r;
 }
 
-OG_HD Fe10 fe10_mul(const Fe10 &f, const Fe10 &g) {
+template <bool kWide>
+__device__ __forceinline__ Fe10 fe10_mul_t(const Fe10 &f, const Fe10 &g) {
   uint32_t g19[10], f2[10];
 #pragma unroll
   for (int i = 0; i < 10; ++i) {
@@ -272,14 +287,23 @@ OG_HD Fe10 fe10_mul(const Fe10 &f, const Fe10 &g) {
     } else {
       acc = fe10_column(a, b, acc);
     }
-    const int w = fe10_width(k);
-    r.v[k] = (uint32_t)acc & ((1u << w) - 1);
-    acc >>= w;
+    fe10_take(r, k, acc);
   }
-  const uint64_t h0 = (uint64_t)r.v[0] + 19 * acc;
-  r.v[1] += (uint32_t)(h0 >> 26);
-  r.v[0] = (uint32_t)h0 & ((1u << 26) - 1);
+  if (kWide)
+    fe10_fold(r, acc);
+  else
+    fe10_fold32(r, acc);
   return r;
+}
+
+// Operands with F G < 6.4 (fe10_fold32); the looser always goes first.
+__device__ __forceinline__ Fe10 fe10_mul(const Fe10 &f, const Fe10 &g) {
+  return fe10_mul_t<false>(f, g);
+}
+
+// Any operands within fe10_mul's bounds (f up to 5 units, g up to 3.3).
+__device__ __forceinline__ Fe10 fe10_mul_wide(const Fe10 &f, const Fe10 &g) {
+  return fe10_mul_t<true>(f, g);
 }
 #else
 // The host product: every column summed on its own, then one carry pass.  The
@@ -311,6 +335,8 @@ OG_HD Fe10 fe10_mul(const Fe10 &f, const Fe10 &g) {
   }
   return fe10_carry64(h);
 }
+
+OG_HD Fe10 fe10_mul_wide(const Fe10 &f, const Fe10 &g) { return fe10_mul(f, g); }
 #endif
 
 // The low 32 bits of the canonical form of f*g from four of the product's

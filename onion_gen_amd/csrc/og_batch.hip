@@ -115,8 +115,8 @@ og_batch_address_kernel(BatchArgs a) {
   const SlotDesc d = a.slots[slot];  // block-uniform
   if (!(d.f.mode & kModePattern)) return;
   __shared__ ScanBank banks[kMaxBanks];
-  __shared__ uint16_t dfa_lds[kLdsStates * 32];
-  load_filter(banks, dfa_lds, d.f);
+  __shared__ uint16_t dfa_lds[2];  // the address stage walks its automata in global memory
+  load_filter(banks, dfa_lds, d.f, false, false);
   HitSink sink{a.counter, a.hits, a.cap, slot, a.counter + 1 + d.lead, d.budget};
   const uint32_t g = slot * a.chains + local;
   address_stage(a.scratch, a.half, a.nslots * a.chains, g, a.parked[g], d.f, banks, sink);

@@ -69,17 +69,26 @@ __global__ __launch_bounds__(256) OG_SEARCH_ATTR(MODE) void og_search_kernel(Sea
 // runs at four waves per SIMD instead of two; the form that derives the signs
 // (products, an inversion per chunk) is its own instantiation, held to two
 // waves so that it does not spill, and the plain one keeps its registers.
+// The LDS holds the automaton banks (and with kMaps the window prefilter's
+// maps), never the expressions' automata, which the address stage walks in
+// global memory: without maps the registers alone bound the waves per SIMD.
 // kMaps: the set's window prefilter serves the address stage; its two maps
-// (64 KiB) take the LDS, so a CU holds two workgroups, of 1024 threads when
-// the launch allows it.
+// (64 KiB) take the LDS, in 1024-thread workgroups when the launch allows it
+// (one per CU: four waves per SIMD).  Two 640-thread workgroups per CU, five
+// waves per SIMD, measured slower: 367 against 314 ms for 1,000 five-symbol
+// words, whose 104 LDS reads per candidate meet more bank conflicts.
 template <bool kSign, bool kMaps>
-__global__ __launch_bounds__(kMaps ? 1024 : 256)
+constexpr uint32_t kAddressBlock = kMaps ? 1024u : 256u;
+
+template <bool kSign, bool kMaps>
+__global__ __launch_bounds__((kAddressBlock<kSign, kMaps>))
     __attribute__((amdgpu_waves_per_eu(kSign ? 2 : 4, 8))) void og_address_kernel(SearchArgs a) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ ScanBank banks[kMaxBanks];
-  __shared__ uint32_t lds_words[kMaps ? kPatternLdsWords : kLdsStates * 16];
+  __shared__ uint32_t lds_words[kMaps ? kPatternLdsWords : 1];
   uint16_t *dfa_lds = reinterpret_cast<uint16_t *>(lds_words);
-  load_filter(banks, dfa_lds, a.f, kMaps);
+  load_filter(banks, dfa_lds, a.f, kMaps, false);
+  if (g >= a.threads) return;  // the last workgroup of a launch of fewer chains
   HitSink sink{a.counter, a.hits, a.cap, 0, nullptr, 0};
   const uint32_t top = kPlanesPerPair * a.half;
   const lptr32 maps = kMaps ? (lptr32)lds_words : (lptr32) nullptr;
@@ -133,7 +142,12 @@ __host__ __device__ inline void field_op_one(int op, const uint32_t *a, const ui
       case 13: r10 = fe10_sub(x10, y10); break;
       case 14: r10 = fe10_invert(x10); break;
       // loose inputs at the bounds the search reaches: (2x - y) * (x - y)
-      case 16: r10 = fe10_mul(fe10_sub(fe10_add(x10, x10), y10), fe10_sub(x10, y10)); break;
+      case 16: r10 = fe10_mul_wide(fe10_sub(fe10_add(x10, x10), y10), fe10_sub(x10, y10)); break;
+      // the loosest operands the search loops multiply: (x - y) times y, 3 x 1
+      // units, with the 32-bit fold of the top carry
+      case 25: r10 = fe10_mul(fe10_sub(x10, y10), y10); break;
+      // the window inversion the search runs: the divstep recurrence
+      case 26: r10 = fe10_invert_sg(x10); break;
       // a product's output canonicalised as the search kernels do it: x times
       // one with x mod 2^255 split as it stands (values in [p, 2^255) stay
       // above p), and x times y
@@ -157,7 +171,7 @@ __host__ __device__ inline void field_op_one(int op, const uint32_t *a, const ui
       case 24: r = fe_zero(); r.v[0] = fe10_parity_mul(fe10_split(x), y10); break;
       default: r10 = x10; break;
     }
-    if (op < 17) r = fe10_canon(r10);
+    if (op < 17 || op == 25 || op == 26) r = fe10_canon(r10);
   }
   for (int l = 0; l < 8; ++l) out[l] = r.v[l];
 }
@@ -1039,8 +1053,8 @@ int og_hip_issue_bench(int ordinal, int op, uint32_t threads, uint32_t iters, do
   return 0;
 }
 
-// ops 0..5, 10..24, as include/onion_gen_hip.h lists them
-static bool field_op_known(int op) { return (op >= 0 && op <= 5) || (op >= 10 && op <= 24); }
+// ops 0..5, 10..26, as include/onion_gen_hip.h lists them
+static bool field_op_known(int op) { return (op >= 0 && op <= 5) || (op >= 10 && op <= 26); }
 
 int og_field_op_host(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, size_t n) {
   if (!field_op_known(op) || !a || !b || !out) return -1;

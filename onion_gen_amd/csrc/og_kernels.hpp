@@ -40,6 +40,7 @@
 // a filter needs the sign of x.
 #pragma once
 #include "fe10.hpp"
+#include "fe_sginv.hpp"
 #include "ge.hpp"
 #include "keccak.hpp"
 #include "og_scan.hpp"
@@ -202,11 +203,12 @@ __device__ __forceinline__ Point10 pt10_add1(const Point10 &p, const Cached10 &q
   const Fe10 e = fe10_sub(a, b), h = fe10_add(a, b);
   const Fe10 f = fe10_sub(d, c), g = fe10_add(d, c);
   Point10 r;
-  // f (up to 4 units) is the looser operand, so it goes first
-  r.X = fe10_mul(f, e);
-  r.Y = fe10_mul(g, h);
-  r.Z = fe10_mul(f, g);
-  r.T = fe10_mul(e, h);
+  // f (up to 4 units) is the looser operand, so it goes first; operands up
+  // to 4 x 3 units take the wide fold
+  r.X = fe10_mul_wide(f, e);
+  r.Y = fe10_mul_wide(g, h);
+  r.Z = fe10_mul_wide(f, g);
+  r.T = fe10_mul_wide(e, h);
   return r;
 }
 

@@ -516,7 +516,7 @@ __device__ __forceinline__ void drain_deferred(uint32_t *scratch, uint32_t top, 
       sign_terms(ld10(scratch, top - (j0 + i), T, g), scratch, T, k_base, half, table, num, den);
       D = fe10_mul(den, D);
     }
-    Fe10 inv = fe10_invert(D);
+    Fe10 inv = fe10_invert_sg(D);
     for (uint32_t i = cnt; i-- > 0;) {
       Fe10 e = ld10(scratch, top - (j0 + i), T, g);
       Fe10 num, den;
@@ -573,7 +573,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
     }
     // inv = 1/(D Z): the centre takes 1/Z = inv D, and inv Z = 1/D starts
     // the backward pass
-    Fe10 inv = fe10_invert(fe10_mul(D, P.Z));
+    Fe10 inv = fe10_invert_sg(fe10_mul(D, P.Z));
     const uint32_t top = kPlanesPerPair * H;  // the parking plane past the window's
     uint32_t parked = 0;  // the wave's parked entries (uniform)
     {
@@ -675,15 +675,19 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
 // 32 16-bit entries from a 256-byte aligned allocation, so they copy as
 // 32-bit words.
 // With `with_maps` (the search kernel's LDS holds kPatternLdsWords words) the
-// window prefilter's maps are copied instead when the set has them.
+// window prefilter's maps are copied instead when the set has them.  Without
+// `with_states` the automaton is not copied: the address stage walks the
+// expressions' automata in global memory, so its kernels hold only the banks
+// (and the maps) in LDS, which lets more workgroups share a CU.
 __device__ __forceinline__ void load_filter(ScanBank *lds, uint16_t *dfa_lds,
-                                            const DevFilterArgs &f, bool with_maps = false) {
+                                            const DevFilterArgs &f, bool with_maps = false,
+                                            bool with_states = true) {
   uint32_t *dst = reinterpret_cast<uint32_t *>(lds);
   for (uint32_t i = threadIdx.x; i < f.nbanks * kBankWords; i += blockDim.x) dst[i] = f.banks[i];
   if (with_maps && f.hash) {
     uint32_t *d = reinterpret_cast<uint32_t *>(dfa_lds);
     for (uint32_t i = threadIdx.x; i < 2 * kHashSliceWords; i += blockDim.x) d[i] = f.hash[i];
-  } else if (f.lds_states) {
+  } else if (with_states && f.lds_states) {
     const uint32_t *src = reinterpret_cast<const uint32_t *>(f.dfa + f.regex[0]);
     uint32_t *d = reinterpret_cast<uint32_t *>(dfa_lds);
     for (uint32_t i = threadIdx.x; i < f.lds_states * 16; i += blockDim.x) d[i] = src[i];

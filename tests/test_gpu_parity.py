@@ -88,6 +88,11 @@ def test_field_kernels_match_big_integers():
     assert field_op(15, a, b) == [x % P for x in a]
     assert field_op(14, nz, nz) == [pow(x, P - 2, P) for x in nz]
     assert field_op(16, a, b) == [(x - y) * (2 * x - y) % P for x, y in zip(a, b)]
+    assert field_op(25, a, b) == [(x - y) * y % P for x, y in zip(a, b)]
+    # the divstep inversion, against Fermat, over the edges and random values
+    # (0 maps to 0), and on values in [p, 2^255) as a product leaves them
+    inv_in = a[:2017] + [P, P + 1, P + 18, 2 ** 255 - 1, 2 ** 255 + P, 2 ** 256 - 1]
+    assert field_op(26, inv_in, inv_in) == [pow(x % P, P - 2, P) for x in inv_in]
     # a product's output canonicalised as the search does it (no carry pass
     # first): values in [p, 2^255) kept as they stand, and products
     above = [P, P + 1, P + 18, 2 ** 255 - 1, 2 ** 255 + P, 2 ** 256 - 1]

@@ -95,6 +95,11 @@ def test_field_arithmetic_both_forms_on_the_host():
         nz = [x for x in a if x % P][:150]
         assert host_field_op(base + 4, nz, nz) == [pow(x, P - 2, P) for x in nz]
     assert host_field_op(16, a, b) == [(x - y) * (2 * x - y) % P for x, y in zip(a, b)]
+    assert host_field_op(25, a, b) == [(x - y) * y % P for x, y in zip(a, b)]
+    # the divstep inversion, against Fermat, over the edges and random values
+    # (0 maps to 0), and on values in [p, 2^255) as a product leaves them
+    inv_in = a[:167] + [P, P + 1, P + 18, 2 ** 255 - 1, 2 ** 255 + P, 2 ** 256 - 1]
+    assert host_field_op(26, inv_in, inv_in) == [pow(x % P, P - 2, P) for x in inv_in]
     # a product's output canonicalised without a carry pass first: values in
     # [p, 2^255) kept as they stand (op 17), and products (op 18)
     above = [P, P + 1, P + 18, 2 ** 255 - 1, 2 ** 255 + P, 2 ** 256 - 1]
