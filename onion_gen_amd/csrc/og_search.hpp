@@ -107,11 +107,18 @@ __device__ __forceinline__ bool maybe_key(const Fe10 &a, const Fe10 &b, const De
   const uint32_t lo = fe10_mul_low32(a, b, exact);
   if (key_is_prefix<MODE>(f)) return !exact || ((lo ^ f.want[0]) & f.mask[0]) == 0;
   if (key_is_bitmap<MODE>(f)) {
-    const uint32_t h = bswap32(lo);
-    if (!coarse) return !exact || bitmap_bit(f, h);
-    const bool c = !exact || coarse_bit(coarse, f.coarse_bits, h);
-    if (f.coarse_bits == f.bits || !exact) return c;
-    return c && bitmap_bit(f, h);
+    const uint32_t h = bswap32(lo), inexact = exact ? 0u : 1u;
+    if (!coarse) return (inexact | (uint32_t)bitmap_bit(f, h)) != 0;
+    // bitwise, not short-circuit: no branch stands between this candidate's
+    // reads and the next one's, so the two are in flight together
+    const uint32_t c = inexact | (uint32_t)coarse_bit(coarse, f.coarse_bits, h);
+    if (f.coarse_bits == f.bits) return c != 0;
+    // the map in L2 for the lanes whose coarse bit is set; the others read
+    // word 0, one line for all of them
+    const uint32_t slot = (uint32_t)((uint64_t)h >> (32 - f.bits));
+    const uint32_t word = c ? slot >> 5 : 0u;
+    const uint32_t bit = (f.bitmap[word] >> (slot & 31)) & 1u;
+    return (c & (bit | inexact)) != 0;
   }
   return false;
 }
@@ -272,14 +279,12 @@ __device__ __forceinline__ void report(bool hit, uint64_t k, const Fe &c, const 
   append(hit && rank < sink.budget, sink.slot, k, c, sink.counter, sink.hits, sink.cap);
 }
 
-// Key forms only, for the candidate whose y is a*b: the low-32-bit probe from
-// four columns of the product, and the whole product and the exact test on
-// the lanes the wave ballot names.
+// Key forms only, for the candidate whose y is a*b and whose probe said
+// `maybe` (maybe_key): the whole product and the exact test on the lanes the
+// wave ballot names.
 template <uint32_t MODE>
-__device__ __forceinline__ void test_key_only(const Fe10 &a, const Fe10 &b, uint64_t k,
-                                              const DevFilterArgs &f, const HitSink &sink,
-                                              const uint32_t *coarse = nullptr) {
-  const bool maybe = maybe_key<MODE>(a, b, f, coarse);
+__device__ __forceinline__ void settle_key(bool maybe, const Fe10 &a, const Fe10 &b, uint64_t k,
+                                           const DevFilterArgs &f, const HitSink &sink) {
   if (__ballot(maybe) == 0) return;  // the common case: no lane needs more
   bool hit = false;
   Fe c;
@@ -295,6 +300,43 @@ __device__ __forceinline__ void test_key_only(const Fe10 &a, const Fe10 &b, uint
     }
   }
   report(hit, k, c, sink);
+}
+
+// The low-32-bit probe of one candidate (four columns of its product), then
+// settle_key.
+template <uint32_t MODE>
+__device__ __forceinline__ void test_key_only(const Fe10 &a, const Fe10 &b, uint64_t k,
+                                              const DevFilterArgs &f, const HitSink &sink,
+                                              const uint32_t *coarse = nullptr) {
+  settle_key<MODE>(maybe_key<MODE>(a, b, f, coarse), a, b, k, f, sink);
+}
+
+// Both candidates of a pair, their probes side by side under one test of
+// the set's wave-uniform form, so that a bitmap's reads (LDS, then L2) for
+// the two are in flight together; then each is settled.
+template <uint32_t MODE>
+__device__ __forceinline__ void test_key_pair(const Fe10 &am, const Fe10 &ap, const Fe10 &b,
+                                              uint64_t km, uint64_t kp, const DevFilterArgs &f,
+                                              const HitSink &sink, const uint32_t *coarse) {
+  bool mm, mp;
+  if (key_is_bitmap<MODE>(f) && coarse && f.coarse_bits != f.bits) {
+    bool em, ep;
+    const uint32_t hm = bswap32(fe10_mul_low32(am, b, em)), hp = bswap32(fe10_mul_low32(ap, b, ep));
+    // every lane reads; a lane whose coarse bit is clear reads word 0 of the
+    // map in L2, one line for all of them
+    const uint32_t cm = (em ? 0u : 1u) | (uint32_t)coarse_bit(coarse, f.coarse_bits, hm);
+    const uint32_t cp = (ep ? 0u : 1u) | (uint32_t)coarse_bit(coarse, f.coarse_bits, hp);
+    const uint32_t sm = (uint32_t)((uint64_t)hm >> (32 - f.bits));
+    const uint32_t sp = (uint32_t)((uint64_t)hp >> (32 - f.bits));
+    const uint32_t wm = f.bitmap[cm ? sm >> 5 : 0u], wp = f.bitmap[cp ? sp >> 5 : 0u];
+    mm = (cm & (((wm >> (sm & 31)) & 1u) | (em ? 0u : 1u))) != 0;
+    mp = (cp & (((wp >> (sp & 31)) & 1u) | (ep ? 0u : 1u))) != 0;
+  } else {
+    mm = maybe_key<MODE>(am, b, f, coarse);
+    mp = maybe_key<MODE>(ap, b, f, coarse);
+  }
+  settle_key<MODE>(mm, am, b, km, f, sink);
+  settle_key<MODE>(mp, ap, b, kp, f, sink);
 }
 
 // The key-derived part of a candidate under the general forms (and the key
@@ -644,8 +686,7 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
         report(hm, km, cm, sink);
         report(hp, kp, cp, sink);
       } else {
-        test_key_only<MODE>(um, iE, km, f, sink, coarse);
-        test_key_only<MODE>(up, iE, kp, f, sink, coarse);
+        test_key_pair<MODE>(um, up, iE, km, kp, f, sink, coarse);
       }
     }
     if (kPat) {
