@@ -146,11 +146,12 @@ std::vector<uint32_t> HostCurve::offset_table10(uint32_t half) const {
   const std::vector<uint32_t> t8 = offset_table(half);
   std::vector<uint32_t> out((size_t)half * kTableWords);
   for (uint32_t m = 0; m < half; ++m) {
-    Fe c[6];
+    Fe c[kTableTerms];
     for (int i = 0; i < 3; ++i)
       for (int l = 0; l < 8; ++l) c[i].v[l] = t8[(size_t)m * 24 + 8 * i + l];
     offset_terms(c);
-    for (int i = 0; i < 6; ++i) put_fe10(&out[(size_t)m * kTableWords + 10 * i], c[i]);
+    for (uint32_t i = 0; i < kTableTerms; ++i)
+      put_fe10(&out[(size_t)m * kTableWords + 10 * i], c[i]);
   }
   return out;
 }

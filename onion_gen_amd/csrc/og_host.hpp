@@ -20,7 +20,7 @@ struct HostCurve {
   // Q_m = m * 8G for m = 1..half as (x, y, d*x*y), canonical, 24 words each.
   std::vector<uint32_t> offset_table(uint32_t half) const;
   // The kernels' table in the 10-limb form: per entry xq, yq, dq and the
-  // derived xq*dq, yq*dq and -dq^2 (offset_terms), kTableWords words.
+  // derived terms of offset_terms, kTableWords words.
   std::vector<uint32_t> offset_table10(uint32_t half) const;
   // Set-up arguments for chains [first, first + count) of a start point given
   // as 128 bytes xyzt, in a state buffer of SoA stride `threads`; chain c's

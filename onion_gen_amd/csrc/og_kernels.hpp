@@ -19,7 +19,8 @@
 // (fixed for the window) times table constants:
 //     y(P +- Q) = (alpha +- beta) / E,  alpha = YZ*yq + TX*(xq dq),  beta = XZ*xq + TY*(yq dq)
 //     x(P +- Q) = (gamma +- delta) / E, gamma = XZ*yq - TY*(xq dq),  delta = YZ*xq - TX*(yq dq)
-// so the table holds, per offset, xq, yq, dq, xq*dq, yq*dq and -dq^2.
+// so the table holds, per offset, xq, yq, dq, xq*dq, yq*dq, -dq^2, and the
+// negations of xq*dq and yq*dq that make gamma and delta double products.
 //
 // Montgomery's trick runs over the E of each pair.  The forward pass forms
 // E = ZZ + TT*(-dq^2) and the running product, and stores two field elements
@@ -73,17 +74,22 @@ inline size_t scratch_planes(uint32_t half, bool pattern) {
   return pattern ? (size_t)kPlanesPerPair * half + 1 : (size_t)kChainPlanes * half;
 }
 
-// The offset table: per offset m (Q = (m + 1) * 8G), six field elements of
+// The offset table: per offset m (Q = (m + 1) * 8G), eight field elements of
 // ten limbs.
-constexpr uint32_t kTableWords = 60;
-constexpr uint32_t kTabX = 0, kTabY = 10, kTabDq = 20, kTabXdq = 30, kTabYdq = 40, kTabNdq2 = 50;
+constexpr uint32_t kTableTerms = 8;
+constexpr uint32_t kTableWords = 10 * kTableTerms;
+constexpr uint32_t kTabX = 0, kTabY = 10, kTabDq = 20, kTabXdq = 30, kTabYdq = 40, kTabNdq2 = 50,
+                   kTabNxdq = 60, kTabNydq = 70;
 
 // The derived constants of an offset: c[0..2] = xq, yq, dq (canonical) in,
-// c[3..5] = xq*dq, yq*dq, -dq^2 (canonical) out.
-OG_HD void offset_terms(Fe c[6]) {
+// c[3..7] = xq*dq, yq*dq, -dq^2, -xq*dq, -yq*dq (canonical) out.  The negated
+// products let gamma and delta (the x numerators) each be one double product.
+OG_HD void offset_terms(Fe c[kTableTerms]) {
   c[3] = fe_canon(fe_mul(c[0], c[2]));
   c[4] = fe_canon(fe_mul(c[1], c[2]));
   c[5] = fe_canon(fe_sub(fe_zero(), fe_mul(c[2], c[2])));
+  c[6] = fe_canon(fe_sub(fe_zero(), c[3]));
+  c[7] = fe_canon(fe_sub(fe_zero(), c[4]));
 }
 
 constexpr int kMaxBuckets = 16;

@@ -66,9 +66,9 @@ __global__ __launch_bounds__(256) void og_noalloc_table_kernel(TableArgs a) {
   const Fe zinv = fe_invert(q.Z);
   const Fe x = fe_canon(fe_mul(q.X, zinv)), y = fe_canon(fe_mul(q.Y, zinv));
   const Fe dxy = fe_canon(fe_mul(d, fe_mul(x, y)));
-  Fe c[6] = {x, y, dxy};
+  Fe c[kTableTerms] = {x, y, dxy};
   offset_terms(c);
-  for (int i = 0; i < 6; ++i) {
+  for (uint32_t i = 0; i < kTableTerms; ++i) {
     const Fe10 r = fe10_split(c[i]);
     for (int l = 0; l < 10; ++l) a.table[(size_t)m * kTableWords + 10 * i + l] = r.v[l];
   }

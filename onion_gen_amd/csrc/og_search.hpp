@@ -671,11 +671,10 @@ __device__ __forceinline__ void search_chain(Point10 &P, const Cached10 &st, cpt
           if (mm) park(scratch, top, T, g, parked, mm, nm, cm, 0u, km);
           if (mp) park(scratch, top, T, g, parked, mp, np, cp, 0u, kp);
         } else if (mm | mp) {
-          // x(P +- Q) = (gamma +- delta) / E
-          const Fe10 ga = fe10_carry(fe10_sub(fe10_mulc(XZ, ld10_const(q + kTabY)),
-                                              fe10_mulc(TY, ld10_const(q + kTabXdq))));
-          const Fe10 de = fe10_carry(fe10_sub(fe10_mulc(YZ, ld10_const(q + kTabX)),
-                                              fe10_mulc(TX, ld10_const(q + kTabYdq))));
+          // x(P +- Q) = (gamma +- delta) / E, gamma = XZ yq + TY (-xq dq) and
+          // delta = YZ xq + TX (-yq dq), each one double product
+          const Fe10 ga = fe10_mulc2(XZ, ld10_const(q + kTabY), TY, ld10_const(q + kTabNxdq));
+          const Fe10 de = fe10_mulc2(YZ, ld10_const(q + kTabX), TX, ld10_const(q + kTabNydq));
           if (mm)
             park(scratch, top, T, g, parked, mm, nm, cm, fe10_parity_mul(fe10_sub(ga, de), iE),
                  km);
