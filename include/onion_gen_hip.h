@@ -176,14 +176,15 @@ void og_hip_engine_destroy(og_engine *e);
  * (one double product).  Op 23 is the key forms' probe word for the product of
  * op 16's operands, from four of its columns, in output word 0, and in word 1
  * whether it is exact: the canonical product's low 32 bits are word 0 when
- * word 1 is 1, else word 0 plus 0, 19 or 38.  Op 24 is op 19 taken as the
- * search takes it, from four columns where they settle it.  Op 25 is
+ * word 1 is 1, else word 0 plus 0, 19 or 38.  Op 27 is the same from three
+ * columns: bits 0..25 of the canonical product (word 0 masked to 26 bits),
+ * exact on the same terms.  Op 24 is op 19 taken as the search takes it,
+ * from three columns where they settle it.  Op 25 is
  * (x - y) y in the 10-limb form, the loosest operands the search loops
  * multiply with the 32-bit fold of the top carry.  Op 26 is the inverse of x
  * by the divstep recurrence the search windows use (0 maps to 0).  Inputs are
- * n raw
- * 32-byte little-endian
- * words (any value below 2^256); outputs are canonical encodings.  The _host
+ * n raw 32-byte little-endian words (any value below 2^256); outputs are
+ * canonical encodings.  The _host
  * variant runs the same code on the CPU. */
 int og_hip_field_op(int ordinal, int op, const uint8_t *a, const uint8_t *b, uint8_t *out,
                     size_t n, char *err, size_t errcap);

@@ -339,8 +339,8 @@ OG_HD Fe10 fe10_mul(const Fe10 &f, const Fe10 &g) {
 OG_HD Fe10 fe10_mul_wide(const Fe10 &f, const Fe10 &g) { return fe10_mul(f, g); }
 #endif
 
-// The low 32 bits of the canonical form of f*g from four of the product's
-// ten columns, exact unless `exact` comes back false.  fe10_mul's result is
+// The low bits of the canonical form of f*g from some of the product's ten
+// columns, exact unless `exact` comes back false.  fe10_mul's result is
 // V mod 2^255 + 19c for the column sums' integer V = sum S_k 2^a_k and
 // c = floor(V / 2^255); columns 2..9 start at bit 51 or above, so its low
 // word is S_0 + S_1 2^26 + 19c mod 2^32.  c is taken from columns 8 and 9
@@ -349,50 +349,56 @@ OG_HD Fe10 fe10_mul_wide(const Fe10 &f, const Fe10 &g) { return fe10_mul(f, g); 
 // when t mod 2^51 is within 2^40 of 2^51.  The canonical form differs from
 // the result (by -p, +19 in the low word) only when the result is at least
 // p, which needs V mod 2^255 within 2^37 of 2^255, the same condition.  So
-// the returned word is the canonical low word whenever t mod 2^51 is below
+// the word is the canonical low word whenever t mod 2^51 is below
 // 2^51 - 2^41, that is unless bits 15..24 of S_9 + floor(S_8 / 2^26) are all
 // ones (about one product in 1,024); otherwise the canonical low word is the
-// returned one plus 0, 19 or 38.  Same operand bounds as fe10_mul.  The key
-// forms probe this word and the search loop takes the sign of x from its bit
-// 0, forming the whole product only where `exact` is false or a probe
-// passes: 40 of its 100 products.
-OG_HD uint32_t fe10_mul_low32(const Fe10 &f, const Fe10 &g, bool &exact) {
-  uint32_t g19[10], f2[10];
+// returned one plus 0, 19 or 38.  Same operand bounds as fe10_mul.
+//
+// Column 1 reaches only bits 26 and up, so fe10_mul_low26 leaves it out:
+// S_0 + 19c, whose bits 0..25 are the canonical form's (the first three key
+// bytes and the sign of x), from 30 of the 100 products; fe10_low_col1 adds
+// the column for bits 26..31.  The key forms probe these bits and the search
+// loop takes the sign of x from bit 0, forming the whole product only where
+// `exact` is false or a probe passes.
+template <int K>
+OG_HD uint64_t fe10_low_column(const Fe10 &f, const Fe10 &g) {
+  uint32_t a[10], b[10];
 #pragma unroll
   for (int i = 0; i < 10; ++i) {
-    g19[i] = 19u * g.v[i];
-    f2[i] = (i & 1) ? 2u * f.v[i] : f.v[i];
+    const int j = K - i;
+    const int jj = j < 0 ? j + 10 : j;
+    a[i] = ((i & 1) && (jj & 1)) ? 2u * f.v[i] : f.v[i];
+    b[i] = j < 0 ? 19u * g.v[jj] : g.v[j];
   }
-  uint64_t S[4];
-  const int ks[4] = {0, 1, 8, 9};
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int k = ks[c];
-    uint32_t a[10], b[10];
-#pragma unroll
-    for (int i = 0; i < 10; ++i) {
-      const int j = k - i;
-      const int jj = j < 0 ? j + 10 : j;
-      a[i] = ((i & 1) && (jj & 1)) ? f2[i] : f.v[i];
-      b[i] = j < 0 ? g19[jj] : g.v[j];
-    }
 #ifdef __HIP_DEVICE_COMPILE__
-    // ten v_mad_u64_u32 from zero: of columns 0 and 1 only the low word is
-    // used, which the compiler would otherwise form with 32-bit multiplies
-    // and adds, nineteen instructions
-    S[c] = fe10_column0(a, b);
+  // ten v_mad_u64_u32 from zero: of columns 0 and 1 only the low word is
+  // used, which the compiler would otherwise form with 32-bit multiplies and
+  // adds, nineteen instructions
+  return fe10_column0(a, b);
 #else
-    uint64_t acc = 0;
+  uint64_t acc = 0;
 #pragma unroll
-    for (int i = 0; i < 10; ++i) acc = mad64(a[i], b[i], acc);
-    S[c] = acc;
+  for (int i = 0; i < 10; ++i) acc = mad64(a[i], b[i], acc);
+  return acc;
 #endif
-  }
-  const uint64_t acc9 = S[3] + (S[2] >> 26);
-  exact = ((uint32_t)acc9 & (0x3ffu << 15)) != (0x3ffu << 15);
-  return (uint32_t)S[0] + ((uint32_t)S[1] << 26) + 19u * (uint32_t)(acc9 >> 25);
 }
 
+OG_HD uint32_t fe10_mul_low26(const Fe10 &f, const Fe10 &g, bool &exact) {
+  const uint64_t acc9 = fe10_low_column<9>(f, g) + (fe10_low_column<8>(f, g) >> 26);
+  exact = ((uint32_t)acc9 & (0x3ffu << 15)) != (0x3ffu << 15);
+  return (uint32_t)fe10_low_column<0>(f, g) + 19u * (uint32_t)(acc9 >> 25);
+}
+
+// Column 1's part of the low word: fe10_mul_low26 + this is fe10_mul_low32.
+OG_HD uint32_t fe10_low_col1(const Fe10 &f, const Fe10 &g) {
+  return (uint32_t)fe10_low_column<1>(f, g) << 26;
+}
+
+OG_HD uint32_t fe10_mul_low32(const Fe10 &f, const Fe10 &g, bool &exact) {
+  return fe10_mul_low26(f, g, exact) + fe10_low_col1(f, g);
+}
+
+constexpr uint32_t kLow26 = (1u << 26) - 1;
 
 OG_HD Fe10 fe10_sq(const Fe10 &f) { return fe10_mul(f, f); }
 
@@ -478,10 +484,10 @@ OG_HD uint32_t fe10_parity_product(const Fe10 &t) {
 }
 
 // Bit 0 of the canonical form of f*g (the sign of x in a packed key): from
-// four columns where that settles it, else from the whole product.
+// three columns where that settles it, else from the whole product.
 OG_HD uint32_t fe10_parity_mul(const Fe10 &f, const Fe10 &g) {
   bool exact;
-  const uint32_t lo = fe10_mul_low32(f, g, exact);
+  const uint32_t lo = fe10_mul_low26(f, g, exact);
   uint32_t sign = lo & 1u;
   if (!exact) sign = fe10_parity_product(fe10_mul(f, g));
   return sign;

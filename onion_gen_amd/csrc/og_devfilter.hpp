@@ -269,10 +269,14 @@ struct DevFilterHost {
       prefix_words(fs.filters[fs.fast[0]], want, mask);
     } else if (!fs.fast.empty()) {
       mode |= kModeBitmap;
-      // The reference's index when it built one; otherwise a 24-bit map at
-      // any occupancy, which is still a valid "maybe" for the device.
-      const std::vector<uint64_t> w = fs.index_bits ? fs.words : bitmap_for(fs, 24);
-      bits = fs.index_bits ? fs.index_bits : 24;
+      // A map of at least 24 bits, at any occupancy: behind it the exact
+      // table decides, so the map only sets how many candidates reach that
+      // table.  The reference's own index is taken when it is wider (up to 32
+      // bits); a narrower one (its automatic choice stops at 1% occupancy,
+      // 20 bits for 10,000 prefixes) would send about 16 times as many.
+      const bool theirs = fs.index_bits > 24;
+      const std::vector<uint64_t> w = theirs ? fs.words : bitmap_for(fs, 24);
+      bits = theirs ? fs.index_bits : 24;
       set_bitmap(w);
       if (fs.buckets.size() > (size_t)kMaxBuckets)
         return "more distinct prefix lengths than the device table holds";

@@ -165,9 +165,18 @@ __host__ __device__ inline void field_op_one(int op, const uint32_t *a, const ui
         r.v[1] = exact ? 1u : 0u;
         break;
       }
+      // the same from three columns: bits 0..25 of the canonical product
+      case 27: {
+        bool exact;
+        r = fe_zero();
+        r.v[0] = fe10_mul_low26(fe10_sub(fe10_add(x10, x10), y10), fe10_sub(x10, y10), exact) &
+                 kLow26;
+        r.v[1] = exact ? 1u : 0u;
+        break;
+      }
       // bit 0 of the canonical x times y as the search takes the sign of x,
-      // from four columns where they settle it (x mod 2^255 split as it stands,
-      // as for op 19)
+      // from three columns where they settle it (x mod 2^255 split as it
+      // stands, as for op 19)
       case 24: r = fe_zero(); r.v[0] = fe10_parity_mul(fe10_split(x), y10); break;
       default: r10 = x10; break;
     }
@@ -1054,7 +1063,7 @@ int og_hip_issue_bench(int ordinal, int op, uint32_t threads, uint32_t iters, do
 }
 
 // ops 0..5, 10..26, as include/onion_gen_hip.h lists them
-static bool field_op_known(int op) { return (op >= 0 && op <= 5) || (op >= 10 && op <= 26); }
+static bool field_op_known(int op) { return (op >= 0 && op <= 5) || (op >= 10 && op <= 27); }
 
 int og_field_op_host(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, size_t n) {
   if (!field_op_known(op) || !a || !b || !out) return -1;

@@ -73,8 +73,8 @@ __device__ __forceinline__ bool bitmap_bit(const DevFilterArgs &f, uint32_t head
   return (f.bitmap[slot >> 5] >> (slot & 31)) & 1u;
 }
 
-// The cheap test every candidate gets: on the low 32 bits of the key (its
-// first four bytes), taken from four columns of its product (fe10_mul_low32).
+// The cheap test every candidate gets: on the low bits of the key (its first
+// three bytes), taken from three columns of its product (fe10_mul_low26).
 // Never false for a candidate the exact test accepts: where that word is not
 // settled the probe passes.
 // A MODE with both key bits set leaves the key form to the filter at run time
@@ -100,12 +100,24 @@ __device__ __forceinline__ bool coarse_bit(const uint32_t *coarse, uint32_t cbit
   return (coarse[slot >> 5] >> (slot & 31)) & 1u;
 }
 
+// The probe word of a*b for the key forms: bits 0..25 of the canonical product
+// from three columns (the first three key bytes, all a prefix compare or a
+// map of up to 24 bits reads), with column 1 added for a wider map.
+template <uint32_t MODE>
+__device__ __forceinline__ uint32_t key_probe_word(const Fe10 &a, const Fe10 &b,
+                                                   const DevFilterArgs &f, bool &exact) {
+  uint32_t lo = fe10_mul_low26(a, b, exact);
+  if (key_is_bitmap<MODE>(f) && f.bits > 24) lo += fe10_low_col1(a, b);  // wave-uniform
+  return lo;
+}
+
 template <uint32_t MODE>
 __device__ __forceinline__ bool maybe_key(const Fe10 &a, const Fe10 &b, const DevFilterArgs &f,
                                           const uint32_t *coarse = nullptr) {
   bool exact;
-  const uint32_t lo = fe10_mul_low32(a, b, exact);
-  if (key_is_prefix<MODE>(f)) return !exact || ((lo ^ f.want[0]) & f.mask[0]) == 0;
+  const uint32_t lo = key_probe_word<MODE>(a, b, f, exact);
+  // a prefix compares its first three bytes here; settle_key compares the rest
+  if (key_is_prefix<MODE>(f)) return !exact || ((lo ^ f.want[0]) & f.mask[0] & kLow26) == 0;
   if (key_is_bitmap<MODE>(f)) {
     const uint32_t h = bswap32(lo), inexact = exact ? 0u : 1u;
     if (!coarse) return (inexact | (uint32_t)bitmap_bit(f, h)) != 0;
@@ -302,7 +314,7 @@ __device__ __forceinline__ void settle_key(bool maybe, const Fe10 &a, const Fe10
   report(hit, k, c, sink);
 }
 
-// The low-32-bit probe of one candidate (four columns of its product), then
+// The low-bits probe of one candidate (three columns of its product), then
 // settle_key.
 template <uint32_t MODE>
 __device__ __forceinline__ void test_key_only(const Fe10 &a, const Fe10 &b, uint64_t k,
@@ -321,7 +333,8 @@ __device__ __forceinline__ void test_key_pair(const Fe10 &am, const Fe10 &ap, co
   bool mm, mp;
   if (key_is_bitmap<MODE>(f) && coarse && f.coarse_bits != f.bits) {
     bool em, ep;
-    const uint32_t hm = bswap32(fe10_mul_low32(am, b, em)), hp = bswap32(fe10_mul_low32(ap, b, ep));
+    const uint32_t hm = bswap32(key_probe_word<MODE>(am, b, f, em));
+    const uint32_t hp = bswap32(key_probe_word<MODE>(ap, b, f, ep));
     // every lane reads; a lane whose coarse bit is clear reads word 0 of the
     // map in L2, one line for all of them
     const uint32_t cm = (em ? 0u : 1u) | (uint32_t)coarse_bit(coarse, f.coarse_bits, hm);

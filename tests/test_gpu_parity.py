@@ -119,7 +119,16 @@ def test_field_kernels_match_big_integers():
     assert all(d == 0 for d, e in zip(diffs, exact) if e)
     assert all(d in (0, 19, 38) for d in diffs)
     assert exact[len(edges):].count(False) <= len(a) // 200  # random operands: about 1 in 1,024
-    # the sign of x as the search takes it: from four columns where they settle it
+    # the three-column word (op 27): bits 0..25 of the same, on the same terms,
+    # and op 23's word less column 1
+    got26 = field_op(27, a, b)
+    d26 = [((2 * x - y) * (x - y) % P - w) % 2 ** 26 for x, y, w in zip(a, b, got26)]
+    assert [w >> 32 == 1 for w in got26] == exact
+    assert all(d == 0 for d, e in zip(d26, exact) if e)
+    assert all(d in (0, 19, 38) for d in d26)
+    assert [w & (2 ** 26 - 1) for w in got26] == [w % 2 ** 26 for w in got]
+    assert all(w % 2 ** 32 < 2 ** 26 for w in got26)
+    # the sign of x as the search takes it: from three columns where they settle it
     assert field_op(24, a, b) == [(x % 2 ** 255) * y % P & 1 for x, y in zip(a, b)]
     assert field_op(24, above, [1] * len(above)) == [(x % 2 ** 255) % P & 1 for x in above]
 
@@ -179,13 +188,19 @@ def test_shape_does_not_change_results():
         assert run_engine(start, fs, 1 << 17, **shape) == base
 
 
-def test_dictionary_bitmap_and_exact_table():
-    say("case: dictionary 10000 prefixes 5..8, 2^24-bit bitmap, n=2^22")
+@pytest.mark.parametrize("index_bits", [24, 0, 28])
+def test_dictionary_bitmap_and_exact_table(index_bits):
+    """The device's map in front of the exact table: 24 bits when the set's
+    own index is 24 bits or narrower (0: the automatic choice, 20 bits for
+    10,000 prefixes), the set's index when wider (28: the probe word then
+    takes its fourth column)."""
+    say(f"case: dictionary 10000 prefixes 5..8, index bits {index_bits}, n=2^22")
     words = workload.dictionary(10000, seed=2)
+    fs = filters.FilterSet.parse(words, index_bits=index_bits)
+    assert fs.index_words()[1] == (index_bits or 20)
     start = key.start_point(workload.chacha_stream(3, 32))
     n = 1 << 22
-    got = run_engine(start, filters.FilterSet.parse(words, index_bits=24), n,
-                     threads=1 << 13, half=256)
+    got = run_engine(start, fs, n, threads=1 << 13, half=256)
     want = oracle_hits(start, words, n)
     assert got == want and len(want) > 0
 
@@ -289,10 +304,11 @@ def test_sign_stage_chosen_from_the_filters():
         assert eng.sign_stage() == 1
 
 
-@pytest.mark.parametrize("bits", [16, 20, 24])
+@pytest.mark.parametrize("bits", [16, 20, 24, 28])
 def test_bitmap_survivors_are_exactly_the_slots_set(bits):
     """The bitmap probe through each workgroup's LDS copy: the whole map at
-    16 bits, its 18-bit or-fold in front of the map in L2 at 20 and 24 bits.
+    16 bits, its 18-bit or-fold in front of the map in L2 at 20, 24 and 28
+    bits (28: the probe word takes its fourth column).
     A bitmap-only engine (Engine::new, src/gpu/mod.rs:346-353) must report
     exactly the candidates whose canonical key's leading `bits` bits name a set
     slot - every key of the range enumerated by the oracle's BatchEngine."""

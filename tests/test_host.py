@@ -125,7 +125,16 @@ def test_field_arithmetic_both_forms_on_the_host():
     assert all(d == 0 for d, e in zip(diffs, exact) if e)
     assert all(d in (0, 19, 38) for d in diffs)
     assert exact[len(edges):].count(False) <= len(a) // 200  # random operands: about 1 in 1,024
-    # the sign of x as the search takes it: from four columns where they settle it
+    # the three-column word (op 27): bits 0..25 of the same, on the same terms,
+    # and op 23's word less column 1
+    got26 = host_field_op(27, a, b)
+    d26 = [((2 * x - y) * (x - y) % P - w) % 2 ** 26 for x, y, w in zip(a, b, got26)]
+    assert [w >> 32 == 1 for w in got26] == exact
+    assert all(d == 0 for d, e in zip(d26, exact) if e)
+    assert all(d in (0, 19, 38) for d in d26)
+    assert [w & (2 ** 26 - 1) for w in got26] == [w % 2 ** 26 for w in got]
+    assert all(w % 2 ** 32 < 2 ** 26 for w in got26)
+    # the sign of x as the search takes it: from three columns where they settle it
     assert host_field_op(24, a, b) == [(x % 2 ** 255) * y % P & 1 for x, y in zip(a, b)]
     assert host_field_op(24, above, [1] * len(above)) == [(x % 2 ** 255) % P & 1 for x in above]
     # the largest limbs a tight value can carry: all-ones limbs
