@@ -56,13 +56,13 @@ def field_muls_per_candidate(half, key_form=True, with_x=False):
     the divstep recurrence, whose 20 batches make 90 products of 30-bit limbs
     each (the updates of f, g, d and e): 1,800 products, 28.1 multiplications'
     worth; its divsteps are logic, not products.
-    The key forms make only four columns of each y (40 of its 100 products,
-    the probe word of fe10_mul_low32), so their pair counts 8.8; the whole y
+    The key forms make only three columns of each y (30 of its 100 products,
+    the probe word of fe10_mul_low26), so their pair counts 8.6; the whole y
     of the rare lanes whose probe passes is left out.  +3 per candidate when
     the sign of x is needed (gamma and delta, two each, and the two x per
     pair).  Priced at 64 32x32-bit products each, the radix-independent count
     of an 8 x 32 schoolbook."""
-    pair = 8.8 if key_form else 10.0
+    pair = 8.6 if key_form else 10.0
     window = pair * half + 6 + 4 + 1800 / 64 + 7
     return window / (2 * half + 1) + (3 if with_x else 0)
 

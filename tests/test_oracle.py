@@ -261,3 +261,40 @@ def test_pair_engine_matches_batch_engine():
         want = sorted((k, pk[:31] + bytes([pk[31] & 0x7F]))
                       for k, pk, _ in search(start, 1, n, 256, fs))
         assert got == want and want, (batch, len(got), len(want))
+
+
+def _distinct_prefixes(n, seed):
+    """n eight-symbol prefixes whose first 16 bits are i = 0..n-1 (so no two
+    share a slot of a map 16 to 24 bits wide), the other 24 bits random."""
+    from onion_gen_amd import workload
+    rng = random.Random(seed)
+    out = []
+    for i in range(n):
+        v = (i << 24) | rng.getrandbits(24)
+        out.append("".join(workload.ALPHABET[(v >> (35 - 5 * s)) & 31] for s in range(8)))
+    return out
+
+
+def test_index_width_follows_its_closed_form(oracle):
+    """The automatic width of the prefix index (src/filter.rs:2230-2299,
+    2312-2351): the narrowest of 16..24 bits whose occupancy is at most 1%,
+    else the widest whose occupancy is at most 25%, else no index.  For n
+    prefixes that each fill one slot, occupancy is n / 2^bits, so the width is
+    the least b >= 16 with 100 n <= 2^b; n two-symbol prefixes fill n / 1024
+    of any map, so up to 10 of them take 16 bits, 11 to 256 take 24, and more
+    take none.  Checked on the product's parser and the oracle's."""
+    from onion_gen_amd import filters
+    cases = [1, 655, 656, 1310, 1311, 2621, 2622, 5242, 5243, 10485, 10486, 20971,
+             20972, 41943, 41944, 65536]
+    for n in cases:
+        want = next((b for b in range(16, 25) if 100 * n <= 1 << b), 24)
+        lines = _distinct_prefixes(n, n)
+        assert filters.FilterSet.parse(lines).index_words()[1] == want, n
+        assert OracleFilters(lines).index()[1] == want, n
+    from onion_gen_amd import workload
+    two = [a + b for a in workload.ALPHABET for b in workload.ALPHABET]
+    for n, want in ((1, 16), (10, 16), (11, 24), (256, 24), (257, None)):
+        lines = random.Random(n).sample(two, n)
+        mine, theirs = filters.FilterSet.parse(lines).index_words(), OracleFilters(lines).index()
+        assert (mine[1] if mine else None) == want, n
+        assert (theirs[1] if theirs else None) == want, n
