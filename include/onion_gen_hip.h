@@ -122,7 +122,7 @@ void og_hip_engine_shape(const og_engine *e, uint32_t *threads, uint32_t *half, 
 int og_hip_engine_sign_stage(const og_engine *e);
 
 /* The search kernels an engine runs: 1 the key forms (prefix compare, bitmap
- * and exact table: no parking planes, the four-column probe of y), 2 the
+ * and exact table: no parking planes, the three-column probe of y), 2 the
  * pattern forms (the automata, the checksum stage); -1 for a null engine.  No
  * counterpart in the reference. */
 int og_hip_engine_kernels(const og_engine *e);

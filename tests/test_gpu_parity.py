@@ -177,6 +177,23 @@ def test_prefix_hits_match_oracle(prefix, n, shape):
         oracle_hits(start, prefix, n)
 
 
+@pytest.mark.parametrize("k,length", [(1000, 5), (50000, 6), (200000, 7)])
+def test_long_prefix_taken_from_a_candidate(k, length):
+    """Prefixes of 5 to 7 symbols reach past the probe's three key bytes: the
+    probe passes on its 24 bits and the whole product settles the rest.  The
+    prefix is the address of candidate k (from the oracle's chain), so the
+    search has at least that hit."""
+    from tests.oracle_lib import address, batch_chain
+    start = key.start_point(hashlib.sha256(b"long prefix %d" % k).digest())
+    n = 1 << 18
+    pk = batch_chain(start, 256, n // 256)[k]
+    prefix = address(pk)[:length]
+    say(f"case: prefix {prefix} from candidate {k}, n=2^18")
+    got = run_engine(start, filters.FilterSet.parse(prefix), n, threads=512, half=128)
+    want = oracle_hits(start, prefix, n)
+    assert got == want and k in [h[0] for h in want]
+
+
 def test_shape_does_not_change_results():
     """Chain count, window width and rounds are performance knobs only
     (src/batch.rs:235-264)."""
