@@ -364,6 +364,15 @@ int og_filters_scan_pair_host(const og_filterset *s, const uint8_t a[32], const 
 int og_filters_address_window_host(const og_filterset *s, const uint8_t address[35],
                                    int *window, int *match);
 
+/* The device's exact table behind the bitmap (per prefix length a sorted key
+ * array and its directory) run on the host for one canonical key (sign bit
+ * clear): *hit = whether the key's first 64 bits, masked to some length,
+ * equal a key of that length.  Exact for prefixes of up to 12 symbols, a
+ * "maybe" for longer ones.  No counterpart in the reference; it exists so that
+ * the table is checked against the set's own match_key.  Returns 0, or -1 for
+ * a null argument or a set without the table (not a dictionary of prefixes). */
+int og_filters_exact_table_host(const og_filterset *s, const uint8_t packed[32], int *hit);
+
 /* ------------------------------------------------------------------ batch */
 
 /* Many independent searches in one launch.  No counterpart in the reference,

@@ -52,6 +52,7 @@ SIGNATURES = {
     "og_filters_describe": (_c.c_size_t, [_c.c_void_p, _c.c_char_p, _c.c_size_t]),
     "og_filters_address_window_host": (_c.c_int, [_c.c_void_p, _c.c_char_p,
                                                   _c.POINTER(_c.c_int), _c.POINTER(_c.c_int)]),
+    "og_filters_exact_table_host": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.POINTER(_c.c_int)]),
     "og_filters_scan_pair_host": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_char_p,
                                              _c.POINTER(_c.c_int), _c.POINTER(_c.c_int)]),
     "og_filters_scan_host": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_int,

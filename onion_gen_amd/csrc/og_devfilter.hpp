@@ -114,8 +114,10 @@ struct DevFilterHost {
   bool hash_addr = false;         // the address stage uses it
   std::vector<uint32_t> coarse;   // the bitmap or-folded to coarse_bits bits
   uint32_t coarse_bits = 0;
-  std::vector<uint32_t> buckets;  // [offset, count, mask_lo, mask_hi]
+  std::vector<uint32_t> buckets;  // kBucketWords per distinct prefix length
   std::vector<uint64_t> keys;
+  std::vector<uint32_t> dirs;     // the buckets' directories
+  uint32_t dir_steps = 0;
   std::vector<uint32_t> pats;     // kPatWords per pattern
   std::vector<ScanBank> banks;    // the automaton over the same patterns
   std::vector<uint32_t> regex;    // per expression: entry offset, first state, live offset
@@ -152,9 +154,11 @@ struct DevFilterHost {
   uint16_t *d_dfa = nullptr;
   uint32_t *d_dfa_end = nullptr;
   uint64_t *d_keys = nullptr;
+  uint32_t *d_dirs = nullptr;
 
   size_t device_bytes() const {
-    return bitmap.size() * 4 + coarse.size() * 4 + hash.size() * 4 + buckets.size() * 4 + keys.size() * 8 + pats.size() * 4 +
+    return bitmap.size() * 4 + coarse.size() * 4 + hash.size() * 4 + buckets.size() * 4 + keys.size() * 8 +
+           dirs.size() * 4 + pats.size() * 4 +
            banks.size() * sizeof(ScanBank) + regex.size() * 4 + dfa.size() * 2 + dfa_end.size() * 4 +
            live.size() * 4;
   }
@@ -280,13 +284,29 @@ struct DevFilterHost {
       set_bitmap(w);
       if (fs.buckets.size() > (size_t)kMaxBuckets)
         return "more distinct prefix lengths than the device table holds";
+      uint32_t run = 0;  // the most keys behind one directory entry
       for (const Bucket &b : fs.buckets) {
-        buckets.push_back((uint32_t)keys.size());
-        buckets.push_back((uint32_t)b.keys.size());
-        buckets.push_back((uint32_t)b.mask);
-        buckets.push_back((uint32_t)(b.mask >> 32));
+        // the directory over the top D bits of the bucket's sorted keys,
+        // about one key an entry: entry t is the first key whose top D bits
+        // are t or more, entry 2^D the count
+        const uint32_t n = (uint32_t)b.keys.size();
+        uint32_t D = 1;
+        while (D < kDirMaxBits && (1u << D) < n) ++D;
+        const uint32_t shift = 64 - D, at = (uint32_t)dirs.size();
+        dirs.resize(at + (1u << D) + 1);
+        uint32_t i = 0;
+        for (uint32_t t = 0; t <= (1u << D); ++t) {
+          while (i < n && (b.keys[i] >> shift) < t) ++i;
+          dirs[at + t] = i;
+          if (t) run = std::max(run, dirs[at + t] - dirs[at + t - 1]);
+        }
+        const uint32_t rec[kBucketWords] = {(uint32_t)keys.size(), n, (uint32_t)b.mask,
+                                            (uint32_t)(b.mask >> 32), at, shift, 0, 0};
+        buckets.insert(buckets.end(), rec, rec + kBucketWords);
         keys.insert(keys.end(), b.keys.begin(), b.keys.end());
       }
+      dir_steps = 0;  // halvings that bring a run of `run` keys to one place
+      while ((1u << dir_steps) <= run) ++dir_steps;
     }
     if (!fs.general.empty() || !fs.regexes.empty()) {
       if (fs.regexes.size() > (size_t)kMaxRegexes)
@@ -458,6 +478,7 @@ struct DevFilterHost {
     if (e == hipSuccess) e = upload(&d_hash, hash, s);
     if (e == hipSuccess) e = upload(&d_buckets, buckets, s);
     if (e == hipSuccess) e = upload(&d_keys, keys, s);
+    if (e == hipSuccess) e = upload(&d_dirs, dirs, s);
     if (e == hipSuccess) e = upload(&d_pats, pats, s);
     if (e == hipSuccess) e = upload(&d_regex, regex, s);
     if (e == hipSuccess) e = upload(&d_dfa, dfa, s);
@@ -480,6 +501,8 @@ struct DevFilterHost {
     d_coarse = d_hash = nullptr;
     (void)hipFree(d_buckets);
     (void)hipFree(d_keys);
+    (void)hipFree(d_dirs);
+    d_dirs = nullptr;
     (void)hipFree(d_pats);
     (void)hipFree(d_banks);
     (void)hipFree(d_regex);
@@ -506,9 +529,11 @@ struct DevFilterHost {
     a.hash_key = d_hash && hash_key ? 1u : 0u;
     a.hash_addr = d_hash && hash_addr ? 1u : 0u;
     a.coarse_bits = d_coarse ? coarse_bits : 0u;
-    a.nbuckets = (uint32_t)(buckets.size() / 4);
+    a.nbuckets = (uint32_t)(buckets.size() / kBucketWords);
     a.buckets = d_buckets;
     a.keys = d_keys;
+    a.dirs = d_dirs;
+    a.dir_steps = dir_steps;
     a.nbanks = (uint32_t)banks.size();
     a.banks = d_banks;
     a.npat = (uint32_t)(pats.size() / kPatWords);

@@ -403,6 +403,21 @@ size_t og_filters_describe(const og_filterset *s, char *out, size_t cap) {
   return t.size();
 }
 
+int og_filters_exact_table_host(const og_filterset *s, const uint8_t packed[32], int *hit) {
+  if (!s || !packed || !hit) return -1;
+  const DevFilterHost &h = const_cast<og_filterset *>(s)->host_tables();
+  if (!s->tables_err.empty() || !(h.mode & kModeBitmap) || h.buckets.empty()) return -1;
+  const uint32_t hi = ((uint32_t)packed[0] << 24) | ((uint32_t)packed[1] << 16) |
+                      ((uint32_t)packed[2] << 8) | packed[3];
+  const uint32_t lo = ((uint32_t)packed[4] << 24) | ((uint32_t)packed[5] << 16) |
+                      ((uint32_t)packed[6] << 8) | packed[7];
+  *hit = exact_table(hi, lo, h.buckets.data(), h.keys.data(), h.dirs.data(),
+                     (uint32_t)(h.buckets.size() / kBucketWords), h.dir_steps)
+             ? 1
+             : 0;
+  return 0;
+}
+
 int og_filters_scan_host(const og_filterset *s, const uint8_t packed[32], int sign,
                          int *key_hit, int *need, int *hit) {
   if (!s || !packed || !key_hit || !need || !hit) return -1;

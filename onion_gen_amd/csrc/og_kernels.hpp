@@ -127,8 +127,10 @@ struct DevFilterArgs {
   uint32_t bits;                  // kModeBitmap
   const uint32_t *bitmap;
   uint32_t nbuckets;              // exact table behind the bitmap
-  const uint32_t *buckets;        // [nbuckets][offset, count, mask_lo, mask_hi]
+  const uint32_t *buckets;        // [nbuckets][kBucketWords]
   const uint64_t *keys;
+  const uint32_t *dirs;           //   the buckets' directories
+  uint32_t dir_steps;             //   halvings after a directory lookup
   uint32_t nbanks;                // kModePattern: the automaton
   const uint32_t *banks;          // nbanks records of kBankWords
   uint32_t npat;                  // the same patterns, for the symbol-wise check

@@ -463,4 +463,55 @@ OG_HD bool regex_address(const uint32_t w[9], const uint32_t *recs, T16 dfa, T32
   return false;
 }
 
+// The exact table behind a bitmap: per distinct prefix length a bucket of
+// kBucketWords words - offset and count of its sorted keys (masked big-endian
+// 64-bit heads), the mask (low, high word), its directory's offset and the
+// shift that takes a key's top D bits as the directory index.  The directory
+// has 2^D + 1 entries: entry t is the first key whose top D bits are t or more.
+constexpr int kBucketWords = 8;
+constexpr uint32_t kDirMaxBits = 20;
+
+// The exact table: whether the key's first 64 bits (big-endian, head_hi
+// first), masked to some bucket's prefix length, equal one of that bucket's
+// sorted keys.  Four buckets at a time side by side, so their loads are in
+// flight together on the device: the directory entry of the key's top bits gives the run of
+// keys that share them (about one), and `steps` halvings of the run find the
+// first key not below the probe.  Out of line: the rare lanes that get here
+// (a bitmap pass or an inexact probe word) keep its registers out of the
+// search loop's.  The host runs the same code for the tests.
+static __host__ __device__ __noinline__ bool exact_table(uint32_t head_hi, uint32_t head_lo,
+                                         const uint32_t *buckets, const uint64_t *keys,
+                                         const uint32_t *dirs, uint32_t nbuckets, uint32_t steps) {
+  const uint64_t h = ((uint64_t)head_hi << 32) | head_lo;
+  bool found = false;
+  for (uint32_t b0 = 0; b0 < nbuckets; b0 += 4) {
+    uint64_t probe[4];
+    uint32_t lo[4], hi[4], end[4], off[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t b = b0 + j < nbuckets ? b0 + j : nbuckets - 1;  // wave-uniform
+      const uint32_t *r = buckets + (size_t)kBucketWords * b;
+      probe[j] = h & ((uint64_t)r[2] | ((uint64_t)r[3] << 32));
+      const uint32_t t = (uint32_t)(probe[j] >> r[5]);
+      off[j] = r[0];
+      lo[j] = dirs[r[4] + t];
+      hi[j] = end[j] = b0 + j < nbuckets ? dirs[r[4] + t + 1] : lo[j];
+    }
+    for (uint32_t s = 0; s < steps; ++s) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t mid = (lo[j] + hi[j]) >> 1;
+        if (lo[j] < hi[j]) {
+          if (keys[off[j] + mid] < probe[j]) lo[j] = mid + 1;
+          else hi[j] = mid;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (lo[j] < end[j]) found |= keys[off[j] + lo[j]] == probe[j];
+  }
+  return found;
+}
+
 }  // namespace og

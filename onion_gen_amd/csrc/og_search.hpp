@@ -146,23 +146,9 @@ __device__ __forceinline__ bool exact_key(const Fe &c, const DevFilterArgs &f) {
   }
   if (key_is_bitmap<MODE>(f)) {
     const uint32_t head = bswap32(c.v[0]);
-    if (!bitmap_bit(f, head)) return false;
-    if (f.nbuckets == 0) return true;
-    const uint64_t h = ((uint64_t)head << 32) | bswap32(c.v[1]);
-    for (uint32_t b = 0; b < f.nbuckets; ++b) {
-      const uint32_t off = f.buckets[4 * b], cnt = f.buckets[4 * b + 1];
-      const uint64_t mask = (uint64_t)f.buckets[4 * b + 2] | ((uint64_t)f.buckets[4 * b + 3] << 32);
-      const uint64_t probe = h & mask;
-      const uint64_t *keys = f.keys + off;
-      uint32_t lo = 0, hi = cnt;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (keys[mid] < probe) lo = mid + 1;
-        else hi = mid;
-      }
-      if (lo < cnt && keys[lo] == probe) return true;
-    }
-    return false;
+    // the table holds every key the map does, so with one it alone decides
+    if (f.nbuckets == 0) return bitmap_bit(f, head);
+    return exact_table(head, bswap32(c.v[1]), f.buckets, f.keys, f.dirs, f.nbuckets, f.dir_steps);
   }
   return false;
 }

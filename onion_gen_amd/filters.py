@@ -71,6 +71,15 @@ class FilterSet:
             raise OgError("the set runs no automaton bank or expression")
         return (bool(kh[0]), bool(nd[0])), (bool(kh[1]), bool(nd[1]))
 
+    def exact_table(self, packed):
+        """The device's exact table behind the bitmap, run on the host for one
+        canonical key: whether a prefix of some length in the set matches its
+        first 64 bits."""
+        hit = ctypes.c_int()
+        if lib().og_filters_exact_table_host(self.handle, bytes(packed), ctypes.byref(hit)) != 0:
+            raise OgError("the set has no exact table (not a dictionary of prefixes)")
+        return bool(hit.value)
+
     def address_window(self, address):
         """The address stage on a 35-byte address: (window, match) - whether
         the address-stage window prefilter lets the automata walk (always,

@@ -605,3 +605,49 @@ def test_address_window_prefilter_never_stops_a_matching_walk():
         matches += match
     # about 0.23% of random addresses pass: 52 probes of about 4.4e-5 each
     assert walks <= 40 and walks >= matches, (walks, matches)
+
+
+def _key_with_prefix(word, rng):
+    """A canonical packed key (sign clear) whose address starts with `word`."""
+    bits = 5 * len(word)
+    v = int("".join(format(workload.ALPHABET.index(c), "05b") for c in word) or "0", 2)
+    k = (v << (256 - bits)) | (rng.getrandbits(256) & ((1 << (256 - bits)) - 1))
+    pk = bytearray(k.to_bytes(32, "big"))
+    pk[31] &= 0x7F
+    return bytes(pk)
+
+
+def test_exact_table_matches_the_sets_own_matcher():
+    """The device's exact table behind the bitmap (og_scan.hpp exact_table:
+    a directory over each length's sorted keys, four lengths side by side),
+    run on the host, against FilterSet::match_key (src/filter.rs:2194-2217):
+    on keys that start with a word of the set, with a word whose last symbol
+    is changed, and at random.  A hit of the set's matcher is always a hit of
+    the table; for words of up to 12 symbols the two agree, and a longer word
+    is compared on its first 64 bits (a "maybe")."""
+    rng = random.Random(21)
+    mixed = set()
+    while len(mixed) < 400:
+        n = rng.choice([2, 3, 4, 5, 6, 7, 9, 11, 12, 13, 14])
+        mixed.add("".join(rng.choice(workload.ALPHABET) for _ in range(n)))
+    for words in (workload.dictionary(10000, seed=2), sorted(mixed)):
+        fs = filters.FilterSet.parse(words)
+        # a long word's first 64 bits: its first 12 symbols and the top 4 bits
+        # of the 13th
+        heads = {(w[:12], workload.ALPHABET.index(w[12]) >> 1) for w in words if len(w) >= 13}
+        keys = []
+        for w in rng.sample(words, min(600, len(words))):
+            keys.append(_key_with_prefix(w, rng))
+            other = rng.choice([c for c in workload.ALPHABET if c != w[-1]])
+            keys.append(_key_with_prefix(w[:-1] + other, rng))
+        keys += [_key_with_prefix("", rng) for _ in range(3000)]
+        hits = 0
+        for pk in keys:
+            table, match = fs.exact_table(pk), fs.match_key(pk) >= 0
+            hits += match
+            assert table or not match, pk.hex()
+            if table and not match:
+                head = int.from_bytes(pk[:8], "big")
+                sym = "".join(workload.ALPHABET[(head >> (59 - 5 * i)) & 31] for i in range(12))
+                assert (sym, head & 0xF) in heads, pk.hex()
+        assert hits >= min(600, len(words))
