@@ -24,6 +24,10 @@ inline std::string build_banks(const FilterSet &fs, std::vector<ScanBank> &out,
                                bool *every = nullptr, bool general = true) {
   out.clear();
   std::vector<int> used;
+  // per bank: the patterns placed in it, and whether the one (or first)
+  // is a literal of 1..6 symbols that may start anywhere
+  std::vector<int> placed;
+  std::vector<uint32_t> lit_val, lit_len;
   // 0: placed, 1: no room
   auto place = [&](const std::vector<uint32_t> &sets, Anchor anchor, bool proxy) {
     const int len = (int)sets.size();
@@ -37,10 +41,22 @@ inline std::string build_banks(const FilterSet &fs, std::vector<ScanBank> &out,
       b.jstart = kAddressLen;
       out.push_back(b);
       used.push_back(0);
+      placed.push_back(0);
+      lit_val.push_back(0);
+      lit_len.push_back(0);
     }
     ScanBank &b = out[k];
     const int off = used[k];
     used[k] += kept;
+    bool literal = anchor == kAnywhere && !proxy && len >= 1 && len <= 6;
+    uint32_t v = 0;
+    for (int i = 0; i < len && literal; ++i) {
+      literal = __builtin_popcount(sets[i]) == 1;
+      v |= (uint32_t)__builtin_ctz(sets[i] | (1u << 31)) << (27 - 5 * i);
+    }
+    ++placed[k];
+    lit_val[k] = v;
+    lit_len[k] = literal ? (uint32_t)len : 0u;
     for (int i = 0; i < kept; ++i)
       for (int c = 0; c < 32; ++c)
         if ((sets[i] >> c) & 1u) b.b[c] |= 1u << (off + i);
@@ -82,12 +98,22 @@ inline std::string build_banks(const FilterSet &fs, std::vector<ScanBank> &out,
     }
     if (!placed && every) *every = true;
   }
-  for (ScanBank &b : out) {
+  for (size_t k = 0; k < out.size(); ++k) {
+    ScanBank &b = out[k];
     for (int c = 0; c < 32; ++c) {
       b.u51[c >> 4] |= b.b[c];
       b.uall |= b.b[c];
     }
     b.u54 = b.b[0] | b.b[8] | b.b[16] | b.b[24];  // a, i, q, y
+    if (placed[k] == 1 && lit_len[k]) {
+      // one literal: compared at each placement, about 3 instructions a
+      // placement where the automaton takes about 6 a symbol
+      b.lit = 1;
+      b.lit_val = lit_val[k];
+      b.lit_mask = ~0u << (32 - 5 * lit_len[k]);
+      for (uint32_t j = 44; j < 49; ++j)
+        if (j + lit_len[k] <= 49) b.lit_tail |= 1u << (j - 44);
+    }
   }
   return "";
 }

@@ -382,6 +382,11 @@ size_t og_filters_describe(const og_filterset *s, char *out, size_t cap) {
          std::to_string(dict ? 0 : fs.general.size()) + " general form(s)";
     if (!fs.regexes.empty()) t += " and the proxies of " + std::to_string(fs.regexes.size()) + " expression(s)";
     t += e.empty() ? "\n" : " (" + e + ")\n";
+    size_t lits = 0;
+    for (const ScanBank &b : h.banks) lits += b.lit ? 1 : 0;
+    if (lits)
+      t += "literal compare: " + std::to_string(lits) +
+           " bank(s) of one literal, compared at each placement in the key pass\n";
   }
   if (h.key_proxy)
     t += "key tables: " + std::to_string(h.proxy_set.fast.size()) +

@@ -51,7 +51,14 @@ struct ScanBank {
   uint32_t u54;     // a, i, q or y
   uint32_t jstart;  // no pattern of the bank starts before this symbol
   uint32_t fmaybe;  // last bits of expression proxies
-  uint32_t pad[5];
+  // A bank that holds one unanchored literal of 1..6 symbols: its key pass
+  // compares the literal at each placement inside symbols 0..48 (lit_val, the
+  // literal's bits from bit 31 down, under lit_mask; bit j - 44 of lit_tail
+  // set where placement j in 44..48 ends by symbol 48) instead of walking
+  // every symbol.
+  uint32_t lit;
+  uint32_t lit_val, lit_mask, lit_tail;
+  uint32_t pad[1];
 };
 static_assert(sizeof(ScanBank) == kBankWords * 4, "bank layout");
 
@@ -76,15 +83,58 @@ OG_HD void key_words(const Fe &c, uint32_t w[8]) {
 // truncated pattern reached its last kept bit), so the full address decides.
 // `maybe`: an expression proxy occurred, or may reach past 48.  Each flag is
 // or-ed in, so the banks of a set accumulate.
+// The 32 bits from bit POS on of a string held in N big-endian words (zeros
+// past them).
+template <int POS, int N>
+OG_HD uint32_t key_bits32(const uint32_t *w) {
+  constexpr int i = POS >> 5, sh = POS & 31;
+  static_assert(i < N, "window past the string");
+  if constexpr (sh == 0) {
+    return w[i];
+  } else {
+    const uint32_t next = i + 1 < N ? w[i + 1 < N ? i + 1 : N - 1] : 0u;
+    return (w[i] << sh) | (next >> (32 - sh));
+  }
+}
+
+// The key pass of a one-literal bank: the least masked difference between the
+// literal and the key at placements J..48, so 0 where the literal sits (every
+// placement up to 43 fits a literal of at most 6 symbols; placements 44..48
+// count where bit J - 44 of `tail` says the literal ends by symbol 48).  Three
+// instructions a placement (a funnel shift, the masked difference, a minimum),
+// with no table read and no branch.
+template <int J>
+OG_HD uint32_t literal_min_from(const uint32_t w[8], uint32_t val, uint32_t mask, uint32_t tail,
+                                uint32_t m) {
+  if constexpr (J > 48) {
+    return m;
+  } else {
+    uint32_t x = (key_bits32<5 * J, 8>(w) ^ val) & mask;
+    if constexpr (J >= 44) x |= ((tail >> (J - 44)) & 1u) ^ 1u;
+#ifdef __HIP_DEVICE_COMPILE__
+    // keep the scheduler from hoisting every placement's shift at once, which
+    // would hold 49 values live in a loop already at its register limit
+    if constexpr (J % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+#endif
+    return literal_min_from<J + 1>(w, val, mask, tail, x < m ? x : m);
+  }
+}
+
 OG_HD void scan_bank_key(const ScanBank &s, const uint32_t w[8], bool &hit, bool &need,
                          bool &maybe) {
   const uint32_t y248 = w[7] & 1u;  // stream bit 255: the top bit of symbol 51
   const uint32_t js = s.jstart, iany = s.iany, fm = s.fmaybe;
   uint32_t D = 0, acc = 0;
-  // A bank whose patterns all start past the key span skips the key pass.
-  // Otherwise every symbol runs, with the injection masked before jstart
-  // (no branch per symbol): D stays 0 until a pattern may start.
-  if (js < 49) {
+  if (s.lit) {
+    hit |= literal_min_from<0>(w, s.lit_val, s.lit_mask, s.lit_tail, ~0u) == 0;
+    // D after symbol 48 from the partial matches a placement from 43 on
+    // leaves (one of at most 6 symbols reaching past 48 starts at 44 or later)
+#pragma unroll
+    for (int j = 43; j < 49; ++j) D = ((D << 1) | iany) & s.b[sym_at(w, j)];
+  } else if (js < 49) {
+    // A bank whose patterns all start past the key span skips the key pass.
+    // Otherwise every symbol runs, with the injection masked before jstart
+    // (no branch per symbol): D stays 0 until a pattern may start.
 #pragma unroll
     for (int j = 0; j < 49; ++j) {
       const uint32_t inj = (j == 0 ? s.istart : 0u) | ((uint32_t)j >= js ? iany : 0u);
@@ -307,20 +357,6 @@ OG_HD void regex_walk49x2(TP t, uint32_t &sa, uint32_t &sb, const uint32_t wa[8]
 // 1 to 4, packed as 8 W + S (DevFilterArgs::hash_ws).
 constexpr uint32_t kHashSliceBits = 18;
 constexpr uint32_t kHashSliceWords = 1u << (kHashSliceBits - 5);  // per map
-
-// The 32 bits from bit POS on of a string held in N big-endian words (zeros
-// past them).
-template <int POS, int N>
-OG_HD uint32_t key_bits32(const uint32_t *w) {
-  constexpr int i = POS >> 5, sh = POS & 31;
-  static_assert(i < N, "window past the string");
-  if constexpr (sh == 0) {
-    return w[i];
-  } else {
-    const uint32_t next = i + 1 < N ? w[i + 1 < N ? i + 1 : N - 1] : 0u;
-    return (w[i] << sh) | (next >> (32 - sh));
-  }
-}
 
 // Whether a window at placements J, J + S, ... <= LAST hits both maps.
 template <int W, int S, int LAST, int N, int J, class TL>

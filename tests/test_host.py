@@ -651,3 +651,44 @@ def test_exact_table_matches_the_sets_own_matcher():
                 sym = "".join(workload.ALPHABET[(head >> (59 - 5 * i)) & 31] for i in range(12))
                 assert (sym, head & 0xF) in heads, pk.hex()
         assert hits >= min(600, len(words))
+
+
+def test_one_literal_bank_agrees_with_oracle_at_every_placement():
+    """A bank of one literal of 1..6 symbols runs the key pass as a compare at
+    each placement (og_scan.hpp literal_hits_from) and the tail from the last
+    six symbols' automaton: on keys with the literal planted at every placement
+    0..55, with its last symbol changed, and at random, the host run of the
+    kernels' matcher equals the oracle's whole-address match, and a hit the key
+    pass leaves unsettled is flagged for the address stage."""
+    rng = random.Random(77)
+    for L in range(1, 7):
+        for trial in range(3):
+            pk0 = bytes(rng.getrandbits(8) for _ in range(32))
+            text = _address_text(pk0)
+            lit = "".join(rng.choice(workload.ALPHABET) for _ in range(L))
+            lines = ["contains:" + lit]
+            mine, theirs = filters.FilterSet.parse(lines), OracleFilters(lines)
+            assert any(l.startswith("literal compare: 1") for l in mine.describe())
+            keys = [bytes(rng.getrandbits(8) for _ in range(32)) for _ in range(40)]
+            for s in range(0, 56 - L + 1):
+                for word in (lit, lit[:-1] + rng.choice([c for c in workload.ALPHABET if c != lit[-1]])):
+                    if s + L > 51:
+                        break  # the checksum symbols are not set by the key
+                    # symbols 0..50 are the key's bits 0..254 (big-endian);
+                    # bit 255 stays the original key's
+                    t = text[:s] + word + text[s + L:]
+                    bits = "".join(format(workload.ALPHABET.index(c), "05b") for c in t[:51])
+                    keys.append(int(bits + str(pk0[31] & 1), 2).to_bytes(32, "big"))
+            hits = 0
+            for pk in keys:
+                sign = pk[31] >> 7
+                clear = pk[:31] + bytes([pk[31] & 0x7F])
+                kh, need, hit = mine.scan(clear, sign)
+                want = theirs.match_address(key.address_bytes(pk)) >= 0
+                assert hit == want, (lit, pk.hex())
+                if kh:
+                    assert hit
+                if hit and not kh:
+                    assert need, (lit, pk.hex())
+                hits += hit
+            assert hits >= 56 - L - 5
