@@ -358,16 +358,29 @@ OG_HD void regex_walk49x2(TP t, uint32_t &sa, uint32_t &sb, const uint32_t wa[8]
 constexpr uint32_t kHashSliceBits = 18;
 constexpr uint32_t kHashSliceWords = 1u << (kHashSliceBits - 5);  // per map
 
+// 32 bits whose low 18 are the string's bits P..P+17 (a map slot: its word
+// is bits 5..17, its bit bits 0..4, which a shift reads by itself).
+template <int P, int N>
+OG_HD uint32_t slice_bits(const uint32_t *w) {
+  constexpr int kUp = 32 - (int)kHashSliceBits;
+  if constexpr (P >= kUp) return key_bits32<P - kUp, N>(w);
+  else return w[0] >> (kUp - P);
+}
+
+// The bit of map word `word` a slice names (the shift takes its low 5 bits).
+OG_HD uint32_t slot_bit(uint32_t word, uint32_t slice) { return word >> (slice & 31u); }
+
 // Whether a window at placements J, J + S, ... <= LAST hits both maps.
 template <int W, int S, int LAST, int N, int J, class TL>
 OG_HD uint32_t window_hits_from(const uint32_t *w, TL maps) {
   if constexpr (J > LAST) {
     return 0u;
   } else {
-    const uint32_t a = key_bits32<5 * J, N>(w) >> (32 - kHashSliceBits);
-    const uint32_t b = key_bits32<5 * (J + W) - kHashSliceBits, N>(w) >> (32 - kHashSliceBits);
-    const uint32_t hit =
-        (maps[a >> 5] >> (a & 31)) & (maps[kHashSliceWords + (b >> 5)] >> (b & 31)) & 1u;
+    constexpr uint32_t kWords = kHashSliceWords - 1;
+    const uint32_t a = slice_bits<5 * J, N>(w);
+    const uint32_t b = slice_bits<5 * (J + W) - kHashSliceBits, N>(w);
+    const uint32_t hit = slot_bit(maps[(a >> 5) & kWords], a) &
+                         slot_bit(maps[kHashSliceWords + ((b >> 5) & kWords)], b) & 1u;
     return hit | window_hits_from<W, S, LAST, N, J + S>(w, maps);
   }
 }
