@@ -195,8 +195,14 @@ __device__ __forceinline__ uint32_t onion_checksum_gfx950(const uint32_t pk[8]) 
   // Round 0 on its own: 17 of its lanes are zero and three constant, which
   // the compiler folds.
   kround(a, keccak_rc(0));
+  // The table's address taken once, in scalar registers: left to itself the
+  // compiler reloads it from the global offset table every round, a scalar
+  // load the round's constant then waits on.
+  typedef const __attribute__((address_space(4))) uint64_t *cptr64;
+  cptr64 rc = (cptr64)kKeccakRcDev;
+  asm volatile("" : "+s"(rc));
 #pragma unroll 1
-  for (int round = 1; round < 23; ++round) kround(a, keccak_rc(round));
+  for (int round = 1; round < 23; ++round) kround(a, rc[round]);
   // round 23, cut to the low half of lane 0: chi of b[0], b[1], b[2] and iota
   const KTheta t = ktheta(a);
   const uint32_t b0 = ktd(a[0], t, 0).lo;
