@@ -24,10 +24,14 @@ inline std::string build_banks(const FilterSet &fs, std::vector<ScanBank> &out,
                                bool *every = nullptr, bool general = true) {
   out.clear();
   std::vector<int> used;
-  // per bank: the patterns placed in it, and whether the one (or first)
-  // is a literal of 1..6 symbols that may start anywhere
-  std::vector<int> placed;
-  std::vector<uint32_t> lit_val, lit_len;
+  // per bank: its literals that may start inside the key (value, mask and
+  // info as ScanBank takes them), and whether some other pattern may start
+  // there too (then the bank walks every symbol)
+  struct Lit {
+    uint32_t val, mask, info;
+  };
+  std::vector<std::vector<Lit>> lits;
+  std::vector<bool> walks;
   // 0: placed, 1: no room
   auto place = [&](const std::vector<uint32_t> &sets, Anchor anchor, bool proxy) {
     const int len = (int)sets.size();
@@ -41,22 +45,36 @@ inline std::string build_banks(const FilterSet &fs, std::vector<ScanBank> &out,
       b.jstart = kAddressLen;
       out.push_back(b);
       used.push_back(0);
-      placed.push_back(0);
-      lit_val.push_back(0);
-      lit_len.push_back(0);
+      lits.emplace_back();
+      walks.push_back(false);
     }
     ScanBank &b = out[k];
     const int off = used[k];
     used[k] += kept;
-    bool literal = anchor == kAnywhere && !proxy && len >= 1 && len <= 6;
-    uint32_t v = 0;
-    for (int i = 0; i < len && literal; ++i) {
-      literal = __builtin_popcount(sets[i]) == 1;
-      v |= (uint32_t)__builtin_ctz(sets[i] | (1u << 31)) << (27 - 5 * i);
+    if (anchor == kEnd && len <= 7) {
+      // starts past symbol 48: only the tail sees it
+    } else {
+      // a literal of its first (up to 6) symbols; a longer proxy's first six
+      // only ask for the automata more often, never less
+      bool literal = len >= 1 && len <= (proxy ? 7 : 6);
+      const int n = std::min(len, 6);
+      uint32_t v = 0;
+      for (int i = 0; i < n && literal; ++i) {
+        literal = __builtin_popcount(sets[i]) == 1;
+        v |= (uint32_t)__builtin_ctz(sets[i] | (1u << 31)) << (27 - 5 * i);
+      }
+      for (int i = n; i < len && literal; ++i) literal = __builtin_popcount(sets[i]) == 1;
+      if (literal) {
+        Lit l{v, ~0u << (32 - 5 * n), 0u};
+        for (int j = 44; j < 49; ++j)
+          if (j + len <= 49) l.info |= 1u << (j - 44);
+        if (anchor == kStart) l.info |= kLitStart;
+        if (proxy) l.info |= kLitProxy;
+        lits[k].push_back(l);
+      } else {
+        walks[k] = true;
+      }
     }
-    ++placed[k];
-    lit_val[k] = v;
-    lit_len[k] = literal ? (uint32_t)len : 0u;
     for (int i = 0; i < kept; ++i)
       for (int c = 0; c < 32; ++c)
         if ((sets[i] >> c) & 1u) b.b[c] |= 1u << (off + i);
@@ -105,14 +123,16 @@ inline std::string build_banks(const FilterSet &fs, std::vector<ScanBank> &out,
       b.uall |= b.b[c];
     }
     b.u54 = b.b[0] | b.b[8] | b.b[16] | b.b[24];  // a, i, q, y
-    if (placed[k] == 1 && lit_len[k]) {
-      // one literal: compared at each placement, about 3 instructions a
-      // placement where the automaton takes about 6 a symbol
-      b.lit = 1;
-      b.lit_val = lit_val[k];
-      b.lit_mask = ~0u << (32 - 5 * lit_len[k]);
-      for (uint32_t j = 44; j < 49; ++j)
-        if (j + lit_len[k] <= 49) b.lit_tail |= 1u << (j - 44);
+    if (!walks[k] && !lits[k].empty() && lits[k].size() <= 2) {
+      // literals compared at each placement, about 3 instructions a
+      // placement each (one for a start-anchored one) where the automaton
+      // takes about 6 a symbol
+      b.lit = (uint32_t)lits[k].size();
+      for (size_t i = 0; i < lits[k].size(); ++i) {
+        b.lit_val[i] = lits[k][i].val;
+        b.lit_mask[i] = lits[k][i].mask;
+        b.lit_info[i] = lits[k][i].info;
+      }
     }
   }
   return "";

@@ -386,7 +386,7 @@ size_t og_filters_describe(const og_filterset *s, char *out, size_t cap) {
     for (const ScanBank &b : h.banks) lits += b.lit ? 1 : 0;
     if (lits)
       t += "literal compare: " + std::to_string(lits) +
-           " bank(s) of one literal, compared at each placement in the key pass\n";
+           " bank(s) of at most two literals, compared at their placements in the key pass\n";
   }
   if (h.key_proxy)
     t += "key tables: " + std::to_string(h.proxy_set.fast.size()) +

@@ -36,7 +36,7 @@ namespace og {
 
 constexpr int kBankBits = 32;
 constexpr int kMaxBanks = 8;
-constexpr int kBankWords = 48;
+constexpr int kBankWords = 52;
 
 // One bank, 48 words (the device copy is these words back to back).
 struct ScanBank {
@@ -51,15 +51,18 @@ struct ScanBank {
   uint32_t u54;     // a, i, q or y
   uint32_t jstart;  // no pattern of the bank starts before this symbol
   uint32_t fmaybe;  // last bits of expression proxies
-  // A bank that holds one unanchored literal of 1..6 symbols: its key pass
-  // compares the literal at each placement inside symbols 0..48 (lit_val, the
-  // literal's bits from bit 31 down, under lit_mask; bit j - 44 of lit_tail
-  // set where placement j in 44..48 ends by symbol 48) instead of walking
-  // every symbol.
+  // A bank whose patterns that may start inside the key are at most two
+  // literals (the others end the address): its key pass compares them at
+  // their placements inside symbols 0..48 instead of walking every symbol.
+  // lit is their count; per literal, its first symbols' bits from bit 31 down
+  // (lit_val) under lit_mask, and lit_info: bit j - 44 set where placement j
+  // in 44..48 ends the literal by symbol 48, kLitStart (placement 0 only) and
+  // kLitProxy (an expression's proxy: a match asks for its automata).
   uint32_t lit;
-  uint32_t lit_val, lit_mask, lit_tail;
-  uint32_t pad[1];
+  uint32_t lit_val[2], lit_mask[2], lit_info[2];
+  uint32_t pad[2];
 };
+constexpr uint32_t kLitStart = 1u << 8, kLitProxy = 1u << 9;
 static_assert(sizeof(ScanBank) == kBankWords * 4, "bank layout");
 
 // Symbol j of the address bit string held as big-endian words (eight for the
@@ -126,9 +129,20 @@ OG_HD void scan_bank_key(const ScanBank &s, const uint32_t w[8], bool &hit, bool
   const uint32_t js = s.jstart, iany = s.iany, fm = s.fmaybe;
   uint32_t D = 0, acc = 0;
   if (s.lit) {
-    hit |= literal_min_from<0>(w, s.lit_val, s.lit_mask, s.lit_tail, ~0u) == 0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      if ((uint32_t)p < s.lit) {
+        const uint32_t info = s.lit_info[p];
+        const uint32_t m =
+            (info & kLitStart) ? (key_bits32<0, 8>(w) ^ s.lit_val[p]) & s.lit_mask[p]
+                               : literal_min_from<0>(w, s.lit_val[p], s.lit_mask[p], info, ~0u);
+        if (info & kLitProxy) maybe |= m == 0;
+        else hit |= m == 0;
+      }
+    }
     // D after symbol 48 from the partial matches a placement from 43 on
-    // leaves (one of at most 6 symbols reaching past 48 starts at 44 or later)
+    // leaves (one of at most 7 symbols reaching past 48 starts at 43 or
+    // later; a start-anchored one never does)
 #pragma unroll
     for (int j = 43; j < 49; ++j) D = ((D << 1) | iany) & s.b[sym_at(w, j)];
   } else if (js < 49) {

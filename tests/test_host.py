@@ -653,22 +653,26 @@ def test_exact_table_matches_the_sets_own_matcher():
         assert hits >= min(600, len(words))
 
 
-def test_one_literal_bank_agrees_with_oracle_at_every_placement():
-    """A bank of one literal of 1..6 symbols runs the key pass as a compare at
-    each placement (og_scan.hpp literal_hits_from) and the tail from the last
-    six symbols' automaton: on keys with the literal planted at every placement
-    0..55, with its last symbol changed, and at random, the host run of the
-    kernels' matcher equals the oracle's whole-address match, and a hit the key
-    pass leaves unsettled is flagged for the address stage."""
-    rng = random.Random(77)
+@pytest.mark.parametrize("count", [1, 2])
+def test_literal_bank_agrees_with_oracle_at_every_placement(count):
+    """A bank of one or two literals of 1..6 symbols runs the key pass as a
+    compare at each placement (og_scan.hpp literal_min_from) and the tail from
+    the last six symbols' automaton: on keys with a literal planted at every
+    placement 0..55, with its last symbol changed, and at random, the host run
+    of the kernels' matcher equals the oracle's whole-address match, and a hit
+    the key pass leaves unsettled is flagged for the address stage."""
+    rng = random.Random(77 + count)
     for L in range(1, 7):
         for trial in range(3):
             pk0 = bytes(rng.getrandbits(8) for _ in range(32))
             text = _address_text(pk0)
             lit = "".join(rng.choice(workload.ALPHABET) for _ in range(L))
             lines = ["contains:" + lit]
+            if count == 2:
+                other = "".join(rng.choice(workload.ALPHABET) for _ in range(rng.randint(2, 6)))
+                lines.append("contains:" + other)
             mine, theirs = filters.FilterSet.parse(lines), OracleFilters(lines)
-            assert any(l.startswith("literal compare: 1") for l in mine.describe())
+            assert any(l.startswith("literal compare: 1") for l in mine.describe()), lines
             keys = [bytes(rng.getrandbits(8) for _ in range(32)) for _ in range(40)]
             for s in range(0, 56 - L + 1):
                 for word in (lit, lit[:-1] + rng.choice([c for c in workload.ALPHABET if c != lit[-1]])):
@@ -685,10 +689,10 @@ def test_one_literal_bank_agrees_with_oracle_at_every_placement():
                 clear = pk[:31] + bytes([pk[31] & 0x7F])
                 kh, need, hit = mine.scan(clear, sign)
                 want = theirs.match_address(key.address_bytes(pk)) >= 0
-                assert hit == want, (lit, pk.hex())
+                assert hit == want, (lines, pk.hex())
                 if kh:
                     assert hit
                 if hit and not kh:
-                    assert need, (lit, pk.hex())
+                    assert need, (lines, pk.hex())
                 hits += hit
             assert hits >= 56 - L - 5
