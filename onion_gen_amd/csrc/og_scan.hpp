@@ -100,12 +100,12 @@ OG_HD uint32_t key_bits32(const uint32_t *w) {
   }
 }
 
-// The key pass of a one-literal bank: the least masked difference between the
-// literal and the key at placements J..48, so 0 where the literal sits (every
-// placement up to 43 fits a literal of at most 6 symbols; placements 44..48
-// count where bit J - 44 of `tail` says the literal ends by symbol 48).  Three
-// instructions a placement (a funnel shift, the masked difference, a minimum),
-// with no table read and no branch.
+// The key pass of a literal bank, one literal: the least masked difference
+// between the literal and the key at placements J..48 (m the least so far), so
+// 0 where the literal sits.  Every placement up to 43 fits a literal of at
+// most 6 symbols; placements 44..48 count where bit J - 44 of `tail` says the
+// literal ends by symbol 48.  Three instructions a placement (a funnel shift,
+// the masked difference, a minimum), with no table read and no branch.
 template <int J>
 OG_HD uint32_t literal_min_from(const uint32_t w[8], uint32_t val, uint32_t mask, uint32_t tail,
                                 uint32_t m) {
