@@ -222,6 +222,22 @@ def test_dictionary_bitmap_and_exact_table(index_bits):
     assert got == want and len(want) > 0
 
 
+def test_dictionary_of_mixed_lengths():
+    """A dictionary whose prefixes run from 1 to 13 symbols: the device's 24-bit
+    map holds runs for the short ones, and the exact table one bucket and
+    directory per length (the 13-symbol one compared on its first 64 bits)."""
+    say("case: dictionary of 1- to 13-symbol prefixes, n=2^18")
+    rng = random.Random(31)
+    words = ["a", "2x", "q7z", "bcdefghijklmn"]
+    words += ["".join(rng.choice(workload.ALPHABET) for _ in range(rng.randint(4, 9)))
+              for _ in range(300)]
+    start = key.start_point(workload.chacha_stream(5, 32))
+    n = 1 << 18
+    got = run_engine(start, filters.FilterSet.parse(words), n, threads=1024, half=128)
+    want = oracle_hits(start, words, n)
+    assert got == want and len(want) > 8000
+
+
 def test_reference_constructor_bitmap_only():
     """Engine::new with the bitmap alone over-reports but never under-reports."""
     say("case: bitmap-only engine, 4 prefixes, n=2^20")
