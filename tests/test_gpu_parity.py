@@ -238,6 +238,22 @@ def test_dictionary_of_mixed_lengths():
     assert got == want and len(want) > 8000
 
 
+@pytest.mark.parametrize("lines", [["contains:q2x", "contains:7ab"], ["contains:zz"],
+                                   ["contains:q2x", "suffix:aqd"]])
+def test_literal_banks_match_oracle(lines):
+    """Sets whose key pass compares literals at each placement (one or two
+    substrings, and a substring beside a suffix that only the tail sees)
+    report exactly the oracle's hits."""
+    say(f"case: literal bank {lines}, n=2^18")
+    fs = filters.FilterSet.parse(lines)
+    assert any(l.startswith("literal compare: 1") for l in fs.describe())
+    start = key.start_point(hashlib.sha256(" ".join(lines).encode()).digest())
+    n = 1 << 18
+    got = run_engine(start, fs, n, threads=1024, half=128)
+    want = oracle_hits(start, lines, n)
+    assert got == want and len(want) > 100
+
+
 def test_reference_constructor_bitmap_only():
     """Engine::new with the bitmap alone over-reports but never under-reports."""
     say("case: bitmap-only engine, 4 prefixes, n=2^20")
