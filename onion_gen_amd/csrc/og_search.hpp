@@ -76,7 +76,7 @@ __device__ __forceinline__ bool bitmap_bit(const DevFilterArgs &f, uint32_t head
 // The cheap test every candidate gets: on the low bits of the key (its first
 // three bytes), taken from three columns of its product (fe10_mul_low26).
 // Never false for a candidate the exact test accepts: where that word is not
-// settled the probe passes.
+// settled the probe tests each value the canonical word can take.
 // A MODE with both key bits set leaves the key form to the filter at run time
 // (f.mode, wave-uniform): one instantiation serves every key form, which keeps
 // the slot engine's kernels to one search loop each.
@@ -111,26 +111,43 @@ __device__ __forceinline__ uint32_t key_probe_word(const Fe10 &a, const Fe10 &b,
   return lo;
 }
 
+// Whether the map holds the slot of probe word x: the workgroup's LDS fold
+// first, then the map in L2 for the lanes whose fold bit is set (the others
+// read word 0, one line for all of them).
+template <uint32_t MODE>
+__device__ __forceinline__ uint32_t map_bit(uint32_t x, const DevFilterArgs &f,
+                                            const uint32_t *coarse) {
+  const uint32_t h = bswap32(x);
+  if (!coarse) return (uint32_t)bitmap_bit(f, h);
+  const uint32_t c = (uint32_t)coarse_bit(coarse, f.coarse_bits, h);
+  if (f.coarse_bits == f.bits) return c;
+  const uint32_t slot = (uint32_t)((uint64_t)h >> (32 - f.bits));
+  const uint32_t word = c ? slot >> 5 : 0u;
+  return c & (f.bitmap[word] >> (slot & 31)) & 1u;
+}
+
+// Where the probe word is inexact the canonical word is lo, lo + 19 or
+// lo + 38, so those lanes test all three, in the waves that hold such a lane
+// (a wave-uniform branch, taken for about one candidate slot in 16): only a
+// lane where one of them passes goes on to the whole product.
 template <uint32_t MODE>
 __device__ __forceinline__ bool maybe_key(const Fe10 &a, const Fe10 &b, const DevFilterArgs &f,
                                           const uint32_t *coarse = nullptr) {
   bool exact;
   const uint32_t lo = key_probe_word<MODE>(a, b, f, exact);
-  // a prefix compares its first three bytes here; settle_key compares the rest
-  if (key_is_prefix<MODE>(f)) return !exact || ((lo ^ f.want[0]) & f.mask[0] & kLow26) == 0;
+  if (key_is_prefix<MODE>(f)) {
+    // a prefix compares its first three bytes here; settle_key compares the rest
+    const uint32_t m = f.mask[0] & kLow26, want = f.want[0];
+    bool pass = ((lo ^ want) & m) == 0;
+    if (__ballot(!exact))
+      pass |= !exact && ((((lo + 19u) ^ want) & m) == 0 || (((lo + 38u) ^ want) & m) == 0);
+    return pass;
+  }
   if (key_is_bitmap<MODE>(f)) {
-    const uint32_t h = bswap32(lo), inexact = exact ? 0u : 1u;
-    if (!coarse) return (inexact | (uint32_t)bitmap_bit(f, h)) != 0;
-    // bitwise, not short-circuit: no branch stands between this candidate's
-    // reads and the next one's, so the two are in flight together
-    const uint32_t c = inexact | (uint32_t)coarse_bit(coarse, f.coarse_bits, h);
-    if (f.coarse_bits == f.bits) return c != 0;
-    // the map in L2 for the lanes whose coarse bit is set; the others read
-    // word 0, one line for all of them
-    const uint32_t slot = (uint32_t)((uint64_t)h >> (32 - f.bits));
-    const uint32_t word = c ? slot >> 5 : 0u;
-    const uint32_t bit = (f.bitmap[word] >> (slot & 31)) & 1u;
-    return (c & (bit | inexact)) != 0;
+    uint32_t pass = map_bit<MODE>(lo, f, coarse);
+    if (__ballot(!exact) && !exact)
+      pass |= map_bit<MODE>(lo + 19u, f, coarse) | map_bit<MODE>(lo + 38u, f, coarse);
+    return pass != 0;
   }
   return false;
 }
@@ -319,17 +336,24 @@ __device__ __forceinline__ void test_key_pair(const Fe10 &am, const Fe10 &ap, co
   bool mm, mp;
   if (key_is_bitmap<MODE>(f) && coarse && f.coarse_bits != f.bits) {
     bool em, ep;
-    const uint32_t hm = bswap32(key_probe_word<MODE>(am, b, f, em));
-    const uint32_t hp = bswap32(key_probe_word<MODE>(ap, b, f, ep));
+    const uint32_t lm = key_probe_word<MODE>(am, b, f, em);
+    const uint32_t lp = key_probe_word<MODE>(ap, b, f, ep);
+    const uint32_t hm = bswap32(lm), hp = bswap32(lp);
     // every lane reads; a lane whose coarse bit is clear reads word 0 of the
     // map in L2, one line for all of them
-    const uint32_t cm = (em ? 0u : 1u) | (uint32_t)coarse_bit(coarse, f.coarse_bits, hm);
-    const uint32_t cp = (ep ? 0u : 1u) | (uint32_t)coarse_bit(coarse, f.coarse_bits, hp);
+    const uint32_t cm = (uint32_t)coarse_bit(coarse, f.coarse_bits, hm);
+    const uint32_t cp = (uint32_t)coarse_bit(coarse, f.coarse_bits, hp);
     const uint32_t sm = (uint32_t)((uint64_t)hm >> (32 - f.bits));
     const uint32_t sp = (uint32_t)((uint64_t)hp >> (32 - f.bits));
     const uint32_t wm = f.bitmap[cm ? sm >> 5 : 0u], wp = f.bitmap[cp ? sp >> 5 : 0u];
-    mm = (cm & (((wm >> (sm & 31)) & 1u) | (em ? 0u : 1u))) != 0;
-    mp = (cp & (((wp >> (sp & 31)) & 1u) | (ep ? 0u : 1u))) != 0;
+    uint32_t pm = cm & (wm >> (sm & 31)) & 1u, pp = cp & (wp >> (sp & 31)) & 1u;
+    // inexact words: the canonical word is also possibly lo + 19 or lo + 38
+    if (__ballot(!em || !ep)) {
+      if (!em) pm |= map_bit<MODE>(lm + 19u, f, coarse) | map_bit<MODE>(lm + 38u, f, coarse);
+      if (!ep) pp |= map_bit<MODE>(lp + 19u, f, coarse) | map_bit<MODE>(lp + 38u, f, coarse);
+    }
+    mm = pm != 0;
+    mp = pp != 0;
   } else {
     mm = maybe_key<MODE>(am, b, f, coarse);
     mp = maybe_key<MODE>(ap, b, f, coarse);
